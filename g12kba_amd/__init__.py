@@ -1,0 +1,421 @@
+"""MI355X-native HF-GKBA G1-G2 (GW) propagator — Python host mirror.
+
+Thin ctypes layer over ``libg12kba.so`` (the C ABI declared in
+``include/g12kba.h``).  The functions keep the reference routine names and
+argument roles (G12KBA-GW/src/headers/*.h) so that tests read like the
+reference's own call sites; arrays are numpy complex128 in the reference
+layouts (G1: N x N; G2: N^2 x N^2 with the pair index
+A = i + j*Ns + (a + b*Nb)*Ns^2).
+
+The library is required: importing this package without a built
+``g12kba_amd/lib/libg12kba.so`` raises, and there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libg12kba.so")
+
+G12_OK, G12_GUARD = 0, 1
+QUENCH_FULL, QUENCH_PULSE = 0, 1
+VLAW_EXP, VLAW_COULOMB, VLAW_EXPLICIT = 0, 1, 2
+PREC_FP64, PREC_FP32_3M, PREC_BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": PREC_FP64, "fp32_3m": PREC_FP32_3M, "bf16x3": PREC_BF16X3}
+MAX_NB, MAX_SHELLS = 8, 128
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class G12Error(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    """Mirror of ``g12_params`` (include/g12kba.h)."""
+    _fields_ = [
+        ("Ns", ctypes.c_int), ("Nb", ctypes.c_int),
+        ("t1", ctypes.c_double), ("t2", ctypes.c_double), ("t3", ctypes.c_double),
+        ("U", ctypes.c_double), ("time_steps", ctypes.c_int), ("dt", ctypes.c_double),
+        ("quench_strength", ctypes.c_double), ("Nq", ctypes.c_int), ("q_type", ctypes.c_int),
+        ("HF", ctypes.c_int), ("EHM", ctypes.c_int),
+        ("AS_rate", ctypes.c_double), ("AS_midpoint", ctypes.c_double),
+        ("quench_rate", ctypes.c_double), ("quench_on_midpoint", ctypes.c_double),
+        ("quench_off_midpoint", ctypes.c_double), ("decay_rate", ctypes.c_double),
+        ("V_law", ctypes.c_int), ("nV_profile", ctypes.c_int),
+        ("V_profile", ctypes.c_double * MAX_SHELLS), ("eps", ctypes.c_double * MAX_NB),
+        ("AS_dt", ctypes.c_double), ("AS_tail", ctypes.c_int),
+        ("quench_gate_time", ctypes.c_double), ("p_number", ctypes.c_int),
+        ("precision", ctypes.c_int),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+            "(make -C g12kba_amd/csrc); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    i, d, p = ctypes.c_int, ctypes.c_double, _dp
+    i64, pi64 = ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)
+    vp = ctypes.c_void_p
+    sig = {
+        "g12_params_default": [ctypes.POINTER(Params)],
+        "g12_parse_input": [ctypes.c_char_p, ctypes.POINTER(Params)],
+        "g12_make_g0": [ctypes.POINTER(Params), p, i],
+        "g12_create": [ctypes.POINTER(Params), ctypes.POINTER(vp)],
+        "g12_destroy": [vp],
+        "g12_set_state": [vp, p, p, p],
+        "g12_get_state": [vp, p, p, p],
+        "g12_adiabatic_switching": [vp, ctypes.POINTER(i)],
+        "g12_rk4": [vp, p, i64, pi64],
+        "g12_rk4_advance": [vp, i64, pi64],
+        "g12_get_time": [vp, p, pi64],
+        "g12_timing_enable": [vp, i],
+        "g12_timing_get": [vp, i, pi64, p],
+        "g12_memory_bytes": [vp, pi64],
+        "g12_hf_hamiltonian": [p, p, p, i, i, d, d, d, d, p, p, i, i],
+        "g12_h1_commutator": [p, p, p, i],
+        "g12_h2_commutator": [p, p, p, p, i, i, i],
+        "g12_collision": [p, p, p, i, i, d, p, i],
+        "g12_psi_xxxx": [p, p, i, i, d, p, i],
+        "g12_psi_xyxy": [p, p, i, i, d, p, i],
+        "g12_pi": [p, p, p, p, d, i, i, p, i],
+        "g12_g1_step": [p, p, p, p, d, i, i],
+        "g12_g2_step": [p, p, p, p, p, d, i, i],
+        "g12_dipole": [p, p, i64, i],
+        "g12_write_outputs": [ctypes.c_char_p, p, i64, i, p, d],
+        "g12_save_state": [vp, ctypes.c_char_p],
+        "g12_load_state": [vp, ctypes.c_char_p],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.g12_fermi.argtypes = [d, d, d]
+    L.g12_fermi.restype = d
+    L.g12_version.restype = ctypes.c_char_p
+    return L
+
+
+lib = _load()
+
+# symbols declared in include/g12kba.h (checked by the CPU test-suite)
+EXPORTED = [
+    "g12_params_default", "g12_parse_input", "g12_make_g0", "g12_create", "g12_destroy",
+    "g12_set_state", "g12_get_state", "g12_adiabatic_switching", "g12_rk4", "g12_rk4_advance",
+    "g12_get_time", "g12_timing_enable", "g12_timing_get", "g12_memory_bytes",
+    "g12_hf_hamiltonian", "g12_h1_commutator", "g12_h2_commutator", "g12_collision",
+    "g12_psi_xxxx", "g12_psi_xyxy", "g12_pi", "g12_g1_step", "g12_g2_step", "g12_fermi",
+    "g12_dipole", "g12_write_outputs", "g12_save_state", "g12_load_state", "g12_version",
+]
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise G12Error(f"{what} failed with code {rc}")
+    return rc
+
+
+def _c(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    if shape is not None and a.size != int(np.prod(shape)):
+        raise ValueError(f"expected {shape} elements, got {a.shape}")
+    return a
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _vec(v):
+    v = np.ascontiguousarray([] if v is None else v, dtype=np.float64)
+    return v, (_p(v) if v.size else None)
+
+
+# ---------------------------------------------------------------------------
+# parameters
+
+
+def default_params():
+    p = Params()
+    lib.g12_params_default(ctypes.byref(p))
+    return p
+
+
+def make_params(**kw):
+    """Params from keyword overrides of the compiled defaults.
+
+    ``q_type`` accepts "full"/"pulse"/"none"; ``V_law`` accepts
+    "exp"/"coulomb"/"explicit"; ``precision`` accepts "fp64"/"fp32_3m"/"bf16x3".
+    """
+    p = default_params()
+    q = kw.pop("q_type", None)
+    if q is not None:
+        if q == "none":
+            p.quench_strength, p.Nq = 0.0, 0
+            kw.pop("quench_strength", None)
+            kw.pop("Nq", None)
+        p.q_type = QUENCH_PULSE if q == "pulse" else QUENCH_FULL
+    law = kw.pop("V_law", None)
+    if law is not None:
+        p.V_law = {"exp": VLAW_EXP, "coulomb": VLAW_COULOMB, "explicit": VLAW_EXPLICIT}.get(law, law)
+    prof = kw.pop("V_profile", None)
+    if prof is not None:
+        prof = list(prof)
+        p.nV_profile = len(prof)
+        for k, v in enumerate(prof):
+            p.V_profile[k] = v
+    eps = kw.pop("eps", None)
+    if eps is not None:
+        for k, v in enumerate(eps):
+            p.eps[k] = v
+    prec = kw.pop("precision", None)
+    if prec is not None:
+        p.precision = PRECISIONS.get(prec, prec)
+    for k, v in kw.items():
+        if k in ("HF", "EHM"):
+            v = int(bool(v))
+        setattr(p, k, v)
+    return p
+
+
+def parse_input(path):
+    p = default_params()
+    _check(lib.g12_parse_input(path.encode(), ctypes.byref(p)), "g12_parse_input")
+    return p
+
+
+def v_profile(p):
+    """The V_n / U profile the library uses for parameters p (n = 1..)."""
+    if not p.EHM:
+        return np.zeros(0)
+    n = min(p.Ns - 1, MAX_SHELLS)
+    if p.V_law == VLAW_EXPLICIT:
+        return np.array([p.V_profile[k] for k in range(min(n, p.nV_profile))])
+    d = np.arange(1, n + 1, dtype=np.float64)
+    if p.V_law == VLAW_COULOMB:
+        return 1.0 / (1.0 + d)
+    return np.exp(-p.decay_rate * d)
+
+
+def make_G0(p, fp32_emulation=False):
+    """Non-interacting initial G1 (make_G0, init_G.h:57-75)."""
+    N = p.Ns * p.Nb
+    G1 = np.zeros((N, N), np.complex128)
+    _check(lib.g12_make_g0(ctypes.byref(p), _p(G1), int(fp32_emulation)), "g12_make_g0")
+    return G1
+
+
+# ---------------------------------------------------------------------------
+# routine-level entry points (reference names, arguments and roles)
+
+
+def h_HFx_quench(h, G1x, G1y, Ns, Nb, U, t1, t2, w, epsilon, V, quench_extent):
+    """h += h_HF[G1x]  (HF.h:57-104); returns the updated h."""
+    N = Ns * Nb
+    h = _c(h, (N, N)).copy()
+    eps, eps_p = _vec(epsilon)
+    Vv, Vp = _vec(V)
+    _check(lib.g12_hf_hamiltonian(_p(h), _p(_c(G1x, (N, N))), None, Ns, Nb, U, t1, t2, w, eps_p, Vp,
+                                  Vv.size, quench_extent), "g12_hf_hamiltonian")
+    return h
+
+
+def hG_commutator(G, h):
+    """[h, G] = hG - Gh  (commutators.h:28-51)."""
+    G = _c(G)
+    N = G.shape[0]
+    out = np.zeros((N, N), np.complex128)
+    _check(lib.g12_h1_commutator(_p(G), _p(_c(h, (N, N))), _p(out), N), "g12_h1_commutator")
+    return out
+
+
+def h2xyG2xyxy_commutator(G2, h1x, h1y, Ns, Nb, precision="fp64"):
+    """[H2, G2] with H2 = h(x)1 + 1(x)h  (commutators.h:53-99)."""
+    N = Ns * Nb
+    G2 = _c(G2, (N * N, N * N))
+    out = np.zeros((N * N, N * N), np.complex128)
+    _check(lib.g12_h2_commutator(_p(G2), _p(_c(h1x, (N, N))), _p(_c(h1y, (N, N))), _p(out), Ns, Nb,
+                                 PRECISIONS.get(precision, precision)), "g12_h2_commutator")
+    return out
+
+
+def collision_int(G2xyxy, G2xxxx, Ns, Nb, U, V):
+    """Collision integral I  (collision_integral.h:27-69)."""
+    N = Ns * Nb
+    out = np.zeros((N, N), np.complex128)
+    Vv, Vp = _vec(V)
+    _check(lib.g12_collision(_p(out), _p(_c(G2xyxy, (N ** 2, N ** 2))), _p(_c(G2xxxx, (N ** 2, N ** 2))),
+                             Ns, Nb, U, Vp, Vv.size), "g12_collision")
+    return out
+
+
+def Psi_xxxx(G1, Ns, Nb, U, V):
+    N = Ns * Nb
+    out = np.zeros((N * N, N * N), np.complex128)
+    Vv, Vp = _vec(V)
+    _check(lib.g12_psi_xxxx(_p(out), _p(_c(G1, (N, N))), Ns, Nb, U, Vp, Vv.size), "g12_psi_xxxx")
+    return out
+
+
+def Psi_xyxy(G1, Ns, Nb, U, V):
+    N = Ns * Nb
+    out = np.zeros((N * N, N * N), np.complex128)
+    Vv, Vp = _vec(V)
+    _check(lib.g12_psi_xyxy(_p(out), _p(_c(G1, (N, N))), Ns, Nb, U, Vp, Vv.size), "g12_psi_xyxy")
+    return out
+
+
+def Pi_xyxy(G1y, G2xyxy, G2xxxx, U, Ns, Nb, V):
+    """GW polarisation term  (pi.h:25-95)."""
+    N = Ns * Nb
+    out = np.zeros((N * N, N * N), np.complex128)
+    Vv, Vp = _vec(V)
+    _check(lib.g12_pi(_p(out), _p(_c(G1y, (N, N))), _p(_c(G2xyxy, (N ** 2, N ** 2))),
+                      _p(_c(G2xxxx, (N ** 2, N ** 2))), U, Ns, Nb, Vp, Vv.size), "g12_pi")
+    return out
+
+
+def G1_step(comm, I, K, RK_result, w, Ns, Nb):
+    """Returns (K, RK_result + w K)  (G1_G2_t_step.h:24-47)."""
+    N = Ns * Nb
+    K = _c(K, (N, N)).copy()
+    RK = _c(RK_result, (N, N)).copy()
+    _check(lib.g12_g1_step(_p(_c(comm, (N, N))), _p(_c(I, (N, N))), _p(K), _p(RK), w, Ns, Nb), "g12_g1_step")
+    return K, RK
+
+
+def G2_step(comm, psi, pi, K, RK_result, w, Ns, Nb):
+    """Returns (K, RK_result + w K)  (G1_G2_t_step.h:55-86)."""
+    n = (Ns * Nb) ** 2
+    K = _c(K, (n, n)).copy()
+    RK = _c(RK_result, (n, n)).copy()
+    _check(lib.g12_g2_step(_p(_c(comm, (n, n))), _p(_c(psi, (n, n))), _p(_c(pi, (n, n))), _p(K), _p(RK), w,
+                           Ns, Nb), "g12_g2_step")
+    return K, RK
+
+
+def fermi(t, t0, tau):
+    return lib.g12_fermi(t, t0, tau)
+
+
+def dipole(traj):
+    traj = _c(traj)
+    T, N = traj.shape[0], traj.shape[1]
+    p = np.zeros(T)
+    _check(lib.g12_dipole(_p(p), _p(traj), T, N), "g12_dipole")
+    return p
+
+
+def write_outputs(directory, traj, dip, progress=-1.0):
+    traj = _c(traj)
+    dip = np.ascontiguousarray(dip, dtype=np.float64)
+    _check(lib.g12_write_outputs(directory.encode(), _p(traj), traj.shape[0], traj.shape[1] if traj.ndim == 3 else 0,
+                                 _p(dip), progress), "g12_write_outputs")
+
+
+# ---------------------------------------------------------------------------
+# device-resident propagation
+
+
+class Propagator:
+    """One trajectory on the GPU: Adiabatic_switching / RK4 (AS.h, RK_solver.h)."""
+
+    def __init__(self, params):
+        self.p = params
+        self.N = params.Ns * params.Nb
+        h = ctypes.c_void_p()
+        _check(lib.g12_create(ctypes.byref(params), ctypes.byref(h)), "g12_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.g12_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, G1, G2xx=None, G2xy=None):
+        N = self.N
+        G1 = _c(G1, (N, N))
+        Gxx = None if G2xx is None else _c(G2xx, (N ** 2, N ** 2))
+        Gxy = None if G2xy is None else _c(G2xy, (N ** 2, N ** 2))
+        _check(lib.g12_set_state(self.h, _p(G1), _p(Gxx), _p(Gxy)), "g12_set_state")
+
+    def get_state(self):
+        N = self.N
+        G1 = np.zeros((N, N), np.complex128)
+        Gxx = np.zeros((N * N, N * N), np.complex128)
+        Gxy = np.zeros((N * N, N * N), np.complex128)
+        _check(lib.g12_get_state(self.h, _p(G1), _p(Gxx), _p(Gxy)), "g12_get_state")
+        return G1, Gxx, Gxy
+
+    def get_G1(self):
+        N = self.N
+        G1 = np.zeros((N, N), np.complex128)
+        _check(lib.g12_get_state(self.h, _p(G1), None, None), "g12_get_state")
+        return G1
+
+    def adiabatic_switching(self):
+        n = ctypes.c_int(0)
+        _check(lib.g12_adiabatic_switching(self.h, ctypes.byref(n)), "g12_adiabatic_switching")
+        return n.value
+
+    def rk4(self, traj_cap=None):
+        """Returns (rc, trajectory[steps, N, N])."""
+        if traj_cap is None:
+            traj_cap = self.p.time_steps + 2
+        traj = np.zeros((traj_cap, self.N, self.N), np.complex128)
+        n = ctypes.c_int64(0)
+        rc = _check(lib.g12_rk4(self.h, _p(traj), traj_cap, ctypes.byref(n)), "g12_rk4")
+        return rc, traj[: min(n.value, traj_cap)]
+
+    def advance(self, nsteps):
+        n = ctypes.c_int64(0)
+        return _check(lib.g12_rk4_advance(self.h, nsteps, ctypes.byref(n)), "g12_rk4_advance")
+
+    def time(self):
+        t = ctypes.c_double(0)
+        n = ctypes.c_int64(0)
+        lib.g12_get_time(self.h, ctypes.byref(t), ctypes.byref(n))
+        return t.value, n.value
+
+    def timing(self, on=True):
+        lib.g12_timing_enable(self.h, int(on))
+
+    def timing_get(self, which):
+        n = ctypes.c_int64(0)
+        ms = ctypes.c_double(0)
+        _check(lib.g12_timing_get(self.h, which, ctypes.byref(n), ctypes.byref(ms)), "g12_timing_get")
+        return n.value, ms.value
+
+    def memory_bytes(self):
+        b = ctypes.c_int64(0)
+        lib.g12_memory_bytes(self.h, ctypes.byref(b))
+        return b.value
+
+    def save(self, path):
+        _check(lib.g12_save_state(self.h, path.encode()), "g12_save_state")
+
+    def load(self, path):
+        _check(lib.g12_load_state(self.h, path.encode()), "g12_load_state")
+
+
+def gpu_available():
+    try:
+        import torch  # noqa: F401  (device enumeration only)
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False

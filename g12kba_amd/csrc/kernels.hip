@@ -1,0 +1,776 @@
+// HIP kernels for the HF-GKBA G1-G2 (GW) right-hand side on gfx950.
+//
+// Notation (all complex numbers are double2 = (re, im)):
+//   N = Ns*Nb, one-particle index p = site + orb*Ns.
+//   G2 tensors are held in the natural layout T[p1][p2][q1][q2]
+//   (= M[(p1,p2);(q1,q2)] of the reference, pair order permuted once at
+//   upload; see launch_ref_to_nat).
+//
+// The two-particle commutator [H2, M] (H2 = h(x)1 + 1(x)h, reference
+// commutators.h:53-99) is evaluated as two passes over N x N tiles, each
+// computing the one-particle commutator  O = h.X - X.h  of a tile X:
+//   strided tiles   X[p1][q1] = T[p1][p2][q1][q2] at fixed (p2,q2)
+//                   -> the terms that act on p1 and q1;
+//   contiguous tiles X[p2][q2] = T[p1][p2][q1][q2] at fixed (p1,q1)
+//                   -> the terms that act on p2 and q2.
+// The contiguous pass also evaluates the rest of the G2 right-hand side
+// (Psi, Pi, Pi-dagger, reference G1_G2_t_step.h:55-86) and the RK4 update for
+// its tile, so Psi and Pi are never stored.
+#include "g12_internal.h"
+
+#include <cmath>
+
+namespace g12 {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// i*z and -i*z
+__device__ __forceinline__ double2 cmul_i(double2 a) { return make_double2(-a.y, a.x); }
+__device__ __forceinline__ double2 cmul_mi(double2 a) { return make_double2(a.y, -a.x); }
+
+__device__ __forceinline__ bool failed(const int* fail) { return fail && *fail != 0; }
+
+// ---------------------------------------------------------------------------
+// Interaction tables W_xy / W_xx (N x N):  U on one site, V_n at distance
+// n <= nV, 0 beyond; W_xx(X,X) = 0.  (The loops of HF.h, psi.h, pi.h and
+// collision_integral.h over r and n enumerate exactly these entries.)
+__device__ inline double w_xy(int x, int z, int Ns, const StageParams& sp) {
+    int d = x % Ns - z % Ns;
+    d = d < 0 ? -d : d;
+    if (d == 0) return sp.U;
+    return d <= sp.nV ? sp.V[d - 1] : 0.0;
+}
+
+// h = h_HF[G] for one element (X, Y) (h_HFx_quench, HF.h:57-104):
+//   diagonal: eps + quench + Hartree  -iU g_XX - 2i sum_{Z != X} W_xy(X,Z) g_ZZ
+//   -t1 between neighbouring sites of one orbital, -t2 between adjacent
+//   orbitals of one site, onsite Fock +iU g_YX (transposed), long-range Fock
+//   +iV g_XY.
+__device__ inline double2 hf_element(const double2* G, int X, int Y, int Ns, int Nb, const StageParams& sp) {
+    const int N = Ns * Nb;
+    const int sx = X % Ns, ox = X / Ns, sy = Y % Ns, oy = Y / Ns;
+    double2 v = make_double2(0.0, 0.0);
+    if (X == Y) {
+        v.x += sp.eps[ox];
+        if (sx < sp.Nq) v.x += sp.wq;
+        const double2 gxx = G[(int64_t)X * N + X];
+        double2 acc = cscale(gxx, sp.U);  // -i U g_XX
+        double2 lr = make_double2(0.0, 0.0);
+        for (int Z = 0; Z < N; ++Z) {
+            if (Z == X) continue;
+            const double w = w_xy(X, Z, Ns, sp);
+            if (w != 0.0) lr = cadd(lr, cscale(G[(int64_t)Z * N + Z], w));
+        }
+        acc = cadd(acc, cscale(lr, 2.0));
+        v = cadd(v, cmul_mi(acc));
+    } else {
+        int ds = sx - sy;
+        ds = ds < 0 ? -ds : ds;
+        if (ox == oy && ds == 1) v.x -= sp.t1;
+        if (sx == sy) {
+            int dorb = ox - oy;
+            dorb = dorb < 0 ? -dorb : dorb;
+            if (dorb == 1) v.x -= sp.t2;
+            v = cadd(v, cmul_i(cscale(G[(int64_t)Y * N + X], sp.U)));
+        } else if (ds <= sp.nV) {
+            v = cadd(v, cmul_i(cscale(G[(int64_t)X * N + Y], sp.V[ds - 1])));
+        }
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Stage set-up for G1 (one workgroup): Y1 = S1 + c K1, h = h_HF[Y1],
+// W tables.
+__global__ __launch_bounds__(1024) void k_stage_g1(DevState d, StageParams sp) {
+    if (failed(d.fail)) return;
+    const int N = d.N;
+    const int n2 = N * N;
+    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+        double2 y = d.S1[e];
+        if (!sp.first) y = cadd(y, cscale(d.K1[e], sp.c));
+        d.Y1[e] = y;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+        const int X = e / N, Y = e % N;
+        d.h[e] = hf_element(d.Y1, X, Y, d.Ns, d.Nb, sp);
+        const double w = w_xy(X, Y, d.Ns, sp);
+        d.Wxy[e] = w;
+        d.Wxx[e] = (X == Y) ? 0.0 : w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Q^s[Z][X][Y] = sum_w Y1[X][w] W^s[Z][w] Y1[w][Y] and Qp^s[X][Y][Z] = Q^s[Z][X][Y]
+// for s = xx (index 0, W_xx) and xy (index 1, W_xy).  One block per (Z, X),
+// threads over Y.
+__global__ __launch_bounds__(64) void k_q_tables(DevState d) {
+    if (failed(d.fail)) return;
+    const int N = d.N;
+    const int Z = blockIdx.x / N, X = blockIdx.x % N;
+    const double2* G = d.Y1;
+    for (int Y = threadIdx.x; Y < N; Y += blockDim.x) {
+        double2 axy = make_double2(0.0, 0.0), axx = make_double2(0.0, 0.0);
+        for (int w = 0; w < N; ++w) {
+            const double wv = d.Wxy[(int64_t)Z * N + w];
+            if (wv == 0.0) continue;
+            const double2 t = cscale(cmul(G[(int64_t)X * N + w], G[(int64_t)w * N + Y]), wv);
+            axy = cadd(axy, t);
+            if (w != Z) axx = cadd(axx, t);
+        }
+        const int64_t q = ((int64_t)Z * N + X) * N + Y;
+        const int64_t qp = ((int64_t)X * N + Y) * N + Z;
+        d.Q[0][q] = axx;
+        d.Q[1][q] = axy;
+        d.Qp[0][qp] = axx;
+        d.Qp[1][qp] = axy;
+    }
+}
+
+// F^xx[X][Y][Z] = -i sum_w ( W_xy[Z][w] D^xy[X][Y][w] + W_xx[Z][w] D^xx[X][Y][w] )
+// F^xy[X][Y][Z] = -i sum_w ( W_xy[Z][w] D^xx[X][Y][w] + W_xx[Z][w] D^xy[X][Y][w] )
+// (Pi_xyxy with the swapped G2 pair, pi.h:25-95 / RK_solver.h:163-164), the
+// transposed copies Ft[Z][Y][X], and I[X][Y] = F^xx[X][Y][X]
+// (collision_int, collision_integral.h:27-69).  One block per (X, Y).
+__global__ __launch_bounds__(64) void k_f_tables(DevState d) {
+    if (failed(d.fail)) return;
+    const int N = d.N;
+    const int X = blockIdx.x / N, Y = blockIdx.x % N;
+    const double2* dxx = d.D[0] + ((int64_t)X * N + Y) * N;
+    const double2* dxy = d.D[1] + ((int64_t)X * N + Y) * N;
+    for (int Z = threadIdx.x; Z < N; Z += blockDim.x) {
+        double2 fxx = make_double2(0.0, 0.0), fxy = make_double2(0.0, 0.0);
+        for (int w = 0; w < N; ++w) {
+            const double a = d.Wxy[(int64_t)Z * N + w];
+            if (a == 0.0) continue;
+            const double b = (w == Z) ? 0.0 : a;
+            const double2 vxx = dxx[w], vxy = dxy[w];
+            fxx = cadd(fxx, cadd(cscale(vxy, a), cscale(vxx, b)));
+            fxy = cadd(fxy, cadd(cscale(vxx, a), cscale(vxy, b)));
+        }
+        fxx = cmul_mi(fxx);
+        fxy = cmul_mi(fxy);
+        const int64_t f = ((int64_t)X * N + Y) * N + Z;
+        const int64_t ft = ((int64_t)Z * N + Y) * N + X;
+        d.F[0][f] = fxx;
+        d.F[1][f] = fxy;
+        d.Ft[0][ft] = fxx;
+        d.Ft[1][ft] = fxy;
+        if (Z == X) d.I[(int64_t)X * N + Y] = fxx;
+    }
+}
+
+// G1 right-hand side (one workgroup): comm = [h, Y1], K = -i(comm + I + I^dag),
+// RK accumulation (G1_step, G1_G2_t_step.h:24-47) and the final update.
+__global__ __launch_bounds__(1024) void k_g1_rhs(DevState d, StageParams sp) {
+    if (failed(d.fail)) return;
+    const int N = d.N;
+    const int n2 = N * N;
+    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+        const int X = e / N, Y = e % N;
+        double2 hg = make_double2(0.0, 0.0), gh = make_double2(0.0, 0.0);
+        for (int z = 0; z < N; ++z) {
+            hg = cadd(hg, cmul(d.h[(int64_t)X * N + z], d.Y1[(int64_t)z * N + Y]));
+            gh = cadd(gh, cmul(d.Y1[(int64_t)X * N + z], d.h[(int64_t)z * N + Y]));
+        }
+        double2 v = csub(hg, gh);
+        if (sp.corr) v = cadd(cadd(v, d.I[e]), cconj(d.I[(int64_t)Y * N + X]));
+        const double2 K = cmul_mi(v);
+        const double2 wK = cscale(K, sp.wk);
+        if (sp.first) {
+            d.A1[e] = wK;
+            d.K1[e] = K;
+        } else if (!sp.last) {
+            d.A1[e] = cadd(d.A1[e], wK);
+            d.K1[e] = K;
+        } else {
+            const double2 A = cadd(d.A1[e], wK);
+            d.S1[e] = cadd(d.S1[e], cscale(cscale(A, sp.dt), 1.0 / 6.0) );
+        }
+    }
+}
+
+// Records G1 before a step and applies the density guard
+// (RK_solver.h:100-110): |2 Im G1[0,0]| > 3 or NaN stops the run after the
+// record; fail = step + 1.
+__global__ void k_record(DevState d, int slot, int64_t step) {
+    if (failed(d.fail)) return;
+    const int n2 = d.N * d.N;
+    double2* dst = d.traj ? d.traj + (int64_t)slot * n2 : nullptr;
+    for (int e = threadIdx.x; e < n2; e += blockDim.x)
+        if (dst) dst[e] = d.S1[e];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double dens = 2 * d.S1[0].y;
+        if (fabs(dens) > 3 || isnan(dens)) *d.fail = (int)(step + 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile commutator on FP64 MFMA.
+//
+// A workgroup of 4 waves holds one N x N tile X (padded to NP, a multiple of
+// 16) and h in LDS as separate re/im planes and computes O = h.X - X.h with
+// v_mfma_f64_16x16x4f64: 4 real MFMAs per complex product.  Output 16x16
+// blocks are dealt round-robin to the 4 waves.  MFMA operand layout
+// (16x16x4): A[m][k] in lane m + 16k, B[k][n] in lane n + 16k; result lane
+// holds D[4*(lane/16) + j][lane%16], j = 0..3.
+template <int NP>
+struct TileLds {
+    static constexpr int P = NP + 1;  // padded pitch (doubles)
+    double xr[NP * P];
+    double xi[NP * P];
+    double hr[NP * P];
+    double hi[NP * P];
+};
+
+template <int NP>
+struct TileShape {
+    static constexpr int NT = NP / 16;
+    static constexpr int TT = NT * NT;
+    static constexpr int PER_WAVE = (TT + 3) / 4;
+};
+
+template <int NP>
+__device__ __forceinline__ void tile_mfma_fp64(const TileLds<NP>& L, int hmode, d4 (&ar)[TileShape<NP>::PER_WAVE],
+                                               d4 (&ai)[TileShape<NP>::PER_WAVE], int wave, int lane) {
+    constexpr int P = TileLds<NP>::P;
+    constexpr int NT = TileShape<NP>::NT, TT = TileShape<NP>::TT, PW = TileShape<NP>::PER_WAVE;
+    const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int u = 0; u < PW; ++u) {
+        ar[u] = d4{0.0, 0.0, 0.0, 0.0};
+        ai[u] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    for (int k0 = 0; k0 < NP; k0 += 4) {
+        const int kk = k0 + kq;
+#pragma unroll
+        for (int u = 0; u < PW; ++u) {
+            const int t = wave + 4 * u;
+            if (t < TT) {
+                const int mm = (t / NT) * 16 + r, nn = (t % NT) * 16 + r;
+                if (hmode & 1) {  // + h.X
+                    const double a_r = L.hr[mm * P + kk], a_i = L.hi[mm * P + kk];
+                    const double b_r = L.xr[kk * P + nn], b_i = L.xi[kk * P + nn];
+                    ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r, b_r, ar[u], 0, 0, 0);
+                    ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_i, b_i, ar[u], 0, 0, 0);
+                    ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r, b_i, ai[u], 0, 0, 0);
+                    ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i, b_r, ai[u], 0, 0, 0);
+                }
+                if (hmode & 2) {  // - X.h
+                    const double a_r = L.xr[mm * P + kk], a_i = L.xi[mm * P + kk];
+                    const double b_r = L.hr[kk * P + nn], b_i = L.hi[kk * P + nn];
+                    ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_r, b_r, ar[u], 0, 0, 0);
+                    ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i, b_i, ar[u], 0, 0, 0);
+                    ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_r, b_i, ai[u], 0, 0, 0);
+                    ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_i, b_r, ai[u], 0, 0, 0);
+                }
+            }
+        }
+    }
+}
+
+struct TileArgs {
+    const double2* S[2];   // state (tile input Y = S + c*Kp)
+    const double2* Kp[2];  // previous stage derivative (may be null)
+    double c;
+    const double2* h;      // N x N
+    int hmode;             // 1: +h.X, 2: -X.h, 3: both
+    int tiling;            // 0: strided (p2,q2) tiles, 1: contiguous (p1,q1) tiles
+    // mode 0: out = (accumulate ? out : 0) + O
+    double2* out[2];
+    int accumulate;
+    double2* D[2];         // optional: D[X][Y][Z] from the diagonal strided tiles
+    // mode 1 (fused right-hand side, contiguous tiling)
+    const double2* R[2];   // strided-pass result
+    double2* Kout[2];
+    double2* A[2];
+    double2* Sout[2];
+    const double2* G1;
+    const double* W[2];    // W_xx, W_xy
+    const double2* Q[2];
+    const double2* Qp[2];
+    const double2* F[2];
+    const double2* Ft[2];
+    double wk, dt;
+    int first, last;
+    int N;
+    const int* fail;
+};
+
+// Maps the linear block index to a tile so that each XCD (blocks are dealt
+// round-robin over the 8 XCDs) works through a contiguous range of tiles:
+// neighbouring strided tiles share cache lines in that XCD's L2.
+__device__ __forceinline__ int tile_of_block(int b, int ntiles) {
+    if ((ntiles & 7) != 0) return b;
+    const int per = ntiles >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+template <int NP, int MODE>
+__global__ __launch_bounds__(256) void k_tile(TileArgs a) {
+    if (failed(a.fail)) return;
+    __shared__ TileLds<NP> L;
+    constexpr int P = TileLds<NP>::P;
+    constexpr int NT = TileShape<NP>::NT, TT = TileShape<NP>::TT, PW = TileShape<NP>::PER_WAVE;
+    const int N = a.N;
+    const int64_t N2 = (int64_t)N * N, N3 = N2 * N;
+    const int s = blockIdx.y;
+    const int tile = tile_of_block(blockIdx.x, N * N);
+    const int t0 = tile / N, t1 = tile % N;
+    // element (m, n) of the tile lives at base + m*sm + n*sn
+    int64_t base, sm, sn;
+    if (a.tiling == 0) {  // strided: fixed (p2, q2) = (t0, t1); m = p1, n = q1
+        base = (int64_t)t0 * N2 + t1;
+        sm = N3;
+        sn = N;
+    } else {  // contiguous: fixed (p1, q1) = (t0, t1); m = p2, n = q2
+        base = (int64_t)t0 * N3 + (int64_t)t1 * N;
+        sm = N2;
+        sn = 1;
+    }
+    const double2* S = a.S[s];
+    const double2* Kp = a.Kp[s];
+    for (int e = threadIdx.x; e < NP * NP; e += 256) {
+        const int m = e / NP, n = e % NP;
+        double2 y = make_double2(0.0, 0.0), hv = make_double2(0.0, 0.0);
+        if (m < N && n < N) {
+            const int64_t off = base + m * sm + n * sn;
+            y = S[off];
+            if (Kp) y = cadd(y, cscale(Kp[off], a.c));
+            hv = a.h[m * N + n];
+        }
+        L.xr[m * P + n] = y.x;
+        L.xi[m * P + n] = y.y;
+        L.hr[m * P + n] = hv.x;
+        L.hi[m * P + n] = hv.y;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    d4 ar[PW], ai[PW];
+    tile_mfma_fp64<NP>(L, a.hmode, ar, ai, wave, lane);
+
+    if (MODE == 0) {
+        if (a.D[s] && a.tiling == 0 && t0 == t1) {
+            // D[X=p1][Y=q1][Z] with Z = p2 = q2
+            for (int e = threadIdx.x; e < N * N; e += 256) {
+                const int m = e / N, n = e % N;
+                a.D[s][((int64_t)m * N + n) * N + t0] = make_double2(L.xr[m * P + n], L.xi[m * P + n]);
+            }
+        }
+        double2* out = a.out[s];
+#pragma unroll
+        for (int u = 0; u < PW; ++u) {
+            const int t = wave + 4 * u;
+            if (t >= TT) continue;
+            const int n = (t % NT) * 16 + (lane & 15);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int m = (t / NT) * 16 + 4 * (lane >> 4) + j;
+                if (m < N && n < N) {
+                    const int64_t off = base + m * sm + n * sn;
+                    double2 v = make_double2(ar[u][j], ai[u][j]);
+                    if (a.accumulate) v = cadd(out[off], v);
+                    out[off] = v;
+                }
+            }
+        }
+    } else {
+        // fused right-hand side for tile (p1, q1) = (t0, t1), element (p2, q2) = (m, n):
+        //   comm = R + O
+        //   Psi  = g11 g22 (W[p1][p2] - W[q1][q2]) + i g11 (Q[p1][p2][q2] - Q[q1][p2][q2])
+        //          + i g22 (Qp[p1][q1][p2] - Qp[p1][q1][q2])
+        //   Pi   = g22 (F[p1][q1][p2] - F[p1][q1][q2])
+        //   Pi^+ = conj( g(q1,p1) (Ft[q1][p2][q2] - Ft[p1][p2][q2]) )
+        //   K    = -i (comm + Psi + Pi - Pi^+)
+        const int p1 = t0, q1 = t1;
+        const double2 g11 = a.G1[(int64_t)p1 * N + q1];
+        const double2 gq1p1 = a.G1[(int64_t)q1 * N + p1];
+        const double* W = a.W[s];
+        const double2* Qs = a.Q[s];
+        const double2* Qps = a.Qp[s] + ((int64_t)p1 * N + q1) * N;
+        const double2* Frow = a.F[s] + ((int64_t)p1 * N + q1) * N;
+        const double2* Fts = a.Ft[s];
+        const double2* R = a.R[s];
+#pragma unroll
+        for (int u = 0; u < PW; ++u) {
+            const int t = wave + 4 * u;
+            if (t >= TT) continue;
+            const int n = (t % NT) * 16 + (lane & 15);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int m = (t / NT) * 16 + 4 * (lane >> 4) + j;
+                if (m >= N || n >= N) continue;
+                const int64_t off = base + m * sm + n * sn;
+                const int64_t mn = (int64_t)m * N + n;
+                double2 comm = cadd(R[off], make_double2(ar[u][j], ai[u][j]));
+                const double2 g22 = a.G1[mn];
+                double2 psi = cscale(cmul(g11, g22), W[(int64_t)p1 * N + m] - W[(int64_t)q1 * N + n]);
+                psi = cadd(psi, cmul_i(cmul(g11, csub(Qs[(int64_t)p1 * N2 + mn], Qs[(int64_t)q1 * N2 + mn]))));
+                psi = cadd(psi, cmul_i(cmul(g22, csub(Qps[m], Qps[n]))));
+                const double2 pi = cmul(g22, csub(Frow[m], Frow[n]));
+                const double2 pid = cconj(cmul(gq1p1, csub(Fts[(int64_t)q1 * N2 + mn], Fts[(int64_t)p1 * N2 + mn])));
+                const double2 K = cmul_mi(csub(cadd(cadd(comm, psi), pi), pid));
+                const double2 wK = cscale(K, a.wk);
+                if (a.first) {
+                    a.A[s][off] = wK;
+                    a.Kout[s][off] = K;
+                } else if (!a.last) {
+                    a.A[s][off] = cadd(a.A[s][off], wK);
+                    a.Kout[s][off] = K;
+                } else {
+                    const double2 A = cadd(a.A[s][off], wK);
+                    a.Sout[s][off] = cadd(S[off], cscale(cscale(A, a.dt), 1.0 / 6.0));
+                }
+            }
+        }
+    }
+}
+
+template <int MODE>
+void launch_tile(const TileArgs& a, int nspin, hipStream_t s) {
+    const int N = a.N;
+    dim3 grid(N * N, nspin), block(256);
+    if (N <= 16)
+        hipLaunchKernelGGL((k_tile<16, MODE>), grid, block, 0, s, a);
+    else if (N <= 32)
+        hipLaunchKernelGGL((k_tile<32, MODE>), grid, block, 0, s, a);
+    else if (N <= 48)
+        hipLaunchKernelGGL((k_tile<48, MODE>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((k_tile<64, MODE>), grid, block, 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Layout permutations between the reference pair order and the natural one.
+__device__ __forceinline__ int64_t ref_pair(int p1, int p2, int Ns, int Nb) {
+    const int i = p1 % Ns, a = p1 / Ns, j = p2 % Ns, b = p2 / Ns;
+    return (int64_t)i + (int64_t)j * Ns + ((int64_t)a + (int64_t)b * Nb) * Ns * Ns;
+}
+
+__global__ void k_ref_to_nat(const double2* in, double2* out, int Ns, int Nb) {
+    const int N = Ns * Nb;
+    const int64_t N2 = (int64_t)N * N;
+    const int64_t A = blockIdx.x;  // natural row p1*N + p2
+    const int p1 = (int)(A / N), p2 = (int)(A % N);
+    const int64_t Ar = ref_pair(p1, p2, Ns, Nb);
+    for (int B = threadIdx.x; B < N2; B += blockDim.x) {
+        const int q1 = B / N, q2 = B % N;
+        out[A * N2 + B] = in[Ar * N2 + ref_pair(q1, q2, Ns, Nb)];
+    }
+}
+
+__global__ void k_nat_to_ref(const double2* in, double2* out, int Ns, int Nb) {
+    const int N = Ns * Nb;
+    const int64_t N2 = (int64_t)N * N;
+    const int64_t A = blockIdx.x;
+    const int p1 = (int)(A / N), p2 = (int)(A % N);
+    const int64_t Ar = ref_pair(p1, p2, Ns, Nb);
+    for (int B = threadIdx.x; B < N2; B += blockDim.x) {
+        const int q1 = B / N, q2 = B % N;
+        out[Ar * N2 + ref_pair(q1, q2, Ns, Nb)] = in[A * N2 + B];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Routine-level kernels (reference-routine entry points).
+__global__ void k_hf(double2* h, const double2* G, int Ns, int Nb, StageParams sp) {
+    const int N = Ns * Nb;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * N) return;
+    h[e] = cadd(h[e], hf_element(G, e / N, e % N, Ns, Nb, sp));
+}
+
+__global__ void k_comm1(const double2* G, const double2* h, double2* out, int N) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * N) return;
+    const int X = e / N, Y = e % N;
+    double2 hg = make_double2(0.0, 0.0), gh = make_double2(0.0, 0.0);
+    for (int z = 0; z < N; ++z) {
+        hg = cadd(hg, cmul(h[(int64_t)X * N + z], G[(int64_t)z * N + Y]));
+        gh = cadd(gh, cmul(G[(int64_t)X * N + z], h[(int64_t)z * N + Y]));
+    }
+    out[e] = csub(hg, gh);
+}
+
+__global__ void k_d_gather(const double2* T, double2* D, int N) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N2 = (int64_t)N * N;
+    if (e >= N2 * N) return;
+    const int X = (int)(e / N2), Y = (int)((e / N) % N), Z = (int)(e % N);
+    D[e] = T[(((int64_t)X * N + Z) * N + Y) * N + Z];
+}
+
+__global__ void k_w_tables(double* Wxy, double* Wxx, int Ns, int Nb, StageParams sp) {
+    const int N = Ns * Nb;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * N) return;
+    const int X = e / N, Y = e % N;
+    const double w = w_xy(X, Y, Ns, sp);
+    Wxy[e] = w;
+    Wxx[e] = X == Y ? 0.0 : w;
+}
+
+// F[X][Y][Z] = -i sum_w ( W_xy[Z][w] DP[X][Y][w] + W_xx[Z][w] DQ[X][Y][w] )
+__global__ void k_f_table(const double2* DP, const double2* DQ, const double* Wxy, const double* Wxx,
+                          double2* F, double2* Ft, int N) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N2 = (int64_t)N * N;
+    if (e >= N2 * N) return;
+    const int X = (int)(e / N2), Y = (int)((e / N) % N), Z = (int)(e % N);
+    const double2* dp = DP + ((int64_t)X * N + Y) * N;
+    const double2* dq = DQ + ((int64_t)X * N + Y) * N;
+    double2 f = make_double2(0.0, 0.0);
+    for (int w = 0; w < N; ++w) {
+        const double a = Wxy[(int64_t)Z * N + w], b = Wxx[(int64_t)Z * N + w];
+        if (a == 0.0 && b == 0.0) continue;
+        f = cadd(f, cadd(cscale(dp[w], a), cscale(dq[w], b)));
+    }
+    f = cmul_mi(f);
+    F[e] = f;
+    if (Ft) Ft[((int64_t)Z * N + Y) * N + X] = f;
+}
+
+__global__ void k_q_table(const double2* G, const double* W, double2* Q, double2* Qp, int N) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N2 = (int64_t)N * N;
+    if (e >= N2 * N) return;
+    const int Z = (int)(e / N2), X = (int)((e / N) % N), Y = (int)(e % N);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int w = 0; w < N; ++w) {
+        const double wv = W[(int64_t)Z * N + w];
+        if (wv == 0.0) continue;
+        acc = cadd(acc, cscale(cmul(G[(int64_t)X * N + w], G[(int64_t)w * N + Y]), wv));
+    }
+    Q[e] = acc;
+    if (Qp) Qp[((int64_t)X * N + Y) * N + Z] = acc;
+}
+
+__global__ void k_collision_from_f(const double2* F, double2* I, int N) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * N) return;
+    const int X = e / N, Y = e % N;
+    I[e] = F[((int64_t)X * N + Y) * N + X];
+}
+
+// Psi[p1,p2,q1,q2] (natural layout) from the Q tables, as in the fused pass.
+__global__ void k_psi(const double2* G, const double* W, const double2* Q, const double2* Qp, double2* out, int N) {
+    const int64_t N2 = (int64_t)N * N;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N2 * N2) return;
+    const int p1 = (int)(e / (N2 * N)), p2 = (int)((e / N2) % N), q1 = (int)((e / N) % N), q2 = (int)(e % N);
+    const double2 g11 = G[(int64_t)p1 * N + q1], g22 = G[(int64_t)p2 * N + q2];
+    const int64_t mn = (int64_t)p2 * N + q2;
+    const double2* Qps = Qp + ((int64_t)p1 * N + q1) * N;
+    double2 psi = cscale(cmul(g11, g22), W[(int64_t)p1 * N + p2] - W[(int64_t)q1 * N + q2]);
+    psi = cadd(psi, cmul_i(cmul(g11, csub(Q[(int64_t)p1 * N2 + mn], Q[(int64_t)q1 * N2 + mn]))));
+    psi = cadd(psi, cmul_i(cmul(g22, csub(Qps[p2], Qps[q2]))));
+    out[e] = psi;
+}
+
+__global__ void k_pi(const double2* G, const double2* F, double2* out, int N) {
+    const int64_t N2 = (int64_t)N * N;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N2 * N2) return;
+    const int p1 = (int)(e / (N2 * N)), p2 = (int)((e / N2) % N), q1 = (int)((e / N) % N), q2 = (int)(e % N);
+    const double2* Frow = F + ((int64_t)p1 * N + q1) * N;
+    out[e] = cmul(G[(int64_t)p2 * N + q2], csub(Frow[p2], Frow[q2]));
+}
+
+__global__ void k_g1_step(const double2* comm, const double2* I, double2* K, double2* RK, double w, int N) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * N) return;
+    const int X = e / N, Y = e % N;
+    const double2 k = cmul_mi(cadd(cadd(comm[e], I[e]), cconj(I[(int64_t)Y * N + X])));
+    K[e] = k;
+    RK[e] = cadd(RK[e], cscale(k, w));
+}
+
+// natural layout: the particle-swapped transposed partner of (p1,p2,q1,q2)
+// is (q2,q1,p2,p1)
+__global__ void k_g2_step(const double2* comm, const double2* psi, const double2* pi, double2* K, double2* RK,
+                          double w, int N) {
+    const int64_t N2 = (int64_t)N * N;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N2 * N2) return;
+    const int p1 = (int)(e / (N2 * N)), p2 = (int)((e / N2) % N), q1 = (int)((e / N) % N), q2 = (int)(e % N);
+    const int64_t t = (((int64_t)q2 * N + q1) * N + p2) * N + p1;
+    const double2 k = cmul_mi(csub(cadd(cadd(comm[e], psi[e]), pi[e]), cconj(pi[t])));
+    K[e] = k;
+    RK[e] = cadd(RK[e], cscale(k, w));
+}
+
+inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+
+void launch_stage_g1(const DevState& d, const StageParams& sp, hipStream_t s) {
+    hipLaunchKernelGGL(k_stage_g1, dim3(1), dim3(1024), 0, s, d, sp);
+}
+
+void launch_record(const DevState& d, int slot, int64_t step, hipStream_t s) {
+    hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, s, d, slot, step);
+}
+
+static TileArgs base_args(const DevState& d, const StageParams& sp) {
+    TileArgs a{};
+    for (int s = 0; s < 2; ++s) {
+        a.S[s] = d.S2[s];
+        a.Kp[s] = sp.first ? nullptr : d.K2[s];
+    }
+    a.c = sp.c;
+    a.h = d.h;
+    a.hmode = 3;
+    a.N = d.N;
+    a.fail = d.fail;
+    return a;
+}
+
+void launch_pass_strided(const DevState& d, const StageParams& sp, int precision, hipStream_t s) {
+    (void)precision;
+    TileArgs a = base_args(d, sp);
+    a.tiling = 0;
+    for (int t = 0; t < 2; ++t) {
+        a.out[t] = d.R2[t];
+        a.D[t] = d.D[t];
+    }
+    a.accumulate = 0;
+    launch_tile<0>(a, 2, s);
+}
+
+void launch_tables(const DevState& d, const StageParams& sp, hipStream_t s) {
+    (void)sp;
+    const int N = d.N;
+    hipLaunchKernelGGL(k_q_tables, dim3(N * N), dim3(64), 0, s, d);
+    hipLaunchKernelGGL(k_f_tables, dim3(N * N), dim3(64), 0, s, d);
+}
+
+void launch_g1_rhs(const DevState& d, const StageParams& sp, hipStream_t s) {
+    hipLaunchKernelGGL(k_g1_rhs, dim3(1), dim3(1024), 0, s, d, sp);
+}
+
+void launch_pass_fused(const DevState& d, const StageParams& sp, int precision, hipStream_t s) {
+    (void)precision;
+    TileArgs a = base_args(d, sp);
+    a.tiling = 1;
+    for (int t = 0; t < 2; ++t) {
+        a.R[t] = d.R2[t];
+        a.Kout[t] = d.K2[t];
+        a.A[t] = d.A2[t];
+        a.Sout[t] = d.S2[t];
+        a.Q[t] = d.Q[t];
+        a.Qp[t] = d.Qp[t];
+        a.F[t] = d.F[t];
+        a.Ft[t] = d.Ft[t];
+    }
+    a.W[0] = d.Wxx;
+    a.W[1] = d.Wxy;
+    a.G1 = d.Y1;
+    a.wk = sp.wk;
+    a.dt = sp.dt;
+    a.first = sp.first;
+    a.last = sp.last;
+    launch_tile<1>(a, 2, s);
+}
+
+void launch_ref_to_nat(const double2* in, double2* out, int Ns, int Nb, hipStream_t s) {
+    const int N = Ns * Nb;
+    hipLaunchKernelGGL(k_ref_to_nat, dim3(N * N), dim3(256), 0, s, in, out, Ns, Nb);
+}
+
+void launch_nat_to_ref(const double2* in, double2* out, int Ns, int Nb, hipStream_t s) {
+    const int N = Ns * Nb;
+    hipLaunchKernelGGL(k_nat_to_ref, dim3(N * N), dim3(256), 0, s, in, out, Ns, Nb);
+}
+
+void launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, double2* tmp, int N,
+                  int precision, hipStream_t s) {
+    (void)precision;
+    (void)tmp;
+    TileArgs a{};
+    a.S[0] = T;
+    a.Kp[0] = nullptr;
+    a.N = N;
+    a.out[0] = out;
+    // four accumulating launches: (strided, contiguous) x (left hx, right hy)
+    int first = 1;
+    for (int tiling = 0; tiling < 2; ++tiling)
+        for (int side = 1; side <= 2; ++side) {
+            a.tiling = tiling;
+            a.hmode = side;
+            a.h = side == 1 ? hx : hy;
+            a.accumulate = first ? 0 : 1;
+            first = 0;
+            launch_tile<0>(a, 1, s);
+        }
+}
+
+void launch_hf(double2* h, const double2* G1, int Ns, int Nb, const StageParams& sp, hipStream_t s) {
+    const int n2 = Ns * Nb * Ns * Nb;
+    hipLaunchKernelGGL(k_hf, dim3(blocks_for(n2, 256)), dim3(256), 0, s, h, G1, Ns, Nb, sp);
+}
+
+void launch_comm1(const double2* G, const double2* h, double2* out, int N, hipStream_t s) {
+    hipLaunchKernelGGL(k_comm1, dim3(blocks_for((int64_t)N * N, 256)), dim3(256), 0, s, G, h, out, N);
+}
+
+void launch_d_gather(const double2* T, double2* D, int N, hipStream_t s) {
+    const int64_t n3 = (int64_t)N * N * N;
+    hipLaunchKernelGGL(k_d_gather, dim3(blocks_for(n3, 256)), dim3(256), 0, s, T, D, N);
+}
+
+void launch_w_tables(double* Wxy, double* Wxx, int Ns, int Nb, const StageParams& sp, hipStream_t s) {
+    const int n2 = Ns * Nb * Ns * Nb;
+    hipLaunchKernelGGL(k_w_tables, dim3(blocks_for(n2, 256)), dim3(256), 0, s, Wxy, Wxx, Ns, Nb, sp);
+}
+
+void launch_f_tables(const double2* DP, const double2* DQ, const double* Wxy, const double* Wxx, double2* F,
+                     double2* Ft, int N, hipStream_t s) {
+    const int64_t n3 = (int64_t)N * N * N;
+    hipLaunchKernelGGL(k_f_table, dim3(blocks_for(n3, 256)), dim3(256), 0, s, DP, DQ, Wxy, Wxx, F, Ft, N);
+}
+
+void launch_q_table(const double2* G1, const double* W, double2* Q, double2* Qp, int N, hipStream_t s) {
+    const int64_t n3 = (int64_t)N * N * N;
+    hipLaunchKernelGGL(k_q_table, dim3(blocks_for(n3, 256)), dim3(256), 0, s, G1, W, Q, Qp, N);
+}
+
+void launch_collision_from_f(const double2* F, double2* I, int N, hipStream_t s) {
+    hipLaunchKernelGGL(k_collision_from_f, dim3(blocks_for((int64_t)N * N, 256)), dim3(256), 0, s, F, I, N);
+}
+
+void launch_psi(const double2* G1, const double* W, const double2* Q, const double2* Qp, double2* psi, int N,
+                hipStream_t s) {
+    const int64_t n4 = (int64_t)N * N * N * N;
+    hipLaunchKernelGGL(k_psi, dim3(blocks_for(n4, 256)), dim3(256), 0, s, G1, W, Q, Qp, psi, N);
+}
+
+void launch_pi(const double2* G1, const double2* F, double2* pi, int N, hipStream_t s) {
+    const int64_t n4 = (int64_t)N * N * N * N;
+    hipLaunchKernelGGL(k_pi, dim3(blocks_for(n4, 256)), dim3(256), 0, s, G1, F, pi, N);
+}
+
+void launch_g1_step(const double2* comm, const double2* I, double2* K, double2* RK, double w, int N,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_g1_step, dim3(blocks_for((int64_t)N * N, 256)), dim3(256), 0, s, comm, I, K, RK, w, N);
+}
+
+void launch_g2_step(const double2* comm, const double2* psi, const double2* pi, double2* K, double2* RK, double w,
+                    int N, hipStream_t s) {
+    const int64_t n4 = (int64_t)N * N * N * N;
+    hipLaunchKernelGGL(k_g2_step, dim3(blocks_for(n4, 256)), dim3(256), 0, s, comm, psi, pi, K, RK, w, N);
+}
+
+}  // namespace g12

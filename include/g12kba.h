@@ -1,0 +1,210 @@
+/*
+ * g12kba.h — C ABI of the MI355X-native HF-GKBA G1-G2 (GW) propagator.
+ *
+ * Drop-in boundary for the reference's RK4 stepper and time-derivative
+ * routines (VlcekGroup/G12KBA, headers under G12KBA-GW/src/headers).  Plain pointers and
+ * sizes only.  Complex numbers are interleaved (re, im) doubles, exactly the
+ * memory of std::complex<double>.  Layouts are the reference's:
+ *
+ *   N = Ns*Nb, one-particle index X = site + orb*Ns   (HF.h:62-93)
+ *   G1[X][Y] at X*N + Y                                 (row-major N x N)
+ *   G2[A][B] at A*N*N + B, pair index of p1=(i,a), p2=(j,b):
+ *       A = i + j*Ns + (a + b*Nb)*Ns*Ns                 (commutators.h:59-60)
+ *
+ * Conventions (reference semantics unless noted):
+ *   - every routine returns 0 on success or a negative G12_E* code; no
+ *     exception crosses the ABI;
+ *   - outputs are fully overwritten, except `h` in g12_hf_hamiltonian and
+ *     `RK_result` in g12_g1_step / g12_g2_step, which accumulate (+=);
+ *   - calls are synchronous at return; a context is used by one host thread;
+ *   - one GPU (device 0 unless G12KBA_DEVICE is set).
+ */
+#ifndef G12KBA_H
+#define G12KBA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define G12_OK 0
+#define G12_GUARD 1          /* RK4 density guard tripped (reference return value 1) */
+#define G12_E_ARG (-1)       /* invalid argument */
+#define G12_E_HIP (-2)       /* HIP runtime failure */
+#define G12_E_NOMEM (-3)     /* device allocation failed */
+#define G12_E_SIZE (-4)      /* size outside what the kernels support */
+#define G12_E_IO (-5)        /* file I/O failure */
+
+/* quench protocol */
+#define G12_QUENCH_FULL 0
+#define G12_QUENCH_PULSE 1
+
+/* long-range interaction law V_n (n = 1..Ns-1) for EHM = true */
+#define G12_VLAW_EXP 0       /* V_n = U exp(-decay_rate n)   (RK_solver.h:74, intended law) */
+#define G12_VLAW_COULOMB 1   /* V_n = U / (1 + n) */
+#define G12_VLAW_EXPLICIT 2  /* V_n = U * V_profile[n-1] */
+
+/* precision of the two-particle contractions ([H2,G2]); G1, I and the RK
+ * state are FP64 in every mode */
+#define G12_PREC_FP64 0
+#define G12_PREC_FP32_3M 1   /* 3M complex FP32 products, FP32 accumulation */
+#define G12_PREC_BF16X3 2    /* BF16 hi/lo split (hh + hl + lh), FP32 accumulation */
+
+#define G12_MAX_NB 8
+#define G12_MAX_SHELLS 128
+
+/* Run parameters: every global of read_input.h:27-49 plus the choices the
+ * reference leaves undefined or hard-codes (its AS dt, tail length, quench
+ * gate time, V law, orbital energies, filling). */
+typedef struct g12_params {
+  int Ns, Nb;
+  double t1, t2, t3;          /* t3 is read but unused, as in the reference */
+  double U;
+  int time_steps;
+  double dt;                  /* dt_fixed */
+  double quench_strength;
+  int Nq;
+  int q_type;                 /* G12_QUENCH_* ("none" = full with strength 0, Nq 0) */
+  int HF;                     /* 1: Hartree-Fock only (G2 frozen, no collision term) */
+  int EHM;                    /* 1: long-range interactions V_n */
+  double AS_rate, AS_midpoint;
+  double quench_rate, quench_on_midpoint, quench_off_midpoint;
+  double decay_rate;          /* 0.7 (read_input.h:29) */
+  int V_law;                  /* G12_VLAW_* */
+  int nV_profile;             /* entries in V_profile (EXPLICIT law) */
+  double V_profile[G12_MAX_SHELLS];
+  double eps[G12_MAX_NB];     /* orbital energies, default 0 (reference defect B-4) */
+  double AS_dt;               /* 0.1 (AS.h:115) */
+  int AS_tail;                /* 20  (AS.h:96) */
+  double quench_gate_time;    /* 50  (RK_solver.h:141) */
+  int p_number;               /* filled states in G0; <0 means Ns/2 (Main.cpp:52) */
+  int precision;              /* G12_PREC_* */
+} g12_params;
+
+typedef struct g12_ctx g12_ctx;
+
+/* ---- parameters and input file ----------------------------------------- */
+
+/* Compiled defaults of read_input.h:27-49 plus the documented extensions. */
+int g12_params_default(g12_params* p);
+
+/* Parse a reference-format `input` file (key line, value on the next line;
+ * substring key match, first match wins; a missing key or value gives 0 and a
+ * warning on stderr).  Replaces assign_vals, read_input.h:53-146.  Two
+ * optional keys extend the format: `long_range_law` (exp | coulomb) and
+ * `filling`. */
+int g12_parse_input(const char* path, g12_params* p);
+
+/* ---- initial state --------------------------------------------------------
+ * G1(0) = i * sum_{k < p_number} phi_k phi_k^dagger over the lowest
+ * eigenvectors of the U = 0 Hamiltonian.  Replaces make_G0, init_G.h:57-75
+ * (the reference's FP32 LAPACKE_cheev is replaced by an FP64 Jacobi solver;
+ * fp32_emulation = 1 rounds h and the eigenvectors through float). */
+int g12_make_g0(const g12_params* p, double* G1, int fp32_emulation);
+
+/* ---- context, state, drivers --------------------------------------------- */
+
+int g12_create(const g12_params* p, g12_ctx** out);
+int g12_destroy(g12_ctx* ctx);
+
+/* Upload / download (G1, G2xxxx, G2xyxy) in the reference layouts. */
+int g12_set_state(g12_ctx* ctx, const double* G1, const double* G2xx, const double* G2xy);
+int g12_get_state(g12_ctx* ctx, double* G1, double* G2xx, double* G2xy);
+
+/* Adiabatic switching, replaces Adiabatic_switching (AS.h:63-214).
+ * *steps receives the number of RK4 steps taken. */
+int g12_adiabatic_switching(g12_ctx* ctx, int* steps);
+
+/* Production RK4, replaces RK4 (RK_solver.h:64-226).  The G1 matrix before
+ * every step is written to traj (capacity traj_cap matrices, may be NULL);
+ * *steps receives the number of iterations begun (= trajectory length).
+ * Returns 0, G12_GUARD when the density guard trips, or a negative code. */
+int g12_rk4(g12_ctx* ctx, double* traj, int64_t traj_cap, int64_t* steps);
+
+/* Advance n RK4 steps of the production protocol from the context's current
+ * time without recording (benchmarks, checkpoints).  Same guard semantics. */
+int g12_rk4_advance(g12_ctx* ctx, int64_t n, int64_t* steps_done);
+
+/* Current production time and step count of the context. */
+int g12_get_time(g12_ctx* ctx, double* t, int64_t* steps);
+
+/* Per-kernel timing with HIP events on the context's stream.
+ * g12_timing_enable(ctx, 1) starts recording; g12_timing_get returns, for the
+ * kernel class `which` (0: two-particle commutator pass over strided tiles,
+ * 1: fused commutator + right-hand side pass, 2: all other kernels), the number
+ * of launches and their summed duration in milliseconds. */
+int g12_timing_enable(g12_ctx* ctx, int on);
+int g12_timing_get(g12_ctx* ctx, int which, int64_t* launches, double* total_ms);
+
+/* Device footprint of a context in bytes. */
+int g12_memory_bytes(g12_ctx* ctx, int64_t* bytes);
+
+/* ---- routine-level entry points ------------------------------------------
+ * One entry per reference routine, same argument roles and order, host
+ * buffers in the reference layouts.  The work runs on the GPU. */
+
+/* h += h_HF[G1x] (h_HFx_quench, HF.h:57-104).  G1y is ignored as in the
+ * reference; V has nV entries; eps has Nb entries (NULL = zeros). */
+int g12_hf_hamiltonian(double* h, const double* G1x, const double* G1y, int Ns, int Nb, double U,
+                       double t1, double t2, double w, const double* eps, const double* V, int nV,
+                       int quench_extent);
+
+/* comm = h G - G h (hG_commutator, commutators.h:28-51); G, h are N x N. */
+int g12_h1_commutator(const double* G, const double* h, double* comm, int N);
+
+/* comm = [h1x (x) 1 + 1 (x) h1x] G2 - G2 [h1y (x) 1 + 1 (x) h1y]
+ * (h2xyG2xyxy_commutator, commutators.h:53-99). */
+int g12_h2_commutator(const double* G2, const double* h1x, const double* h1y, double* comm, int Ns,
+                      int Nb, int precision);
+
+/* Collision integral (collision_int, collision_integral.h:27-69); note the
+ * reference's argument order (xyxy, xxxx). */
+int g12_collision(double* Ix, const double* G2xyxy, const double* G2xxxx, int Ns, int Nb, double U,
+                  const double* V, int nV);
+
+/* GW two-particle sources (Psi_xxxx psi.h:157-277, Psi_xyxy psi.h:26-154). */
+int g12_psi_xxxx(double* psi, const double* G1, int Ns, int Nb, double U, const double* V, int nV);
+int g12_psi_xyxy(double* psi, const double* G1, int Ns, int Nb, double U, const double* V, int nV);
+
+/* GW polarisation term (Pi_xyxy, pi.h:25-95); called with the G2 pair in
+ * either order exactly as the reference (RK_solver.h:163-164). */
+int g12_pi(double* pi, const double* G1y, const double* G2xyxy, const double* G2xxxx, double U,
+           int Ns, int Nb, const double* V, int nV);
+
+/* Stage derivative and RK accumulation (G1_step / G2_step,
+ * G1_G2_t_step.h:24-47 / 55-86). */
+int g12_g1_step(const double* comm, const double* I, double* K, double* RK_result, double w, int Ns,
+                int Nb);
+int g12_g2_step(const double* comm, const double* psi, const double* pi, double* K,
+                double* RK_result, double w, int Ns, int Nb);
+
+/* ---- host helpers -------------------------------------------------------- */
+
+/* Logistic ramp (fermi, fermi_func.h:21-24). */
+double g12_fermi(double t, double t0, double tau);
+
+/* Dipole trajectory (dipole, dipole.h:25-33); traj holds T N x N matrices. */
+int g12_dipole(double* p, const double* traj, int64_t T, int N);
+
+/* Byte-compatible text outputs (write_to_file.h:26-69): <dir>/real_G1.txt,
+ * <dir>/imag_G1.txt (one row per matrix element, one column per step, a space
+ * after every value) and <dir>/dipole.txt.  `progress` (percent) is written to
+ * <dir>/progress.txt when >= 0. */
+int g12_write_outputs(const char* dir, const double* traj, int64_t T, int N, const double* dipole,
+                      double progress);
+
+/* Binary full-precision state dump / load: (t, steps, G1, G2xx, G2xy) in the
+ * reference layouts. */
+int g12_save_state(g12_ctx* ctx, const char* path);
+int g12_load_state(g12_ctx* ctx, const char* path);
+
+/* Library version string. */
+const char* g12_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* G12KBA_H */

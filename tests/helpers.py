@@ -1,0 +1,25 @@
+"""Shared inputs for the parity tests."""
+import numpy as np
+
+
+def rand_c(rng, *shape, scale=1.0):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * scale
+
+
+def anti_hermitian(rng, n, scale=1.0):
+    a = rand_c(rng, n, n, scale=scale)
+    return 0.5 * (a - a.conj().T)
+
+
+def oracle_cfg(p):
+    """Oracle run configuration from a g12kba_amd.Params."""
+    return dict(Ns=p.Ns, Nb=p.Nb, t1=p.t1, t2=p.t2, U=p.U, dt=p.dt, time_steps=p.time_steps,
+                quench_strength=p.quench_strength, Nq=p.Nq,
+                q_type="pulse" if p.q_type == 1 else "full", HF=bool(p.HF),
+                AS_rate=p.AS_rate, AS_midpoint=p.AS_midpoint, quench_rate=p.quench_rate,
+                quench_on_midpoint=p.quench_on_midpoint, quench_off_midpoint=p.quench_off_midpoint,
+                AS_dt=p.AS_dt, AS_tail=p.AS_tail, gate_time=p.quench_gate_time)
+
+
+# (Ns, Nb, nV) geometries for routine-level parity; nV long-range shells
+GEOMS = [(2, 1, 0), (3, 2, 2), (2, 3, 1), (4, 2, 3), (5, 1, 4), (3, 3, 2)]

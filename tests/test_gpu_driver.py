@@ -1,0 +1,101 @@
+"""Device-resident AS / RK4 drivers against the CPU oracle (FP64)."""
+import numpy as np
+import pytest
+
+from tests.helpers import oracle_cfg, rand_c
+
+pytestmark = pytest.mark.gpu
+
+g = pytest.importorskip("g12kba_amd")
+from oracle.oracle import Oracle  # noqa: E402
+from oracle import fast  # noqa: E402
+
+O = Oracle()
+
+
+def small_state(rng, p, scale=1e-2):
+    N = p.Ns * p.Nb
+    G1 = g.make_G0(p)
+    A = rand_c(rng, N * N, N * N, scale=scale)
+    B = rand_c(rng, N * N, N * N, scale=scale)
+    return G1, A - A.conj().T, B - B.conj().T
+
+
+@pytest.mark.parametrize("ehm,law,qt", [(False, "exp", "full"), (True, "coulomb", "pulse"), (True, "exp", "full")])
+@pytest.mark.parametrize("Ns,Nb", [(3, 2), (4, 2), (2, 3)])
+def test_rk4_matches_oracle(Ns, Nb, ehm, law, qt):
+    p = g.make_params(Ns=Ns, Nb=Nb, t1=1.0, t2=0.4, U=1.4, dt=0.02, time_steps=12, quench_strength=1.3,
+                      Nq=max(1, Ns // 2), q_type=qt, quench_rate=2.0, quench_on_midpoint=0.05,
+                      quench_off_midpoint=0.15, quench_gate_time=0.06, EHM=ehm, V_law=law)
+    rng = np.random.default_rng(Ns * 100 + Nb)
+    G1, Gxx, Gxy = small_state(rng, p)
+    with g.Propagator(p) as prop:
+        prop.set_state(G1, Gxx, Gxy)
+        rc, traj = prop.rk4()
+        G1g, Gxxg, Gxyg = prop.get_state()
+    rc2, G1o, Gxxo, Gxyo, trajo = O.rk4(oracle_cfg(p), G1, Gxx, Gxy, V_profile=g.v_profile(p))
+    assert rc == rc2 == 0
+    assert traj.shape == trajo.shape
+    assert np.abs(traj - trajo).max() < 1e-12
+    for a, b in ((G1g, G1o), (Gxxg, Gxxo), (Gxyg, Gxyo)):
+        assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(b).max())
+
+
+def test_adiabatic_switching_matches_oracle():
+    p = g.make_params(Ns=3, Nb=2, t1=1.0, t2=0.5, U=1.0, AS_rate=0.05, AS_midpoint=0.3, AS_tail=4,
+                      EHM=True, V_law="coulomb")
+    rng = np.random.default_rng(11)
+    G1, Gxx, Gxy = small_state(rng, p, scale=0.0)
+    with g.Propagator(p) as prop:
+        prop.set_state(G1, Gxx, Gxy)
+        n = prop.adiabatic_switching()
+        G1g, Gxxg, Gxyg = prop.get_state()
+    G1o, Gxxo, Gxyo, n2 = O.adiabatic_switching(oracle_cfg(p), G1, Gxx, Gxy, V_profile=g.v_profile(p))
+    assert n == n2
+    assert np.abs(G1g - G1o).max() < 1e-12
+    assert np.abs(Gxxg - Gxxo).max() < 1e-12 and np.abs(Gxyg - Gxyo).max() < 1e-12
+
+
+def test_density_guard():
+    """A state with 2 Im G1[0,0] > 3 is recorded once and RK4 returns 1 (RK_solver.h:100-110)."""
+    p = g.make_params(Ns=3, Nb=1, U=1.0, dt=0.02, time_steps=10)
+    G1 = g.make_G0(p)
+    G1[0, 0] = 2j
+    with g.Propagator(p) as prop:
+        prop.set_state(G1)
+        rc, traj = prop.rk4()
+        after = prop.get_G1()
+    assert rc == 1 and len(traj) == 1
+    assert np.array_equal(traj[0], G1) and np.array_equal(after, G1)
+
+
+def test_hf_only_freezes_g2():
+    p = g.make_params(Ns=4, Nb=1, U=1.0, HF=True, dt=0.05, time_steps=6)
+    rng = np.random.default_rng(3)
+    G1, Gxx, Gxy = small_state(rng, p)
+    with g.Propagator(p) as prop:
+        prop.set_state(G1, Gxx, Gxy)
+        rc, traj = prop.rk4()
+        G1g, Gxxg, Gxyg = prop.get_state()
+    rc2, G1o, _, _, _ = O.rk4(oracle_cfg(p), G1, Gxx, Gxy)
+    assert rc == 0 and np.array_equal(Gxxg, Gxx) and np.array_equal(Gxyg, Gxy)
+    assert np.abs(G1g - G1o).max() < 1e-13
+
+
+def test_n32_stage_matches_factorised_oracle():
+    """One RK4 step at the N = 32 production size against the numpy oracle."""
+    p = g.make_params(Ns=16, Nb=2, t1=1.0, t2=0.3, U=2.0, dt=0.01, time_steps=1, EHM=True, V_law="coulomb")
+    rng = np.random.default_rng(5)
+    G1, Gxx, Gxy = small_state(rng, p, scale=1e-3)
+    with g.Propagator(p) as prop:
+        prop.set_state(G1, Gxx, Gxy)
+        rc, _ = prop.rk4()
+        G1g, Gxxg, Gxyg = prop.get_state()
+    cfg = oracle_cfg(p)
+    Ns, Nb = 16, 2
+    rc2, G1o, Txx, Txy, _ = fast.rk4(cfg, G1, fast.ref_to_nat(Gxx, Ns, Nb), fast.ref_to_nat(Gxy, Ns, Nb),
+                                     V_profile=list(g.v_profile(p)))
+    assert rc == rc2 == 0
+    assert np.abs(G1g - G1o).max() < 1e-13
+    assert np.abs(Gxxg - fast.nat_to_ref(Txx, Ns, Nb)).max() < 1e-13
+    assert np.abs(Gxyg - fast.nat_to_ref(Txy, Ns, Nb)).max() < 1e-13
