@@ -374,7 +374,7 @@ double* D(std::vector<cplx>& v) { return reinterpret_cast<double*>(v.data()); }
 // give the interaction strength and the quench for stage k.
 template <class FU, class FQ>
 void rk4_step(const orc_params& p, Work& W, cplx* G1, cplx* Gxx, cplx* Gxy, double dt_full,
-              FU stage_U, FQ stage_q) {
+              FU stage_U, FQ stage_q, int nstages = 4) {
     const int Ns = p.Ns, Nb = p.Nb;
     const bool corr = (p.U != 0.0) && !p.hf_only;
     std::fill(W.A1.begin(), W.A1.end(), 0.0);
@@ -384,7 +384,7 @@ void rk4_step(const orc_params& p, Work& W, cplx* G1, cplx* Gxx, cplx* Gxy, doub
     std::fill(W.Kxx.begin(), W.Kxx.end(), 0.0);
     std::fill(W.Kxy.begin(), W.Kxy.end(), 0.0);
     double dtk = dt_full;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < nstages; ++k) {
         dtk = (k == 1 || k == 2) ? dt_full / 2 : dt_full;
         const double wk = (k == 1 || k == 2) ? 2.0 : 1.0;
         for (int64_t e = 0; e < W.n2; ++e) W.Y1[e] = G1[e] + W.K1[e] * dtk;
@@ -509,6 +509,18 @@ void orc_rk4_fixed_step(const orc_params* pp, double* G1_, double* Gxx_, double*
     auto sU = [&](int, double) { return p.U; };
     auto sQ = [&](int, double, double* w, int* nq) { *w = 0.0; *nq = 0; };
     rk4_step(p, W, C(G1_), C(Gxx_), C(Gxy_), dt, sU, sQ);
+}
+
+// The first nstages stages of one RK4 step (stage input, derivative, RK
+// accumulation; the state update only after stage 4).  Used to time a
+// bounded sample of the CPU path.
+void orc_rk4_partial_step(const orc_params* pp, double* G1_, double* Gxx_, double* Gxy_, double dt,
+                          int nstages) {
+    const orc_params& p = *pp;
+    Work W(p.Ns, p.Nb);
+    auto sU = [&](int, double) { return p.U; };
+    auto sQ = [&](int, double, double* w, int* nq) { *w = 0.0; *nq = 0; };
+    rk4_step(p, W, C(G1_), C(Gxx_), C(Gxy_), dt, sU, sQ, nstages);
 }
 
 }  // extern "C"

@@ -74,6 +74,7 @@ class Oracle:
         L.orc_rk4.argtypes = [ctypes.POINTER(OrcParams), p, p, p, p, ctypes.c_int64,
                               ctypes.POINTER(ctypes.c_int64)]
         L.orc_rk4_fixed_step.argtypes = [ctypes.POINTER(OrcParams), p, p, p, d]
+        L.orc_rk4_partial_step.argtypes = [ctypes.POINTER(OrcParams), p, p, p, d, i]
 
     # --- routines (same argument roles and order as the reference) ---
     def hf(self, h, G1, Ns, Nb, U, t1, t2, w, eps, V, quench_extent):
@@ -192,6 +193,12 @@ class Oracle:
         rc = self.L.orc_rk4(ctypes.byref(p), _ptr(G1), _ptr(Gxx), _ptr(Gxy), _ptr(traj),
                             traj_cap, ctypes.byref(n))
         return rc, G1, Gxx, Gxy, traj[: min(n.value, traj_cap)]
+
+    def partial_step(self, cfg, G1, Gxx, Gxy, dt, nstages, V_profile=None):
+        p, keep = self.params(cfg, V_profile)
+        G1, Gxx, Gxy = _c(G1).copy(), _c(Gxx).copy(), _c(Gxy).copy()
+        self.L.orc_rk4_partial_step(ctypes.byref(p), _ptr(G1), _ptr(Gxx), _ptr(Gxy), dt, nstages)
+        return G1, Gxx, Gxy
 
     def fixed_step(self, cfg, G1, Gxx, Gxy, dt):
         p, keep = self.params(cfg)
