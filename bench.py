@@ -3,8 +3,7 @@
 
 Workload (N = 64): 1-D chain with Ns = 32 sites and Nb = 2 orbitals, long-range
 interactions V_n = U/(1+n), U = 1.5, t1 = 1, t2 = 0.5, full quench of strength
-1.0 on the first 8 sites, dt = 0.01 (quench gate open, so every stage carries
-the quench).  G1 starts from the non-interacting state; the two G2 tensors
+1.0 on the first 8 sites, dt = 0.01, quench ramp midpoint 30 and rate 2.0.  G1 starts from the non-interacting state; the two G2 tensors
 (2^24 complex entries each) hold a small seeded anti-Hermitian perturbation.
 A "step" is one RK4 step (4 stages) of the production protocol; inputs are
 resident in HBM before the timed region.
@@ -88,12 +87,7 @@ def main():
     prop.set_state(G1, Gxx, Gxy)
     if a.warmup:
         prop.advance(a.warmup)
-    try:
-        import torch
-        sync = torch.cuda.synchronize
-    except Exception:  # the library synchronises its own stream at return
-        def sync():
-            pass
+    sync = g.synchronize
     prop.timing(True)
     sync()
     t0 = time.perf_counter()

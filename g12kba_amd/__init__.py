@@ -75,6 +75,7 @@ def _load():
         "g12_timing_enable": [vp, i],
         "g12_timing_get": [vp, i, pi64, p],
         "g12_memory_bytes": [vp, pi64],
+        "g12_synchronize": [],
         "g12_hf_hamiltonian": [p, p, p, i, i, d, d, d, d, p, p, i, i],
         "g12_h1_commutator": [p, p, p, i],
         "g12_h2_commutator": [p, p, p, p, i, i, i],
@@ -105,7 +106,7 @@ lib = _load()
 EXPORTED = [
     "g12_params_default", "g12_parse_input", "g12_make_g0", "g12_create", "g12_destroy",
     "g12_set_state", "g12_get_state", "g12_adiabatic_switching", "g12_rk4", "g12_rk4_advance",
-    "g12_get_time", "g12_timing_enable", "g12_timing_get", "g12_memory_bytes",
+    "g12_get_time", "g12_timing_enable", "g12_timing_get", "g12_memory_bytes", "g12_synchronize",
     "g12_hf_hamiltonian", "g12_h1_commutator", "g12_h2_commutator", "g12_collision",
     "g12_psi_xxxx", "g12_psi_xyxy", "g12_pi", "g12_g1_step", "g12_g2_step", "g12_fermi",
     "g12_dipole", "g12_write_outputs", "g12_save_state", "g12_load_state", "g12_version",
@@ -411,6 +412,11 @@ class Propagator:
 
     def load(self, path):
         _check(lib.g12_load_state(self.h, path.encode()), "g12_load_state")
+
+
+def synchronize():
+    """Device-wide synchronisation of the library's GPU."""
+    _check(lib.g12_synchronize(), "g12_synchronize")
 
 
 def gpu_available():

@@ -282,6 +282,11 @@ int g12_destroy(g12_ctx* c) {
     return 0;
 }
 
+int g12_synchronize(void) {
+    if (select_device() < 0) return G12_E_HIP;
+    return hipDeviceSynchronize() == hipSuccess ? 0 : G12_E_HIP;
+}
+
 int g12_memory_bytes(g12_ctx* c, int64_t* bytes) {
     if (!c || !bytes) return G12_E_ARG;
     *bytes = c->bytes;

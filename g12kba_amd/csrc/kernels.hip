@@ -32,6 +32,7 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cdiv6(double2 a) { return make_double2(a.x / 6.0, a.y / 6.0); }
 // i*z and -i*z
 __device__ __forceinline__ double2 cmul_i(double2 a) { return make_double2(-a.y, a.x); }
 __device__ __forceinline__ double2 cmul_mi(double2 a) { return make_double2(a.y, -a.x); }
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(1024) void k_g1_rhs(DevState d, StageParams sp) {
             d.K1[e] = K;
         } else {
             const double2 A = cadd(d.A1[e], wK);
-            d.S1[e] = cadd(d.S1[e], cscale(cscale(A, sp.dt), 1.0 / 6.0) );
+            d.S1[e] = cadd(d.S1[e], cdiv6(cscale(A, sp.dt)));
         }
     }
 }
@@ -218,12 +219,14 @@ __global__ void k_record(DevState d, int slot, int64_t step) {
 // ---------------------------------------------------------------------------
 // Tile commutator on FP64 MFMA.
 //
-// A workgroup of 4 waves holds one N x N tile X (padded to NP, a multiple of
-// 16) and h in LDS as separate re/im planes and computes O = h.X - X.h with
-// v_mfma_f64_16x16x4f64: 4 real MFMAs per complex product.  Output 16x16
-// blocks are dealt round-robin to the 4 waves.  MFMA operand layout
-// (16x16x4): A[m][k] in lane m + 16k, B[k][n] in lane n + 16k; result lane
-// holds D[4*(lane/16) + j][lane%16], j = 0..3.
+// Each tile is an N x N slice X of a G2 tensor (padded to NP, a multiple of
+// 16); the kernel computes O = h.X - X.h with v_mfma_f64_16x16x4f64, four real
+// MFMAs per complex product, and 16x16 output blocks dealt round-robin to the
+// waves.  Workgroups are persistent: h stays in LDS for the whole launch, and
+// while a tile is multiplied the next tile's elements are already loading into
+// registers.  MFMA operand layout (16x16x4 f64): A[m][k] in lane m + 16k,
+// B[k][n] in lane n + 16k; result register j of lane l holds
+// D[4j + l/16][l%16].
 template <int NP>
 struct TileLds {
     static constexpr int P = NP + 1;  // padded pitch (doubles)
@@ -234,47 +237,72 @@ struct TileLds {
 };
 
 template <int NP>
-struct TileShape {
+struct TileCfg {
+    static constexpr int NWAVE = NP >= 48 ? 8 : 4;
+    static constexpr int NTHR = NWAVE * 64;
     static constexpr int NT = NP / 16;
     static constexpr int TT = NT * NT;
-    static constexpr int PER_WAVE = (TT + 3) / 4;
+    static constexpr int PW = (TT + NWAVE - 1) / NWAVE;  // output blocks per wave
+    static constexpr int PF = (NP * NP + NTHR - 1) / NTHR;  // prefetched elements per thread
 };
 
 template <int NP>
-__device__ __forceinline__ void tile_mfma_fp64(const TileLds<NP>& L, int hmode, d4 (&ar)[TileShape<NP>::PER_WAVE],
-                                               d4 (&ai)[TileShape<NP>::PER_WAVE], int wave, int lane) {
+__device__ __forceinline__ void tile_mfma_fp64(const TileLds<NP>& L, int hmode, d4 (&ar)[TileCfg<NP>::PW],
+                                               d4 (&ai)[TileCfg<NP>::PW], int wave, int lane) {
     constexpr int P = TileLds<NP>::P;
-    constexpr int NT = TileShape<NP>::NT, TT = TileShape<NP>::TT, PW = TileShape<NP>::PER_WAVE;
+    constexpr int NT = TileCfg<NP>::NT, TT = TileCfg<NP>::TT, PW = TileCfg<NP>::PW, NW = TileCfg<NP>::NWAVE;
     const int r = lane & 15, kq = lane >> 4;
 #pragma unroll
     for (int u = 0; u < PW; ++u) {
         ar[u] = d4{0.0, 0.0, 0.0, 0.0};
         ai[u] = d4{0.0, 0.0, 0.0, 0.0};
     }
-    for (int k0 = 0; k0 < NP; k0 += 4) {
-        const int kk = k0 + kq;
+    if (hmode & 1) {  // + h.X : A = h rows, B = X columns
+        for (int k0 = 0; k0 < NP; k0 += 4) {
+            const int kk = k0 + kq;
+            double a_r[PW], a_i[PW], b_r[PW], b_i[PW];
 #pragma unroll
-        for (int u = 0; u < PW; ++u) {
-            const int t = wave + 4 * u;
-            if (t < TT) {
+            for (int u = 0; u < PW; ++u) {
+                const int t = wave + NW * u;
                 const int mm = (t / NT) * 16 + r, nn = (t % NT) * 16 + r;
-                if (hmode & 1) {  // + h.X
-                    const double a_r = L.hr[mm * P + kk], a_i = L.hi[mm * P + kk];
-                    const double b_r = L.xr[kk * P + nn], b_i = L.xi[kk * P + nn];
-                    ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r, b_r, ar[u], 0, 0, 0);
-                    ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_i, b_i, ar[u], 0, 0, 0);
-                    ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r, b_i, ai[u], 0, 0, 0);
-                    ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i, b_r, ai[u], 0, 0, 0);
-                }
-                if (hmode & 2) {  // - X.h
-                    const double a_r = L.xr[mm * P + kk], a_i = L.xi[mm * P + kk];
-                    const double b_r = L.hr[kk * P + nn], b_i = L.hi[kk * P + nn];
-                    ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_r, b_r, ar[u], 0, 0, 0);
-                    ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i, b_i, ar[u], 0, 0, 0);
-                    ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_r, b_i, ai[u], 0, 0, 0);
-                    ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_i, b_r, ai[u], 0, 0, 0);
-                }
+                const bool on = t < TT;
+                a_r[u] = on ? L.hr[mm * P + kk] : 0.0;
+                a_i[u] = on ? L.hi[mm * P + kk] : 0.0;
+                b_r[u] = on ? L.xr[kk * P + nn] : 0.0;
+                b_i[u] = on ? L.xi[kk * P + nn] : 0.0;
             }
+#pragma unroll
+            for (int u = 0; u < PW; ++u) ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[u], b_r[u], ar[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < PW; ++u) ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[u], b_i[u], ai[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < PW; ++u) ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_i[u], b_i[u], ar[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < PW; ++u) ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[u], b_r[u], ai[u], 0, 0, 0);
+        }
+    }
+    if (hmode & 2) {  // - X.h : A = X rows, B = h columns
+        for (int k0 = 0; k0 < NP; k0 += 4) {
+            const int kk = k0 + kq;
+            double a_r[PW], a_i[PW], b_r[PW], b_i[PW];
+#pragma unroll
+            for (int u = 0; u < PW; ++u) {
+                const int t = wave + NW * u;
+                const int mm = (t / NT) * 16 + r, nn = (t % NT) * 16 + r;
+                const bool on = t < TT;
+                a_r[u] = on ? L.xr[mm * P + kk] : 0.0;
+                a_i[u] = on ? L.xi[mm * P + kk] : 0.0;
+                b_r[u] = on ? L.hr[kk * P + nn] : 0.0;
+                b_i[u] = on ? L.hi[kk * P + nn] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < PW; ++u) ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_r[u], b_r[u], ar[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < PW; ++u) ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_r[u], b_i[u], ai[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < PW; ++u) ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[u], b_i[u], ar[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < PW; ++u) ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_i[u], b_r[u], ai[u], 0, 0, 0);
         }
     }
 }
@@ -304,32 +332,41 @@ struct TileArgs {
     double wk, dt;
     int first, last;
     int N;
+    int nspin;
     const int* fail;
 };
 
-// Maps the linear block index to a tile so that each XCD (blocks are dealt
-// round-robin over the 8 XCDs) works through a contiguous range of tiles:
-// neighbouring strided tiles share cache lines in that XCD's L2.
-__device__ __forceinline__ int tile_of_block(int b, int ntiles) {
-    if ((ntiles & 7) != 0) return b;
-    const int per = ntiles >> 3;
-    return (b & 7) * per + (b >> 3);
-}
+// Persistent-tile schedule.  Workgroups are dealt round-robin to the 8 XCDs;
+// each XCD owns a contiguous range of the tile list and its workgroups step
+// through it together, so tiles that are neighbours in memory (consecutive
+// q2 of the strided pass share cache lines) are in flight on one L2 at once.
+struct TileSched {
+    int total, per_xcd, wg_per_xcd, xcd, local;
+    bool xcd_mode;
+    __device__ TileSched(int total_, int grid, int b) : total(total_) {
+        xcd_mode = (total % 8 == 0) && (grid % 8 == 0);
+        xcd = b & 7;
+        local = b >> 3;
+        per_xcd = total >> 3;
+        wg_per_xcd = grid >> 3;
+    }
+    // tile of iteration `it` for this workgroup, or -1 when done
+    __device__ int tile(int it, int b, int grid) const {
+        if (xcd_mode) {
+            const int i = it * wg_per_xcd + local;
+            return i < per_xcd ? xcd * per_xcd + i : -1;
+        }
+        const int i = b + it * grid;
+        return i < total ? i : -1;
+    }
+};
 
-template <int NP, int MODE>
-__global__ __launch_bounds__(256) void k_tile(TileArgs a) {
-    if (failed(a.fail)) return;
-    __shared__ TileLds<NP> L;
-    constexpr int P = TileLds<NP>::P;
-    constexpr int NT = TileShape<NP>::NT, TT = TileShape<NP>::TT, PW = TileShape<NP>::PER_WAVE;
-    const int N = a.N;
+__device__ __forceinline__ void tile_geometry(int tiling, int N, int tile, int64_t& base, int64_t& sm, int64_t& sn,
+                                              int& t0, int& t1) {
     const int64_t N2 = (int64_t)N * N, N3 = N2 * N;
-    const int s = blockIdx.y;
-    const int tile = tile_of_block(blockIdx.x, N * N);
-    const int t0 = tile / N, t1 = tile % N;
-    // element (m, n) of the tile lives at base + m*sm + n*sn
-    int64_t base, sm, sn;
-    if (a.tiling == 0) {  // strided: fixed (p2, q2) = (t0, t1); m = p1, n = q1
+    t0 = tile / N;
+    t1 = tile % N;
+    if (tiling == 0) {  // strided: fixed (p2, q2) = (t0, t1); m = p1, n = q1
         base = (int64_t)t0 * N2 + t1;
         sm = N3;
         sn = N;
@@ -338,118 +375,195 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
         sm = N2;
         sn = 1;
     }
+}
+
+template <int NP>
+__device__ __forceinline__ void tile_prefetch(const TileArgs& a, int gt, double2 (&pre)[TileCfg<NP>::PF]) {
+    constexpr int NTHR = TileCfg<NP>::NTHR, PF = TileCfg<NP>::PF;
+    const int N = a.N;
+    const int s = gt / (N * N);
+    int64_t base, sm, sn;
+    int t0, t1;
+    tile_geometry(a.tiling, N, gt % (N * N), base, sm, sn, t0, t1);
     const double2* S = a.S[s];
     const double2* Kp = a.Kp[s];
-    for (int e = threadIdx.x; e < NP * NP; e += 256) {
-        const int m = e / NP, n = e % NP;
-        double2 y = make_double2(0.0, 0.0), hv = make_double2(0.0, 0.0);
-        if (m < N && n < N) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+        const int e = threadIdx.x + NTHR * i;
+        double2 y = make_double2(0.0, 0.0);
+        if (e < N * N) {
+            const int m = e / N, n = e % N;
             const int64_t off = base + m * sm + n * sn;
             y = S[off];
             if (Kp) y = cadd(y, cscale(Kp[off], a.c));
-            hv = a.h[m * N + n];
         }
-        L.xr[m * P + n] = y.x;
-        L.xi[m * P + n] = y.y;
+        pre[i] = y;
+    }
+}
+
+template <int NP, int MODE>
+__global__ __launch_bounds__(TileCfg<NP>::NTHR) void k_tile(TileArgs a) {
+    if (failed(a.fail)) return;
+    __shared__ TileLds<NP> L;
+    constexpr int P = TileLds<NP>::P;
+    constexpr int NT = TileCfg<NP>::NT, TT = TileCfg<NP>::TT, PW = TileCfg<NP>::PW, NW = TileCfg<NP>::NWAVE;
+    constexpr int NTHR = TileCfg<NP>::NTHR, PF = TileCfg<NP>::PF;
+    const int N = a.N;
+    const int64_t N2 = (int64_t)N * N;
+    const int total = a.nspin * N * N;
+    const TileSched sched(total, gridDim.x, blockIdx.x);
+    // h (shared by every tile of the launch) and the zero padding of X
+    for (int e = threadIdx.x; e < NP * NP; e += NTHR) {
+        const int m = e / NP, n = e % NP;
+        double2 hv = make_double2(0.0, 0.0);
+        if (m < N && n < N) hv = a.h[m * N + n];
         L.hr[m * P + n] = hv.x;
         L.hi[m * P + n] = hv.y;
+        L.xr[m * P + n] = 0.0;
+        L.xi[m * P + n] = 0.0;
     }
-    __syncthreads();
+    int gt = sched.tile(0, blockIdx.x, gridDim.x);
+    double2 pre[PF];
+    if (gt >= 0) tile_prefetch<NP>(a, gt, pre);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    d4 ar[PW], ai[PW];
-    tile_mfma_fp64<NP>(L, a.hmode, ar, ai, wave, lane);
-
-    if (MODE == 0) {
-        if (a.D[s] && a.tiling == 0 && t0 == t1) {
-            // D[X=p1][Y=q1][Z] with Z = p2 = q2
-            for (int e = threadIdx.x; e < N * N; e += 256) {
+    for (int it = 0; gt >= 0; ++it) {
+        __syncthreads();  // previous tile's readers of X are done
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int e = threadIdx.x + NTHR * i;
+            if (e < N * N) {
                 const int m = e / N, n = e % N;
-                a.D[s][((int64_t)m * N + n) * N + t0] = make_double2(L.xr[m * P + n], L.xi[m * P + n]);
+                L.xr[m * P + n] = pre[i].x;
+                L.xi[m * P + n] = pre[i].y;
             }
         }
-        double2* out = a.out[s];
+        __syncthreads();
+        const int s = gt / (N * N);
+        int64_t base, sm, sn;
+        int t0, t1;
+        tile_geometry(a.tiling, N, gt % (N * N), base, sm, sn, t0, t1);
+        const int next = sched.tile(it + 1, blockIdx.x, gridDim.x);
+        if (next >= 0) tile_prefetch<NP>(a, next, pre);  // overlaps the MFMA loop
+
+        d4 ar[PW], ai[PW];
+        tile_mfma_fp64<NP>(L, a.hmode, ar, ai, wave, lane);
+
+        if (MODE == 0) {
+            if (a.D[s] && a.tiling == 0 && t0 == t1) {
+                // D[X=p1][Y=q1][Z] with Z = p2 = q2
+                for (int e = threadIdx.x; e < N * N; e += NTHR) {
+                    const int m = e / N, n = e % N;
+                    a.D[s][((int64_t)m * N + n) * N + t0] = make_double2(L.xr[m * P + n], L.xi[m * P + n]);
+                }
+            }
+            double2* out = a.out[s];
 #pragma unroll
-        for (int u = 0; u < PW; ++u) {
-            const int t = wave + 4 * u;
-            if (t >= TT) continue;
-            const int n = (t % NT) * 16 + (lane & 15);
+            for (int u = 0; u < PW; ++u) {
+                const int t = wave + NW * u;
+                if (t >= TT) continue;
+                const int n = (t % NT) * 16 + (lane & 15);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int m = (t / NT) * 16 + 4 * (lane >> 4) + j;
-                if (m < N && n < N) {
+                for (int j = 0; j < 4; ++j) {
+                    const int m = (t / NT) * 16 + 4 * j + (lane >> 4);
+                    if (m < N && n < N) {
+                        const int64_t off = base + m * sm + n * sn;
+                        double2 v = make_double2(ar[u][j], ai[u][j]);
+                        if (a.accumulate) v = cadd(out[off], v);
+                        out[off] = v;
+                    }
+                }
+            }
+        } else {
+            // fused right-hand side for tile (p1, q1) = (t0, t1), element (p2, q2) = (m, n):
+            //   comm = R + O
+            //   Psi  = g11 g22 (W[p1][p2] - W[q1][q2]) + i g11 (Q[p1][p2][q2] - Q[q1][p2][q2])
+            //          + i g22 (Qp[p1][q1][p2] - Qp[p1][q1][q2])
+            //   Pi   = g22 (F[p1][q1][p2] - F[p1][q1][q2])
+            //   Pi^+ = conj( g(q1,p1) (Ft[q1][p2][q2] - Ft[p1][p2][q2]) )
+            //   K    = -i (comm + Psi + Pi - Pi^+)
+            const int p1 = t0, q1 = t1;
+            const double2 g11 = a.G1[(int64_t)p1 * N + q1];
+            const double2 gq1p1 = a.G1[(int64_t)q1 * N + p1];
+            const double* W = a.W[s];
+            const double2* Qs = a.Q[s];
+            const double2* Qps = a.Qp[s] + ((int64_t)p1 * N + q1) * N;
+            const double2* Frow = a.F[s] + ((int64_t)p1 * N + q1) * N;
+            const double2* Fts = a.Ft[s];
+            const double2* R = a.R[s];
+            const double2* S = a.S[s];
+#pragma unroll
+            for (int u = 0; u < PW; ++u) {
+                const int t = wave + NW * u;
+                if (t >= TT) continue;
+                const int n = (t % NT) * 16 + (lane & 15);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int m = (t / NT) * 16 + 4 * j + (lane >> 4);
+                    if (m >= N || n >= N) continue;
                     const int64_t off = base + m * sm + n * sn;
-                    double2 v = make_double2(ar[u][j], ai[u][j]);
-                    if (a.accumulate) v = cadd(out[off], v);
-                    out[off] = v;
+                    const int64_t mn = (int64_t)m * N + n;
+                    double2 comm = cadd(R[off], make_double2(ar[u][j], ai[u][j]));
+                    const double2 g22 = a.G1[mn];
+                    double2 psi = cscale(cmul(g11, g22), W[(int64_t)p1 * N + m] - W[(int64_t)q1 * N + n]);
+                    psi = cadd(psi, cmul_i(cmul(g11, csub(Qs[(int64_t)p1 * N2 + mn], Qs[(int64_t)q1 * N2 + mn]))));
+                    psi = cadd(psi, cmul_i(cmul(g22, csub(Qps[m], Qps[n]))));
+                    const double2 pi = cmul(g22, csub(Frow[m], Frow[n]));
+                    const double2 pid =
+                        cconj(cmul(gq1p1, csub(Fts[(int64_t)q1 * N2 + mn], Fts[(int64_t)p1 * N2 + mn])));
+                    const double2 K = cmul_mi(csub(cadd(cadd(comm, psi), pi), pid));
+                    const double2 wK = cscale(K, a.wk);
+                    if (a.first) {
+                        a.A[s][off] = wK;
+                        a.Kout[s][off] = K;
+                    } else if (!a.last) {
+                        a.A[s][off] = cadd(a.A[s][off], wK);
+                        a.Kout[s][off] = K;
+                    } else {
+                        const double2 A = cadd(a.A[s][off], wK);
+                        a.Sout[s][off] = cadd(S[off], cdiv6(cscale(A, a.dt)));
+                    }
                 }
             }
         }
-    } else {
-        // fused right-hand side for tile (p1, q1) = (t0, t1), element (p2, q2) = (m, n):
-        //   comm = R + O
-        //   Psi  = g11 g22 (W[p1][p2] - W[q1][q2]) + i g11 (Q[p1][p2][q2] - Q[q1][p2][q2])
-        //          + i g22 (Qp[p1][q1][p2] - Qp[p1][q1][q2])
-        //   Pi   = g22 (F[p1][q1][p2] - F[p1][q1][q2])
-        //   Pi^+ = conj( g(q1,p1) (Ft[q1][p2][q2] - Ft[p1][p2][q2]) )
-        //   K    = -i (comm + Psi + Pi - Pi^+)
-        const int p1 = t0, q1 = t1;
-        const double2 g11 = a.G1[(int64_t)p1 * N + q1];
-        const double2 gq1p1 = a.G1[(int64_t)q1 * N + p1];
-        const double* W = a.W[s];
-        const double2* Qs = a.Q[s];
-        const double2* Qps = a.Qp[s] + ((int64_t)p1 * N + q1) * N;
-        const double2* Frow = a.F[s] + ((int64_t)p1 * N + q1) * N;
-        const double2* Fts = a.Ft[s];
-        const double2* R = a.R[s];
-#pragma unroll
-        for (int u = 0; u < PW; ++u) {
-            const int t = wave + 4 * u;
-            if (t >= TT) continue;
-            const int n = (t % NT) * 16 + (lane & 15);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int m = (t / NT) * 16 + 4 * (lane >> 4) + j;
-                if (m >= N || n >= N) continue;
-                const int64_t off = base + m * sm + n * sn;
-                const int64_t mn = (int64_t)m * N + n;
-                double2 comm = cadd(R[off], make_double2(ar[u][j], ai[u][j]));
-                const double2 g22 = a.G1[mn];
-                double2 psi = cscale(cmul(g11, g22), W[(int64_t)p1 * N + m] - W[(int64_t)q1 * N + n]);
-                psi = cadd(psi, cmul_i(cmul(g11, csub(Qs[(int64_t)p1 * N2 + mn], Qs[(int64_t)q1 * N2 + mn]))));
-                psi = cadd(psi, cmul_i(cmul(g22, csub(Qps[m], Qps[n]))));
-                const double2 pi = cmul(g22, csub(Frow[m], Frow[n]));
-                const double2 pid = cconj(cmul(gq1p1, csub(Fts[(int64_t)q1 * N2 + mn], Fts[(int64_t)p1 * N2 + mn])));
-                const double2 K = cmul_mi(csub(cadd(cadd(comm, psi), pi), pid));
-                const double2 wK = cscale(K, a.wk);
-                if (a.first) {
-                    a.A[s][off] = wK;
-                    a.Kout[s][off] = K;
-                } else if (!a.last) {
-                    a.A[s][off] = cadd(a.A[s][off], wK);
-                    a.Kout[s][off] = K;
-                } else {
-                    const double2 A = cadd(a.A[s][off], wK);
-                    a.Sout[s][off] = cadd(S[off], cscale(cscale(A, a.dt), 1.0 / 6.0));
-                }
-            }
-        }
+        gt = next;
     }
+}
+
+int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+template <int NP, int MODE>
+void launch_tile_np(const TileArgs& a, hipStream_t s) {
+    const int total = a.nspin * a.N * a.N;
+    // NP >= 48: one 133 KB workgroup per CU; NP <= 32: several per CU
+    const int per_cu = NP >= 48 ? 1 : (NP == 32 ? 4 : 8);
+    int grid = device_cus() * per_cu;
+    if (grid > total) grid = total;
+    if (grid >= 8) grid &= ~7;
+    hipLaunchKernelGGL((k_tile<NP, MODE>), dim3(grid), dim3(TileCfg<NP>::NTHR), 0, s, a);
 }
 
 template <int MODE>
-void launch_tile(const TileArgs& a, int nspin, hipStream_t s) {
+void launch_tile(TileArgs a, int nspin, hipStream_t s) {
+    a.nspin = nspin;
     const int N = a.N;
-    dim3 grid(N * N, nspin), block(256);
     if (N <= 16)
-        hipLaunchKernelGGL((k_tile<16, MODE>), grid, block, 0, s, a);
+        launch_tile_np<16, MODE>(a, s);
     else if (N <= 32)
-        hipLaunchKernelGGL((k_tile<32, MODE>), grid, block, 0, s, a);
+        launch_tile_np<32, MODE>(a, s);
     else if (N <= 48)
-        hipLaunchKernelGGL((k_tile<48, MODE>), grid, block, 0, s, a);
+        launch_tile_np<48, MODE>(a, s);
     else
-        hipLaunchKernelGGL((k_tile<64, MODE>), grid, block, 0, s, a);
+        launch_tile_np<64, MODE>(a, s);
 }
-
 // ---------------------------------------------------------------------------
 // Layout permutations between the reference pair order and the natural one.
 __device__ __forceinline__ int64_t ref_pair(int p1, int p2, int Ns, int Nb) {

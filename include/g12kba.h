@@ -138,6 +138,9 @@ int g12_get_time(g12_ctx* ctx, double* t, int64_t* steps);
 int g12_timing_enable(g12_ctx* ctx, int on);
 int g12_timing_get(g12_ctx* ctx, int which, int64_t* launches, double* total_ms);
 
+/* Waits for all work on the library's GPU (device-wide synchronisation). */
+int g12_synchronize(void);
+
 /* Device footprint of a context in bytes. */
 int g12_memory_bytes(g12_ctx* ctx, int64_t* bytes);
 
