@@ -19,6 +19,7 @@
 #include "g12_internal.h"
 
 #include <cmath>
+#include <cstdlib>
 
 namespace g12 {
 namespace {
@@ -90,20 +91,57 @@ __device__ inline double2 hf_element(const double2* G, int X, int Y, int Ns, int
 
 // ---------------------------------------------------------------------------
 // Stage set-up for G1 (one workgroup): Y1 = S1 + c K1, h = h_HF[Y1],
-// W tables.
+// W tables.  The Hartree sums read the diagonal of Y1 from LDS.
+__device__ inline double2 hf_element_diag(const double2* G, const double2* diag, int X, int Y, int Ns, int Nb,
+                                          const StageParams& sp) {
+    const int N = Ns * Nb;
+    const int sx = X % Ns, ox = X / Ns, sy = Y % Ns, oy = Y / Ns;
+    double2 v = make_double2(0.0, 0.0);
+    if (X == Y) {
+        v.x += sp.eps[ox];
+        if (sx < sp.Nq) v.x += sp.wq;
+        double2 acc = cscale(diag[X], sp.U);  // -i U g_XX
+        double2 lr = make_double2(0.0, 0.0);
+        for (int Z = 0; Z < N; ++Z) {
+            if (Z == X) continue;
+            const double w = w_xy(X, Z, Ns, sp);
+            if (w != 0.0) lr = cadd(lr, cscale(diag[Z], w));
+        }
+        acc = cadd(acc, cscale(lr, 2.0));
+        v = cadd(v, cmul_mi(acc));
+    } else {
+        int ds = sx - sy;
+        ds = ds < 0 ? -ds : ds;
+        if (ox == oy && ds == 1) v.x -= sp.t1;
+        if (sx == sy) {
+            int dorb = ox - oy;
+            dorb = dorb < 0 ? -dorb : dorb;
+            if (dorb == 1) v.x -= sp.t2;
+            v = cadd(v, cmul_i(cscale(G[(int64_t)Y * N + X], sp.U)));
+        } else if (ds <= sp.nV) {
+            v = cadd(v, cmul_i(cscale(G[(int64_t)X * N + Y], sp.V[ds - 1])));
+        }
+    }
+    return v;
+}
+
 __global__ __launch_bounds__(1024) void k_stage_g1(DevState d, StageParams sp) {
     if (failed(d.fail)) return;
+    __shared__ double2 diag[G12_MAX_NB * 64];
     const int N = d.N;
     const int n2 = N * N;
     for (int e = threadIdx.x; e < n2; e += blockDim.x) {
         double2 y = d.S1[e];
         if (!sp.first) y = cadd(y, cscale(d.K1[e], sp.c));
         d.Y1[e] = y;
+        if (e / N == e % N) diag[e / N] = y;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < n2; e += blockDim.x) {
         const int X = e / N, Y = e % N;
-        d.h[e] = hf_element(d.Y1, X, Y, d.Ns, d.Nb, sp);
+        const double2 y = d.Y1[e];
+        d.h[e] = hf_element_diag(d.Y1, diag, X, Y, d.Ns, d.Nb, sp);
+        (void)y;
         const double w = w_xy(X, Y, d.Ns, sp);
         d.Wxy[e] = w;
         d.Wxx[e] = (X == Y) ? 0.0 : w;
@@ -116,15 +154,22 @@ __global__ __launch_bounds__(1024) void k_stage_g1(DevState d, StageParams sp) {
 // threads over Y.
 __global__ __launch_bounds__(64) void k_q_tables(DevState d) {
     if (failed(d.fail)) return;
+    __shared__ double2 u[G12_MAX_NB * 64];
+    __shared__ double wz[G12_MAX_NB * 64];
     const int N = d.N;
     const int Z = blockIdx.x / N, X = blockIdx.x % N;
     const double2* G = d.Y1;
+    for (int w = threadIdx.x; w < N; w += blockDim.x) {
+        wz[w] = d.Wxy[(int64_t)w * N + Z];  // W_xy is symmetric
+        u[w] = G[(int64_t)X * N + w];
+    }
+    __syncthreads();
     for (int Y = threadIdx.x; Y < N; Y += blockDim.x) {
         double2 axy = make_double2(0.0, 0.0), axx = make_double2(0.0, 0.0);
         for (int w = 0; w < N; ++w) {
-            const double wv = d.Wxy[(int64_t)Z * N + w];
+            const double wv = wz[w];
             if (wv == 0.0) continue;
-            const double2 t = cscale(cmul(G[(int64_t)X * N + w], G[(int64_t)w * N + Y]), wv);
+            const double2 t = cscale(cmul(u[w], G[(int64_t)w * N + Y]), wv);
             axy = cadd(axy, t);
             if (w != Z) axx = cadd(axx, t);
         }
@@ -144,14 +189,18 @@ __global__ __launch_bounds__(64) void k_q_tables(DevState d) {
 // (collision_int, collision_integral.h:27-69).  One block per (X, Y).
 __global__ __launch_bounds__(64) void k_f_tables(DevState d) {
     if (failed(d.fail)) return;
+    __shared__ double2 dxx[G12_MAX_NB * 64], dxy[G12_MAX_NB * 64];
     const int N = d.N;
     const int X = blockIdx.x / N, Y = blockIdx.x % N;
-    const double2* dxx = d.D[0] + ((int64_t)X * N + Y) * N;
-    const double2* dxy = d.D[1] + ((int64_t)X * N + Y) * N;
+    for (int w = threadIdx.x; w < N; w += blockDim.x) {
+        dxx[w] = d.D[0][((int64_t)X * N + Y) * N + w];
+        dxy[w] = d.D[1][((int64_t)X * N + Y) * N + w];
+    }
+    __syncthreads();
     for (int Z = threadIdx.x; Z < N; Z += blockDim.x) {
         double2 fxx = make_double2(0.0, 0.0), fxy = make_double2(0.0, 0.0);
         for (int w = 0; w < N; ++w) {
-            const double a = d.Wxy[(int64_t)Z * N + w];
+            const double a = d.Wxy[(int64_t)w * N + Z];  // W_xy is symmetric
             if (a == 0.0) continue;
             const double b = (w == Z) ? 0.0 : a;
             const double2 vxx = dxx[w], vxy = dxy[w];
@@ -170,18 +219,25 @@ __global__ __launch_bounds__(64) void k_f_tables(DevState d) {
     }
 }
 
-// G1 right-hand side (one workgroup): comm = [h, Y1], K = -i(comm + I + I^dag),
-// RK accumulation (G1_step, G1_G2_t_step.h:24-47) and the final update.
-__global__ __launch_bounds__(1024) void k_g1_rhs(DevState d, StageParams sp) {
+// G1 right-hand side: comm = [h, Y1], K = -i(comm + I + I^dag), RK
+// accumulation (G1_step, G1_G2_t_step.h:24-47) and the final update.  One
+// block per row X; the rows X of h and Y1 are staged in LDS.
+__global__ __launch_bounds__(64) void k_g1_rhs(DevState d, StageParams sp) {
     if (failed(d.fail)) return;
+    __shared__ double2 hrow[G12_MAX_NB * 64], grow[G12_MAX_NB * 64];
     const int N = d.N;
-    const int n2 = N * N;
-    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
-        const int X = e / N, Y = e % N;
+    const int X = blockIdx.x;
+    for (int z = threadIdx.x; z < N; z += blockDim.x) {
+        hrow[z] = d.h[(int64_t)X * N + z];
+        grow[z] = d.Y1[(int64_t)X * N + z];
+    }
+    __syncthreads();
+    for (int Y = threadIdx.x; Y < N; Y += blockDim.x) {
+        const int64_t e = (int64_t)X * N + Y;
         double2 hg = make_double2(0.0, 0.0), gh = make_double2(0.0, 0.0);
         for (int z = 0; z < N; ++z) {
-            hg = cadd(hg, cmul(d.h[(int64_t)X * N + z], d.Y1[(int64_t)z * N + Y]));
-            gh = cadd(gh, cmul(d.Y1[(int64_t)X * N + z], d.h[(int64_t)z * N + Y]));
+            hg = cadd(hg, cmul(hrow[z], d.Y1[(int64_t)z * N + Y]));
+            gh = cadd(gh, cmul(grow[z], d.h[(int64_t)z * N + Y]));
         }
         double2 v = csub(hg, gh);
         if (sp.corr) v = cadd(cadd(v, d.I[e]), cconj(d.I[(int64_t)Y * N + X]));
@@ -236,9 +292,9 @@ struct TileLds {
     double hi[NP * P];
 };
 
-template <int NP>
+template <int NP, int NW_>
 struct TileCfg {
-    static constexpr int NWAVE = NP >= 48 ? 8 : 4;
+    static constexpr int NWAVE = NW_;
     static constexpr int NTHR = NWAVE * 64;
     static constexpr int NT = NP / 16;
     static constexpr int TT = NT * NT;
@@ -246,65 +302,80 @@ struct TileCfg {
     static constexpr int PF = (NP * NP + NTHR - 1) / NTHR;  // prefetched elements per thread
 };
 
-template <int NP>
-__device__ __forceinline__ void tile_mfma_fp64(const TileLds<NP>& L, int hmode, d4 (&ar)[TileCfg<NP>::PW],
-                                               d4 (&ai)[TileCfg<NP>::PW], int wave, int lane) {
+template <int NP, int NW>
+__device__ __forceinline__ void load_frags(const double* __restrict__ ar_, const double* __restrict__ ai_,
+                                           const double* __restrict__ br_, const double* __restrict__ bi_, int kk,
+                                           int wave, int r, double (&a_r)[TileCfg<NP, NW>::PW],
+                                           double (&a_i)[TileCfg<NP, NW>::PW], double (&b_r)[TileCfg<NP, NW>::PW],
+                                           double (&b_i)[TileCfg<NP, NW>::PW]) {
     constexpr int P = TileLds<NP>::P;
-    constexpr int NT = TileCfg<NP>::NT, TT = TileCfg<NP>::TT, PW = TileCfg<NP>::PW, NW = TileCfg<NP>::NWAVE;
+    constexpr int NT = TileCfg<NP, NW>::NT, TT = TileCfg<NP, NW>::TT, PW = TileCfg<NP, NW>::PW;
+    constexpr bool FULL = (TT % NW) == 0;  // every wave owns PW blocks
+#pragma unroll
+    for (int u = 0; u < PW; ++u) {
+        const int t = wave + NW * u;
+        const bool on = FULL || t < TT;
+        const int mm = (t / NT) * 16 + r, nn = (t % NT) * 16 + r;
+        a_r[u] = on ? ar_[mm * P + kk] : 0.0;
+        a_i[u] = on ? ai_[mm * P + kk] : 0.0;
+        b_r[u] = on ? br_[kk * P + nn] : 0.0;
+        b_i[u] = on ? bi_[kk * P + nn] : 0.0;
+    }
+}
+
+// acc += sa * A.B for one k-step of a 16x16x4 complex product on 4 real MFMAs
+// (sa = +1: h.X, sa = -1: -X.h); independent accumulators are interleaved so
+// consecutive MFMAs never wait on each other.
+template <int PW, bool NEG>
+__device__ __forceinline__ void mfma_step(d4 (&ar)[PW], d4 (&ai)[PW], const double (&a_r)[PW],
+                                          const double (&a_i)[PW], const double (&b_r)[PW],
+                                          const double (&b_i)[PW]) {
+#pragma unroll
+    for (int u = 0; u < PW; ++u)
+        ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(NEG ? -a_r[u] : a_r[u], b_r[u], ar[u], 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < PW; ++u)
+        ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(NEG ? -a_r[u] : a_r[u], b_i[u], ai[u], 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < PW; ++u)
+        ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(NEG ? a_i[u] : -a_i[u], b_i[u], ar[u], 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < PW; ++u)
+        ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(NEG ? -a_i[u] : a_i[u], b_r[u], ai[u], 0, 0, 0);
+}
+
+template <int NP, int NW, bool NEG>
+__device__ __forceinline__ void tile_product(const double* ar_, const double* ai_, const double* br_,
+                                             const double* bi_, d4 (&ar)[TileCfg<NP, NW>::PW],
+                                             d4 (&ai)[TileCfg<NP, NW>::PW], int wave, int lane) {
+    constexpr int PW = TileCfg<NP, NW>::PW;
     const int r = lane & 15, kq = lane >> 4;
+    double a_r[PW], a_i[PW], b_r[PW], b_i[PW];
+    double n_r[PW], n_i[PW], m_r[PW], m_i[PW];
+    load_frags<NP, NW>(ar_, ai_, br_, bi_, kq, wave, r, a_r, a_i, b_r, b_i);
+#pragma unroll
+    for (int k0 = 0; k0 < NP; k0 += 8) {
+        // operands of step k0+4 load while step k0 multiplies, and so on
+        if (k0 + 4 < NP) load_frags<NP, NW>(ar_, ai_, br_, bi_, k0 + 4 + kq, wave, r, n_r, n_i, m_r, m_i);
+        mfma_step<PW, NEG>(ar, ai, a_r, a_i, b_r, b_i);
+        if (k0 + 4 < NP) {
+            if (k0 + 8 < NP) load_frags<NP, NW>(ar_, ai_, br_, bi_, k0 + 8 + kq, wave, r, a_r, a_i, b_r, b_i);
+            mfma_step<PW, NEG>(ar, ai, n_r, n_i, m_r, m_i);
+        }
+    }
+}
+
+template <int NP, int NW>
+__device__ __forceinline__ void tile_mfma_fp64(const TileLds<NP>& L, int hmode, d4 (&ar)[TileCfg<NP, NW>::PW],
+                                               d4 (&ai)[TileCfg<NP, NW>::PW], int wave, int lane) {
+    constexpr int PW = TileCfg<NP, NW>::PW;
 #pragma unroll
     for (int u = 0; u < PW; ++u) {
         ar[u] = d4{0.0, 0.0, 0.0, 0.0};
         ai[u] = d4{0.0, 0.0, 0.0, 0.0};
     }
-    if (hmode & 1) {  // + h.X : A = h rows, B = X columns
-        for (int k0 = 0; k0 < NP; k0 += 4) {
-            const int kk = k0 + kq;
-            double a_r[PW], a_i[PW], b_r[PW], b_i[PW];
-#pragma unroll
-            for (int u = 0; u < PW; ++u) {
-                const int t = wave + NW * u;
-                const int mm = (t / NT) * 16 + r, nn = (t % NT) * 16 + r;
-                const bool on = t < TT;
-                a_r[u] = on ? L.hr[mm * P + kk] : 0.0;
-                a_i[u] = on ? L.hi[mm * P + kk] : 0.0;
-                b_r[u] = on ? L.xr[kk * P + nn] : 0.0;
-                b_i[u] = on ? L.xi[kk * P + nn] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < PW; ++u) ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[u], b_r[u], ar[u], 0, 0, 0);
-#pragma unroll
-            for (int u = 0; u < PW; ++u) ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[u], b_i[u], ai[u], 0, 0, 0);
-#pragma unroll
-            for (int u = 0; u < PW; ++u) ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_i[u], b_i[u], ar[u], 0, 0, 0);
-#pragma unroll
-            for (int u = 0; u < PW; ++u) ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[u], b_r[u], ai[u], 0, 0, 0);
-        }
-    }
-    if (hmode & 2) {  // - X.h : A = X rows, B = h columns
-        for (int k0 = 0; k0 < NP; k0 += 4) {
-            const int kk = k0 + kq;
-            double a_r[PW], a_i[PW], b_r[PW], b_i[PW];
-#pragma unroll
-            for (int u = 0; u < PW; ++u) {
-                const int t = wave + NW * u;
-                const int mm = (t / NT) * 16 + r, nn = (t % NT) * 16 + r;
-                const bool on = t < TT;
-                a_r[u] = on ? L.xr[mm * P + kk] : 0.0;
-                a_i[u] = on ? L.xi[mm * P + kk] : 0.0;
-                b_r[u] = on ? L.hr[kk * P + nn] : 0.0;
-                b_i[u] = on ? L.hi[kk * P + nn] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < PW; ++u) ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_r[u], b_r[u], ar[u], 0, 0, 0);
-#pragma unroll
-            for (int u = 0; u < PW; ++u) ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_r[u], b_i[u], ai[u], 0, 0, 0);
-#pragma unroll
-            for (int u = 0; u < PW; ++u) ar[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[u], b_i[u], ar[u], 0, 0, 0);
-#pragma unroll
-            for (int u = 0; u < PW; ++u) ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_i[u], b_r[u], ai[u], 0, 0, 0);
-        }
-    }
+    if (hmode & 1) tile_product<NP, NW, false>(L.hr, L.hi, L.xr, L.xi, ar, ai, wave, lane);  // + h.X
+    if (hmode & 2) tile_product<NP, NW, true>(L.xr, L.xi, L.hr, L.hi, ar, ai, wave, lane);   // - X.h
 }
 
 struct TileArgs {
@@ -377,9 +448,9 @@ __device__ __forceinline__ void tile_geometry(int tiling, int N, int tile, int64
     }
 }
 
-template <int NP>
-__device__ __forceinline__ void tile_prefetch(const TileArgs& a, int gt, double2 (&pre)[TileCfg<NP>::PF]) {
-    constexpr int NTHR = TileCfg<NP>::NTHR, PF = TileCfg<NP>::PF;
+template <int NP, int NW>
+__device__ __forceinline__ void tile_prefetch(const TileArgs& a, int gt, double2 (&pre)[TileCfg<NP, NW>::PF]) {
+    constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
     const int N = a.N;
     const int s = gt / (N * N);
     int64_t base, sm, sn;
@@ -401,13 +472,13 @@ __device__ __forceinline__ void tile_prefetch(const TileArgs& a, int gt, double2
     }
 }
 
-template <int NP, int MODE>
-__global__ __launch_bounds__(TileCfg<NP>::NTHR) void k_tile(TileArgs a) {
+template <int NP, int NW, int MODE>
+__global__ __launch_bounds__(64 * NW) void k_tile(TileArgs a) {
     if (failed(a.fail)) return;
     __shared__ TileLds<NP> L;
     constexpr int P = TileLds<NP>::P;
-    constexpr int NT = TileCfg<NP>::NT, TT = TileCfg<NP>::TT, PW = TileCfg<NP>::PW, NW = TileCfg<NP>::NWAVE;
-    constexpr int NTHR = TileCfg<NP>::NTHR, PF = TileCfg<NP>::PF;
+    constexpr int NT = TileCfg<NP, NW>::NT, TT = TileCfg<NP, NW>::TT, PW = TileCfg<NP, NW>::PW;
+    constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
     const int N = a.N;
     const int64_t N2 = (int64_t)N * N;
     const int total = a.nspin * N * N;
@@ -424,7 +495,7 @@ __global__ __launch_bounds__(TileCfg<NP>::NTHR) void k_tile(TileArgs a) {
     }
     int gt = sched.tile(0, blockIdx.x, gridDim.x);
     double2 pre[PF];
-    if (gt >= 0) tile_prefetch<NP>(a, gt, pre);
+    if (gt >= 0) tile_prefetch<NP, NW>(a, gt, pre);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int it = 0; gt >= 0; ++it) {
         __syncthreads();  // previous tile's readers of X are done
@@ -443,10 +514,10 @@ __global__ __launch_bounds__(TileCfg<NP>::NTHR) void k_tile(TileArgs a) {
         int t0, t1;
         tile_geometry(a.tiling, N, gt % (N * N), base, sm, sn, t0, t1);
         const int next = sched.tile(it + 1, blockIdx.x, gridDim.x);
-        if (next >= 0) tile_prefetch<NP>(a, next, pre);  // overlaps the MFMA loop
+        if (next >= 0) tile_prefetch<NP, NW>(a, next, pre);  // overlaps the MFMA loop
 
         d4 ar[PW], ai[PW];
-        tile_mfma_fp64<NP>(L, a.hmode, ar, ai, wave, lane);
+        tile_mfma_fp64<NP, NW>(L, a.hmode, ar, ai, wave, lane);
 
         if (MODE == 0) {
             if (a.D[s] && a.tiling == 0 && t0 == t1) {
@@ -540,15 +611,39 @@ int device_cus() {
     return cus;
 }
 
-template <int NP, int MODE>
-void launch_tile_np(const TileArgs& a, hipStream_t s) {
+// Waves per tile workgroup: G12KBA_TILE_WAVES (4, 8 or 16) overrides the
+// built-in choice for tuning sweeps.
+int tile_waves(int NP) {
+    static int env = -1;
+    if (env < 0) {
+        const char* v = std::getenv("G12KBA_TILE_WAVES");
+        env = v ? std::atoi(v) : 0;
+    }
+    if (env == 4 || env == 8 || env == 16) return env;
+    return NP >= 48 ? 16 : 4;  // measured best on MI355X (FP64, N = 32/48/64)
+}
+
+template <int NP, int NW, int MODE>
+void launch_tile_cfg(const TileArgs& a, hipStream_t s) {
     const int total = a.nspin * a.N * a.N;
-    // NP >= 48: one 133 KB workgroup per CU; NP <= 32: several per CU
-    const int per_cu = NP >= 48 ? 1 : (NP == 32 ? 4 : 8);
+    constexpr int lds = (int)sizeof(TileLds<NP>);
+    int per_cu = (160 * 1024) / lds;
+    const int by_waves = 32 / NW;  // at most 32 waves per CU
+    if (per_cu > by_waves) per_cu = by_waves;
+    if (per_cu < 1) per_cu = 1;
     int grid = device_cus() * per_cu;
     if (grid > total) grid = total;
     if (grid >= 8) grid &= ~7;
-    hipLaunchKernelGGL((k_tile<NP, MODE>), dim3(grid), dim3(TileCfg<NP>::NTHR), 0, s, a);
+    hipLaunchKernelGGL((k_tile<NP, NW, MODE>), dim3(grid), dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
+}
+
+template <int NP, int MODE>
+void launch_tile_np(const TileArgs& a, hipStream_t s) {
+    switch (tile_waves(NP)) {
+        case 16: launch_tile_cfg<NP, 16, MODE>(a, s); break;
+        case 8: launch_tile_cfg<NP, 8, MODE>(a, s); break;
+        default: launch_tile_cfg<NP, 4, MODE>(a, s); break;
+    }
 }
 
 template <int MODE>
@@ -564,6 +659,7 @@ void launch_tile(TileArgs a, int nspin, hipStream_t s) {
     else
         launch_tile_np<64, MODE>(a, s);
 }
+
 // ---------------------------------------------------------------------------
 // Layout permutations between the reference pair order and the natural one.
 __device__ __forceinline__ int64_t ref_pair(int p1, int p2, int Ns, int Nb) {
@@ -772,7 +868,7 @@ void launch_tables(const DevState& d, const StageParams& sp, hipStream_t s) {
 }
 
 void launch_g1_rhs(const DevState& d, const StageParams& sp, hipStream_t s) {
-    hipLaunchKernelGGL(k_g1_rhs, dim3(1), dim3(1024), 0, s, d, sp);
+    hipLaunchKernelGGL(k_g1_rhs, dim3(d.N), dim3(64), 0, s, d, sp);
 }
 
 void launch_pass_fused(const DevState& d, const StageParams& sp, int precision, hipStream_t s) {
