@@ -103,11 +103,15 @@ def hf(G1, Ns, Nb, U, t1, t2, w, eps, V, quench_extent):
 
 
 def comm2(T, hx, hy):
-    """[H2, M] in the natural layout with row Hamiltonian hx, column hy."""
-    out = np.einsum("az,zbcd->abcd", hx, T, optimize=True)
-    out += np.einsum("bz,azcd->abcd", hx, T, optimize=True)
-    out -= np.einsum("abcz,zd->abcd", T, hy, optimize=True)
-    out -= np.einsum("abzd,zc->abcd", T, hy, optimize=True)
+    """[H2, M] in the natural layout with row Hamiltonian hx, column hy.
+
+    Each of the four terms is one (batched) matrix product on a reshaped view:
+    hx on p1, hx on p2, hy on q1 (from the right, transposed), hy on q2."""
+    N = T.shape[0]
+    out = (hx @ T.reshape(N, N ** 3)).reshape(T.shape)
+    out += np.matmul(hx, T.reshape(N, N, N * N)).reshape(T.shape)
+    out -= (T.reshape(N ** 3, N) @ hy).reshape(T.shape)
+    out -= np.matmul(hy.T, T.reshape(N * N, N, N)).reshape(T.shape)
     return out
 
 

@@ -75,6 +75,9 @@ class Oracle:
                               ctypes.POINTER(ctypes.c_int64)]
         L.orc_rk4_fixed_step.argtypes = [ctypes.POINTER(OrcParams), p, p, p, d]
         L.orc_rk4_partial_step.argtypes = [ctypes.POINTER(OrcParams), p, p, p, d, i]
+        L.orcf_run.argtypes = [ctypes.POINTER(OrcParams), p, p, p, i, ctypes.POINTER(i),
+                               ctypes.POINTER(ctypes.c_int64), p, ctypes.c_int64, i, p,
+                               ctypes.c_int64]
 
     # --- routines (same argument roles and order as the reference) ---
     def hf(self, h, G1, Ns, Nb, U, t1, t2, w, eps, V, quench_extent):
@@ -199,6 +202,27 @@ class Oracle:
         G1, Gxx, Gxy = _c(G1).copy(), _c(Gxx).copy(), _c(Gxy).copy()
         self.L.orc_rk4_partial_step(ctypes.byref(p), _ptr(G1), _ptr(Gxx), _ptr(Gxy), dt, nstages)
         return G1, Gxx, Gxy
+
+    def fast_run(self, cfg, G1, Gxx, Gxy, V_profile=None, eps=None, run_as=True, traj_every=0,
+                 traj_cap=0, record_diag=False):
+        """Factorised C++ oracle (oracle/g12_fast.cpp): AS (optional) + RK4.
+
+        Returns dict(rc, G1, Gxx, Gxy, as_steps, rk_steps, traj, diag) with diag
+        the Im G1[i,i] before every RK4 iteration (if record_diag)."""
+        p, keep = self.params(cfg, V_profile, eps)
+        N = cfg["Ns"] * cfg["Nb"]
+        G1, Gxx, Gxy = _c(G1).copy(), _c(Gxx).copy(), _c(Gxy).copy()
+        traj = np.zeros((max(traj_cap, 1), N, N), np.complex128)
+        cap = cfg["time_steps"] + 2
+        diag = np.zeros((cap, N)) if record_diag else None
+        a = ctypes.c_int(0)
+        n = ctypes.c_int64(0)
+        rc = self.L.orcf_run(ctypes.byref(p), _ptr(G1), _ptr(Gxx), _ptr(Gxy), int(run_as), ctypes.byref(a),
+                             ctypes.byref(n), _ptr(traj) if traj_cap else None, traj_cap, traj_every,
+                             _ptr(diag) if record_diag else None, cap if record_diag else 0)
+        nrec = 0 if not traj_every else min(traj_cap, (n.value + traj_every - 1) // traj_every)
+        return dict(rc=rc, G1=G1, Gxx=Gxx, Gxy=Gxy, as_steps=a.value, rk_steps=n.value, traj=traj[:nrec],
+                    diag=None if diag is None else diag[: n.value])
 
     def fixed_step(self, cfg, G1, Gxx, Gxy, dt):
         p, keep = self.params(cfg)

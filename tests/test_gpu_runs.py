@@ -1,0 +1,143 @@
+"""Full protocol runs on the GPU against the oracle's golden trajectories.
+
+config1 (N=8, 304 AS + 3000 RK4 steps) and config2 (N=32, EHM, pulse quench,
+304 AS + 10000 RK4 steps), both FP64: G1 after the run within 1e-10 relative,
+trace conserved to 1e-12, the dipole trajectory and sampled G2 entries.  The
+command-line drop-in is run on config1 and its text files are compared with
+the oracle (6 significant digits, as the reference writes them).
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+g = pytest.importorskip("g12kba_amd")
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLD = os.path.join(HERE, "golden")
+
+CONFIG1_INPUT = """HF
+false
+EHM
+false
+t1
+1
+t2
+0.5
+t3
+0
+U
+1.0
+time_steps
+3000
+dt
+0.02
+quench_strength
+1.0
+Ns
+4
+Nq
+2
+Nb
+2
+q_type
+full
+AS_rate
+0.5
+AS_midpoint
+10
+quench_rate
+2.0
+quench_on_midpoint
+25
+quench_off_midpoint
+55
+"""
+
+
+def config1_params():
+    return g.make_params(Ns=4, Nb=2, t1=1.0, t2=0.5, U=1.0, dt=0.02, time_steps=3000, quench_strength=1.0, Nq=2,
+                         q_type="full", AS_rate=0.5, AS_midpoint=10.0, quench_rate=2.0, quench_on_midpoint=25.0)
+
+
+def config2_params():
+    return g.make_params(Ns=16, Nb=2, t1=1.0, t2=0.3, U=2.0, dt=0.01, time_steps=10000, quench_strength=1.5, Nq=4,
+                         q_type="pulse", AS_rate=0.5, AS_midpoint=10.0, quench_rate=2.0, quench_on_midpoint=30.0,
+                         quench_off_midpoint=60.0, EHM=True, V_law="coulomb")
+
+
+def relerr(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def test_config1_full_run():
+    z = np.load(os.path.join(GOLD, "run_config1.npz"))
+    p = config1_params()
+    with g.Propagator(p) as prop:
+        prop.set_state(z["G1_0"])
+        nas = prop.adiabatic_switching()
+        G1a, Gxxa, Gxya = prop.get_state()
+        rc, traj = prop.rk4()
+        G1e, Gxxe, Gxye = prop.get_state()
+    assert nas == int(z["as_steps"]) == 304
+    assert rc == 0 and len(traj) == int(z["rk_steps"]) == 3000
+    assert relerr(G1a, z["G1_as"]) < 1e-10
+    assert relerr(Gxxa, z["G2xx_as"]) < 1e-10 and relerr(Gxya, z["G2xy_as"]) < 1e-10
+    assert relerr(G1e, z["G1_end"]) < 1e-10
+    assert relerr(Gxxe, z["G2xx_end"]) < 1e-9 and relerr(Gxye, z["G2xy_end"]) < 1e-9
+    assert relerr(traj[::10], z["traj_every10"]) < 1e-10
+    assert np.abs(g.dipole(traj) - z["dipole"]).max() < 1e-10
+    tr = np.trace(traj, axis1=1, axis2=2)
+    assert np.abs(tr - tr[0]).max() < 1e-12
+
+
+def _read_txt(path):
+    with open(path) as f:
+        return np.array([[float(x) for x in line.split()] for line in f if line.strip()])
+
+
+def test_cli_config1_matches_oracle_text():
+    z = np.load(os.path.join(GOLD, "run_config1.npz"))
+    exe = os.path.join(ROOT, "g12kba_amd", "lib", "g12kba")
+    d = tempfile.mkdtemp()
+    with open(os.path.join(d, "input"), "w") as f:
+        f.write(CONFIG1_INPUT)
+    z["G1_0"].astype(np.complex128).tofile(os.path.join(d, "g0.bin"))
+    r = subprocess.run([exe, "--g0", "g0.bin"], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    re_, im_ = _read_txt(os.path.join(d, "real_G1.txt")), _read_txt(os.path.join(d, "imag_G1.txt"))
+    assert re_.shape == (64, 3000) and im_.shape == (64, 3000)
+    # rows = matrix elements, columns = steps (write_to_file.h:52-69)
+    want = z["traj_every10"].reshape(300, 64).T
+    got = (re_ + 1j * im_)[:, ::10]
+    assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max() + 1e-6
+    dip = np.loadtxt(os.path.join(d, "dipole.txt"))
+    assert dip.shape == (3000,)
+    assert np.abs(dip - z["dipole"]).max() <= 1e-5 * np.abs(z["dipole"]).max() + 1e-6
+    assert abs(float(open(os.path.join(d, "progress.txt")).read()) - 100.0) < 1e-6
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLD, "run_config2.npz")), reason="config2 fixture not built")
+def test_config2_full_run():
+    z = np.load(os.path.join(GOLD, "run_config2.npz"))
+    p = config2_params()
+    with g.Propagator(p) as prop:
+        prop.set_state(z["G1_0"])
+        nas = prop.adiabatic_switching()
+        rc, traj = prop.rk4()
+        G1e, Gxxe, Gxye = prop.get_state()
+    assert nas == int(z["as_steps"]) == 304
+    assert rc == 0 and len(traj) == int(z["rk_steps"]) == 10000
+    assert relerr(G1e, z["G1_end"]) < 1e-10
+    assert relerr(traj[::500], z["traj_every500"]) < 1e-10
+    assert np.abs(g.dipole(traj) - z["dipole"]).max() < 1e-10 * np.abs(z["dipole"]).max() + 1e-13
+    tr = np.trace(traj, axis1=1, axis2=2)
+    assert np.abs(tr - tr[0]).max() < 1e-12
+    idx = z["G2_idx"]
+    assert relerr(Gxxe.reshape(-1)[idx], z["G2xx_sample"]) < 1e-9
+    assert relerr(Gxye.reshape(-1)[idx], z["G2xy_sample"]) < 1e-9
+    assert abs(np.linalg.norm(Gxxe) / float(z["G2xx_norm"]) - 1) < 1e-10
