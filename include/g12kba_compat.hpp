@@ -1,0 +1,121 @@
+// g12kba_compat.hpp — the reference's routine names and std::vector
+// signatures on top of the C ABI (include/g12kba.h).
+//
+// A maintainer of the reference program replaces its headers
+// (HF.h, commutators.h, collision_integral.h, psi.h, pi.h, G1_G2_t_step.h,
+// fermi_func.h, dipole.h) by this one header and links libg12kba.so; the
+// call sites in RK_solver.h / AS.h compile unchanged and the work runs on
+// the GPU.  Errors (negative return codes of the C ABI) throw
+// std::runtime_error, since the reference routines return void.
+#ifndef G12KBA_COMPAT_HPP
+#define G12KBA_COMPAT_HPP
+
+#include <complex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "g12kba.h"
+
+namespace g12kba_compat {
+inline void check(int rc, const char* what) {
+    if (rc < 0) throw std::runtime_error(std::string(what) + " failed with code " + std::to_string(rc));
+}
+inline double* d(std::vector<std::complex<double>>& v) { return reinterpret_cast<double*>(v.data()); }
+inline const double* cd(const std::vector<std::complex<double>>& v) {
+    return reinterpret_cast<const double*>(v.data());
+}
+}  // namespace g12kba_compat
+
+// HF.h:57
+inline void h_HFx_quench(std::vector<std::complex<double>>& h, std::vector<std::complex<double>>& G1x,
+                         std::vector<std::complex<double>> G1y, int Ns, int Nb, double U, double t1, double t2,
+                         double w, std::vector<double>& epsilon, std::vector<double>& V, int quench_extent) {
+    std::vector<double> eps(Nb, 0.0);
+    for (int a = 0; a < Nb && a < (int)epsilon.size(); ++a) eps[a] = epsilon[a];
+    g12kba_compat::check(g12_hf_hamiltonian(g12kba_compat::d(h), g12kba_compat::cd(G1x), g12kba_compat::cd(G1y), Ns,
+                                            Nb, U, t1, t2, w, eps.data(), V.data(), (int)V.size(), quench_extent),
+                         "h_HFx_quench");
+}
+
+// commutators.h:28
+inline void hG_commutator(std::vector<std::complex<double>>& G, std::vector<std::complex<double>>& h,
+                          std::vector<std::complex<double>>& comm) {
+    int N = 0;
+    while ((size_t)(N + 1) * (N + 1) <= G.size()) ++N;
+    g12kba_compat::check(g12_h1_commutator(g12kba_compat::cd(G), g12kba_compat::cd(h), g12kba_compat::d(comm), N),
+                         "hG_commutator");
+}
+
+// commutators.h:53
+inline void h2xyG2xyxy_commutator(std::vector<std::complex<double>>& G2xyxy, std::vector<std::complex<double>>& h1x,
+                                  std::vector<std::complex<double>>& h1y, std::vector<std::complex<double>>& comm,
+                                  int Ns, int Nb) {
+    g12kba_compat::check(g12_h2_commutator(g12kba_compat::cd(G2xyxy), g12kba_compat::cd(h1x), g12kba_compat::cd(h1y),
+                                           g12kba_compat::d(comm), Ns, Nb, G12_PREC_FP64),
+                         "h2xyG2xyxy_commutator");
+}
+
+// collision_integral.h:27
+inline void collision_int(std::vector<std::complex<double>>& Ix, std::vector<std::complex<double>>& G2xyxy,
+                          std::vector<std::complex<double>>& G2xxxx, int Ns, int Nb, double U, std::vector<double>& V) {
+    g12kba_compat::check(g12_collision(g12kba_compat::d(Ix), g12kba_compat::cd(G2xyxy), g12kba_compat::cd(G2xxxx), Ns,
+                                       Nb, U, V.data(), (int)V.size()),
+                         "collision_int");
+}
+
+// psi.h:157 and psi.h:26
+inline void Psi_xxxx(std::vector<std::complex<double>>& psi, std::vector<std::complex<double>>& G1, int Ns, int Nb,
+                     double U, std::vector<double>& V) {
+    g12kba_compat::check(
+        g12_psi_xxxx(g12kba_compat::d(psi), g12kba_compat::cd(G1), Ns, Nb, U, V.data(), (int)V.size()), "Psi_xxxx");
+}
+inline void Psi_xyxy(std::vector<std::complex<double>>& psi, std::vector<std::complex<double>>& G1, int Ns, int Nb,
+                     double U, std::vector<double>& V) {
+    g12kba_compat::check(
+        g12_psi_xyxy(g12kba_compat::d(psi), g12kba_compat::cd(G1), Ns, Nb, U, V.data(), (int)V.size()), "Psi_xyxy");
+}
+
+// pi.h:25
+inline void Pi_xyxy(std::vector<std::complex<double>>& pi, std::vector<std::complex<double>>& G1y,
+                    std::vector<std::complex<double>>& G2xyxy, std::vector<std::complex<double>>& G2xxxx, double U,
+                    int Ns, int Nb, std::vector<double>& V) {
+    g12kba_compat::check(g12_pi(g12kba_compat::d(pi), g12kba_compat::cd(G1y), g12kba_compat::cd(G2xyxy),
+                                g12kba_compat::cd(G2xxxx), U, Ns, Nb, V.data(), (int)V.size()),
+                         "Pi_xyxy");
+}
+
+// G1_G2_t_step.h:24 and :55
+inline void G1_step(std::vector<std::complex<double>>& h1G1_comm, std::vector<std::complex<double>>& I,
+                    std::vector<std::complex<double>>& K, std::vector<std::complex<double>>& RK_result, double w,
+                    int Ns, int Nb) {
+    g12kba_compat::check(g12_g1_step(g12kba_compat::cd(h1G1_comm), g12kba_compat::cd(I), g12kba_compat::d(K),
+                                     g12kba_compat::d(RK_result), w, Ns, Nb),
+                         "G1_step");
+}
+inline void G2_step(std::vector<std::complex<double>>& h2G2_comm, std::vector<std::complex<double>>& psi,
+                    std::vector<std::complex<double>>& pi, std::vector<std::complex<double>>& K,
+                    std::vector<std::complex<double>>& RK_result, double w, int Ns, int Nb) {
+    g12kba_compat::check(g12_g2_step(g12kba_compat::cd(h2G2_comm), g12kba_compat::cd(psi), g12kba_compat::cd(pi),
+                                     g12kba_compat::d(K), g12kba_compat::d(RK_result), w, Ns, Nb),
+                         "G2_step");
+}
+
+// fermi_func.h:21
+inline double fermi(double t, double t0, double tau) { return g12_fermi(t, t0, tau); }
+
+// dipole.h:25
+inline void dipole(std::vector<double>& p, std::vector<std::vector<std::complex<double>>>& G) {
+    if (G.empty()) return;
+    int N = 0;
+    while ((size_t)(N + 1) * (N + 1) <= G[0].size()) ++N;
+    std::vector<std::complex<double>> flat;
+    flat.reserve(G.size() * G[0].size());
+    for (auto& g : G) flat.insert(flat.end(), g.begin(), g.end());
+    std::vector<double> out(G.size());
+    g12kba_compat::check(g12_dipole(out.data(), reinterpret_cast<const double*>(flat.data()), (int64_t)G.size(), N),
+                         "dipole");
+    for (size_t t = 0; t < out.size() && t < p.size(); ++t) p[t] += out[t];
+}
+
+#endif  // G12KBA_COMPAT_HPP
