@@ -23,7 +23,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "G2 RK4 steps/s at N=64 (FP64 vs mixed @≤1e-6 G1 err); batched traj/s"
-PEAK_FP64 = 78.6  # TFLOP/s, dense FP64 (MI355X published)
+# HBM bytes per launch of the dominant kernel (read + written), measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload and calibrated with a
+# plain copy kernel (bytes read = 2 x FETCH_SIZE KiB, written = WRITE_SIZE KiB).
+TRAFFIC_BYTES = {("fp64", "tile_fused"): 4.07e9}
 
 
 def workload_params(g, precision="fp64", time_steps=5000):
@@ -63,12 +66,59 @@ def cpu_baseline(p, G1, Gxx, Gxy, threads):
                       f"{t1 - t0:.1f} s, x4 per step"}
 
 
+PEAK = {"fp64": (78.6, "FP64 dense"), "fp32_3m": (157.3, "FP32 dense"), "bf16x3": (2500.0, "BF16 dense")}
+
+
+def run_leg(g, precision, G1, Gxx, Gxy, steps, warmup):
+    """Times `steps` RK4 steps of one precision mode; returns (ms/step, G1 after, kernel timings)."""
+    p = workload_params(g, precision)
+    prop = g.Propagator(p)
+    prop.set_state(G1, Gxx, Gxy)
+    if warmup:
+        prop.advance(warmup)
+    prop.timing(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    rc = prop.advance(steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    if rc != 0:
+        raise SystemExit(f"density guard tripped (rc={rc}, {precision})")
+    k = [prop.timing_get(i) for i in range(3)]
+    prop.timing(False)
+    G1_end = prop.get_G1()
+    prop.close()
+    return 1e3 * (t1 - t0) / steps, G1_end, k
+
+
+def roofline(N, precision, k, steps):
+    # dominant kernel: the tile pass with the larger summed time; one launch of
+    # either pass applies two of the four one-particle terms of [H2,G2] to both
+    # G2 tensors: 2 tensors x N^2 tiles x 2 N^3 complex MACs x 8 flop = 32 N^5
+    # (8 flop per complex MAC whatever the product scheme: 4M, 3M or BF16 split).
+    dom = max((k[0], "tile_strided"), (k[1], "tile_fused"), key=lambda x: x[0][1])
+    (n_launch, ms_total), name = dom
+    avg_ms = ms_total / max(n_launch, 1)
+    flops = 32.0 * N ** 5
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    peak, peak_name = PEAK[precision]
+    return {"bound": "mfma", "kernel": name, "achieved": round(achieved, 3), "peak": peak, "peak_type": peak_name,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+            "traffic": TRAFFIC_BYTES.get((precision, name)),
+            "algorithmic_flop_per_launch": flops, "avg_launch_ms": round(avg_ms, 4),
+            "breakdown_ms_per_step": {"tile_strided": round(k[0][1] / steps, 3),
+                                      "tile_fused": round(k[1][1] / steps, 3),
+                                      "other": round(k[2][1] / steps, 3)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_3m", "bf16x3"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_3m", "bf16x3"],
+                    help="precision of the headline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     a = ap.parse_args()
@@ -80,46 +130,27 @@ def main():
     import g12kba_amd as g
     import numpy as np
 
-    p = workload_params(g, a.precision)
+    p = workload_params(g, "fp64")
     N = p.Ns * p.Nb
     G1, Gxx, Gxy = synthetic_state(g, p)
-    prop = g.Propagator(p)
-    prop.set_state(G1, Gxx, Gxy)
-    if a.warmup:
-        prop.advance(a.warmup)
-    sync = g.synchronize
-    prop.timing(True)
-    sync()
-    t0 = time.perf_counter()
-    rc = prop.advance(a.steps)
-    sync()
-    t1 = time.perf_counter()
-    if rc != 0:
-        raise SystemExit(f"density guard tripped (rc={rc})")
-    ms_step = 1e3 * (t1 - t0) / a.steps
-    k_strided = prop.timing_get(0)
-    k_fused = prop.timing_get(1)
-    k_other = prop.timing_get(2)
-    prop.timing(False)
-    G1_end = prop.get_G1()
-    prop.close()
-    if not np.isfinite(G1_end).all():
+    legs = [a.precision] + ([] if a.no_mixed else [x for x in ("fp64", "fp32_3m", "bf16x3") if x != a.precision])
+    res = {}
+    for prec in legs:
+        res[prec] = run_leg(g, prec, G1, Gxx, Gxy, a.steps, a.warmup)
+    ms_step, G1_head, k_head = res[a.precision]
+    if not np.isfinite(G1_head).all():
         raise SystemExit("non-finite state")
-
-    # dominant kernel: the tile pass with the larger summed time; one launch of
-    # either pass applies two of the four one-particle terms of [H2,G2] to both
-    # G2 tensors: 2 tensors x N^2 tiles x 2 N^3 complex MACs x 8 flop = 32 N^5.
-    dom = max((k_strided, "tile_strided"), (k_fused, "tile_fused"), key=lambda x: x[0][1])
-    (n_launch, ms_total), name = dom
-    avg_ms = ms_total / max(n_launch, 1)
-    flops = 32.0 * N ** 5
-    achieved = flops / (avg_ms * 1e-3) / 1e12
-    roof = {"bound": "mfma", "kernel": name, "achieved": round(achieved, 3), "peak": PEAK_FP64,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": None,
-            "algorithmic_flop_per_launch": flops, "avg_launch_ms": round(avg_ms, 4),
-            "breakdown_ms_per_step": {"tile_strided": round(k_strided[1] / a.steps, 3),
-                                      "tile_fused": round(k_fused[1] / a.steps, 3),
-                                      "other": round(k_other[1] / a.steps, 3)}}
+    mixed = {}
+    if "fp64" in res:
+        d64 = np.diagonal(res["fp64"][1]).imag
+        for prec in ("fp32_3m", "bf16x3"):
+            if prec in res:
+                ms, G1m, km = res[prec]
+                mixed[prec] = {"steps_per_s": round(1e3 / ms, 3), "ms_per_step": round(ms, 4),
+                               "speedup_vs_fp64": round(res["fp64"][0] / ms, 3),
+                               "max_density_err_vs_fp64": float(np.abs(np.diagonal(G1m).imag - d64).max()),
+                               "err_horizon_steps": a.warmup + a.steps,
+                               "roofline": roofline(N, prec, km, a.steps)}
     cpu = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
@@ -134,7 +165,7 @@ def main():
            "config": {"workload": "Ns=32 Nb=2 (N=64) EHM V=U/(1+n) U=1.5 t1=1 t2=0.5 full quench 1.0 on 8 sites "
                                   "dt=0.01", "N": N, "precision": a.precision,
                       "algorithmic_flop_per_step": 256.0 * N ** 5},
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roofline(N, a.precision, k_head, a.steps), "mixed": mixed, "cpu_baseline": cpu}
     print(json.dumps(out))
 
 

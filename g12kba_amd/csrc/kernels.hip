@@ -378,6 +378,230 @@ __device__ __forceinline__ void tile_mfma_fp64(const TileLds<NP>& L, int hmode, 
     if (hmode & 2) tile_product<NP, NW, true>(L.xr, L.xi, L.hr, L.hi, ar, ai, wave, lane);   // - X.h
 }
 
+// ---- precision engines ------------------------------------------------------
+// Each engine owns the LDS image of a tile X and of h, writes a tile element
+// converted from FP64, and runs O = h.X - X.h on its MFMA type.  Results come
+// back as FP64 (re, im) per 16x16 block in MFMA result order; row_of() maps
+// (lane, j) to the block row.
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <int NP, int NW, int PREC>
+struct Engine;
+
+template <int NP, int NW>
+struct Engine<NP, NW, G12_PREC_FP64> {
+    typedef TileLds<NP> Lds;
+    static constexpr int P = Lds::P;
+    __device__ static void put_h(Lds& L, int m, int n, double2 v) {
+        L.hr[m * P + n] = v.x;
+        L.hi[m * P + n] = v.y;
+    }
+    __device__ static void put_x(Lds& L, int m, int n, double2 v) {
+        L.xr[m * P + n] = v.x;
+        L.xi[m * P + n] = v.y;
+    }
+    __device__ static void zero(Lds&, int, int) {}
+    __device__ static int row_of(int lane, int j) { return 4 * j + (lane >> 4); }
+    __device__ static void compute(const Lds& L, int hmode, d4 (&ar)[TileCfg<NP, NW>::PW],
+                                   d4 (&ai)[TileCfg<NP, NW>::PW], int wave, int lane) {
+        tile_mfma_fp64<NP, NW>(L, hmode, ar, ai, wave, lane);
+    }
+};
+
+// 3M complex products on FP32 MFMA: with S = re + im planes,
+//   P1 = Ar Br, P2 = Ai Bi, P3 = As Bs;  Re = P1 - P2, Im = P3 - P1 - P2.
+template <int NP>
+struct TileLds32 {
+    static constexpr int P = NP + 1;
+    float xr[NP * P], xi[NP * P], xs[NP * P];
+    float hr[NP * P], hi[NP * P], hs[NP * P];
+};
+
+template <int NP, int NW>
+struct Engine<NP, NW, G12_PREC_FP32_3M> {
+    typedef TileLds32<NP> Lds;
+    static constexpr int P = Lds::P;
+    static constexpr int NT = TileCfg<NP, NW>::NT, TT = TileCfg<NP, NW>::TT, PW = TileCfg<NP, NW>::PW;
+    __device__ static void put_h(Lds& L, int m, int n, double2 v) {
+        L.hr[m * P + n] = (float)v.x;
+        L.hi[m * P + n] = (float)v.y;
+        L.hs[m * P + n] = (float)(v.x + v.y);
+    }
+    __device__ static void put_x(Lds& L, int m, int n, double2 v) {
+        L.xr[m * P + n] = (float)v.x;
+        L.xi[m * P + n] = (float)v.y;
+        L.xs[m * P + n] = (float)(v.x + v.y);
+    }
+    __device__ static void zero(Lds&, int, int) {}
+    __device__ static int row_of(int lane, int j) { return 4 * (lane >> 4) + j; }
+    template <bool NEG>
+    __device__ static void product(const float* ar_, const float* ai_, const float* as_, const float* br_,
+                                   const float* bi_, const float* bs_, f4 (&p1)[PW], f4 (&p2)[PW], f4 (&p3)[PW],
+                                   int wave, int lane) {
+        constexpr bool FULL = (TT % NW) == 0;
+        const int r = lane & 15, kq = lane >> 4;
+#pragma unroll 4
+        for (int k0 = 0; k0 < NP; k0 += 4) {
+            const int kk = k0 + kq;
+            float a1[PW], a2[PW], a3[PW], b1[PW], b2[PW], b3[PW];
+#pragma unroll
+            for (int u = 0; u < PW; ++u) {
+                const int t = wave + NW * u;
+                const bool on = FULL || t < TT;
+                const int mm = (t / NT) * 16 + r, nn = (t % NT) * 16 + r;
+                a1[u] = on ? ar_[mm * P + kk] : 0.f;
+                a2[u] = on ? ai_[mm * P + kk] : 0.f;
+                a3[u] = on ? as_[mm * P + kk] : 0.f;
+                b1[u] = on ? br_[kk * P + nn] : 0.f;
+                b2[u] = on ? bi_[kk * P + nn] : 0.f;
+                b3[u] = on ? bs_[kk * P + nn] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < PW; ++u) p1[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(NEG ? -a1[u] : a1[u], b1[u], p1[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < PW; ++u) p2[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(NEG ? -a2[u] : a2[u], b2[u], p2[u], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < PW; ++u) p3[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(NEG ? -a3[u] : a3[u], b3[u], p3[u], 0, 0, 0);
+        }
+    }
+    __device__ static void compute(const Lds& L, int hmode, d4 (&ar)[PW], d4 (&ai)[PW], int wave, int lane) {
+        f4 p1[PW], p2[PW], p3[PW];
+#pragma unroll
+        for (int u = 0; u < PW; ++u) {
+            p1[u] = f4{0.f, 0.f, 0.f, 0.f};
+            p2[u] = p1[u];
+            p3[u] = p1[u];
+        }
+        if (hmode & 1) product<false>(L.hr, L.hi, L.hs, L.xr, L.xi, L.xs, p1, p2, p3, wave, lane);
+        if (hmode & 2) product<true>(L.xr, L.xi, L.xs, L.hr, L.hi, L.hs, p1, p2, p3, wave, lane);
+#pragma unroll
+        for (int u = 0; u < PW; ++u)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double q1 = p1[u][j], q2 = p2[u][j], q3 = p3[u][j];
+                ar[u][j] = q1 - q2;
+                ai[u][j] = q3 - q1 - q2;
+            }
+    }
+};
+
+// BF16 hi/lo split on BF16 MFMA (16x16x32): every operand v = hi + lo with
+// hi = bf16(v), lo = bf16(v - hi); a real product keeps hi.hi + hi.lo + lo.hi
+// (the lo.lo term is below the split's own rounding).  Four real products per
+// complex product, FP32 accumulation.  A operands are stored row-major, B
+// operands transposed, so each lane reads 8 consecutive k (16 bytes).
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+
+template <int NP>
+struct TileLdsBf {
+    static constexpr int KP = ((NP + 31) / 32) * 32;  // k extent padded to the MFMA depth
+    static constexpr int PB = KP + 8;                  // pitch (bf16); 16-byte aligned rows
+    // [plane][row][k]; planes: 0-3 h (r hi, r lo, i hi, i lo) row-major,
+    // 4-7 h transposed, 8-11 X row-major, 12-15 X transposed
+    __bf16 v[16][NP * PB];
+};
+
+__device__ __forceinline__ bf8 bf_neg(bf8 x) {
+    u4v u = __builtin_bit_cast(u4v, x);
+    u ^= (u4v){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+    return __builtin_bit_cast(bf8, u);
+}
+
+template <int NP, int NW>
+struct Engine<NP, NW, G12_PREC_BF16X3> {
+    typedef TileLdsBf<NP> Lds;
+    static constexpr int PB = Lds::PB, KP = Lds::KP;
+    static constexpr int NT = TileCfg<NP, NW>::NT, TT = TileCfg<NP, NW>::TT, PW = TileCfg<NP, NW>::PW;
+    __device__ static void split(double v, __bf16& hi, __bf16& lo) {
+        hi = (__bf16)(float)v;
+        lo = (__bf16)(float)(v - (double)(float)hi);
+    }
+    __device__ static void put(Lds& L, int base, int m, int n, double2 v) {
+        __bf16 rh, rl, ih, il;
+        split(v.x, rh, rl);
+        split(v.y, ih, il);
+        L.v[base + 0][m * PB + n] = rh;
+        L.v[base + 1][m * PB + n] = rl;
+        L.v[base + 2][m * PB + n] = ih;
+        L.v[base + 3][m * PB + n] = il;
+        L.v[base + 4][n * PB + m] = rh;  // transposed copy
+        L.v[base + 5][n * PB + m] = rl;
+        L.v[base + 6][n * PB + m] = ih;
+        L.v[base + 7][n * PB + m] = il;
+    }
+    __device__ static void put_h(Lds& L, int m, int n, double2 v) { put(L, 0, m, n, v); }
+    __device__ static void put_x(Lds& L, int m, int n, double2 v) { put(L, 8, m, n, v); }
+    __device__ static void zero(Lds& L, int tid, int nthr) {
+        unsigned* w = reinterpret_cast<unsigned*>(&L);
+        for (int i = tid; i < (int)(sizeof(Lds) / 4); i += nthr) w[i] = 0u;
+    }
+    __device__ static int row_of(int lane, int j) { return 4 * (lane >> 4) + j; }
+    __device__ static bf8 frag(const __bf16* plane, int row, int k) {
+        return *reinterpret_cast<const bf8*>(plane + row * PB + k);
+    }
+    // acc += sign * A.B ; A planes abase (row-major), B planes bbase (transposed)
+    template <bool NEG>
+    __device__ static void product(const Lds& L, int abase, int bbase, f4 (&cr)[PW], f4 (&ci)[PW], int wave,
+                                   int lane) {
+        constexpr bool FULL = (TT % NW) == 0;
+        const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+        for (int k0 = 0; k0 < KP; k0 += 32) {
+            const int kk = k0 + 8 * kq;
+#pragma unroll
+            for (int u = 0; u < PW; ++u) {
+                const int t = wave + NW * u;
+                if (!(FULL || t < TT)) continue;
+                const int mm = (t / NT) * 16 + r, nn = (t % NT) * 16 + r;
+                bf8 arh = frag(L.v[abase + 0], mm, kk), arl = frag(L.v[abase + 1], mm, kk);
+                bf8 aih = frag(L.v[abase + 2], mm, kk), ail = frag(L.v[abase + 3], mm, kk);
+                const bf8 brh = frag(L.v[bbase + 0], nn, kk), brl = frag(L.v[bbase + 1], nn, kk);
+                const bf8 bih = frag(L.v[bbase + 2], nn, kk), bil = frag(L.v[bbase + 3], nn, kk);
+                if (NEG) {
+                    arh = bf_neg(arh);
+                    arl = bf_neg(arl);
+                    aih = bf_neg(aih);
+                    ail = bf_neg(ail);
+                }
+                const bf8 naih = bf_neg(aih), nail = bf_neg(ail);
+                // Re += Ar Br - Ai Bi
+                cr[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brh, cr[u], 0, 0, 0);
+                cr[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brl, cr[u], 0, 0, 0);
+                cr[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, brh, cr[u], 0, 0, 0);
+                cr[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bih, cr[u], 0, 0, 0);
+                cr[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bil, cr[u], 0, 0, 0);
+                cr[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail, bih, cr[u], 0, 0, 0);
+                // Im += Ar Bi + Ai Br
+                ci[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bih, ci[u], 0, 0, 0);
+                ci[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bil, ci[u], 0, 0, 0);
+                ci[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, bih, ci[u], 0, 0, 0);
+                ci[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brh, ci[u], 0, 0, 0);
+                ci[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brl, ci[u], 0, 0, 0);
+                ci[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, brh, ci[u], 0, 0, 0);
+            }
+        }
+    }
+    __device__ static void compute(const Lds& L, int hmode, d4 (&ar)[PW], d4 (&ai)[PW], int wave, int lane) {
+        f4 cr[PW], ci[PW];
+#pragma unroll
+        for (int u = 0; u < PW; ++u) {
+            cr[u] = f4{0.f, 0.f, 0.f, 0.f};
+            ci[u] = cr[u];
+        }
+        if (hmode & 1) product<false>(L, 0, 12, cr, ci, wave, lane);  // + h.X : A = h, B = X^T planes
+        if (hmode & 2) product<true>(L, 8, 4, cr, ci, wave, lane);    // - X.h : A = X, B = h^T planes
+#pragma unroll
+        for (int u = 0; u < PW; ++u)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                ar[u][j] = cr[u][j];
+                ai[u][j] = ci[u][j];
+            }
+    }
+};
+
 struct TileArgs {
     const double2* S[2];   // state (tile input Y = S + c*Kp)
     const double2* Kp[2];  // previous stage derivative (may be null)
@@ -472,11 +696,11 @@ __device__ __forceinline__ void tile_prefetch(const TileArgs& a, int gt, double2
     }
 }
 
-template <int NP, int NW, int MODE>
+template <int NP, int NW, int MODE, int PREC>
 __global__ __launch_bounds__(64 * NW) void k_tile(TileArgs a) {
     if (failed(a.fail)) return;
-    __shared__ TileLds<NP> L;
-    constexpr int P = TileLds<NP>::P;
+    typedef Engine<NP, NW, PREC> E;
+    __shared__ typename E::Lds L;
     constexpr int NT = TileCfg<NP, NW>::NT, TT = TileCfg<NP, NW>::TT, PW = TileCfg<NP, NW>::PW;
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
     const int N = a.N;
@@ -484,14 +708,14 @@ __global__ __launch_bounds__(64 * NW) void k_tile(TileArgs a) {
     const int total = a.nspin * N * N;
     const TileSched sched(total, gridDim.x, blockIdx.x);
     // h (shared by every tile of the launch) and the zero padding of X
+    E::zero(L, threadIdx.x, NTHR);
+    __syncthreads();
     for (int e = threadIdx.x; e < NP * NP; e += NTHR) {
         const int m = e / NP, n = e % NP;
         double2 hv = make_double2(0.0, 0.0);
         if (m < N && n < N) hv = a.h[m * N + n];
-        L.hr[m * P + n] = hv.x;
-        L.hi[m * P + n] = hv.y;
-        L.xr[m * P + n] = 0.0;
-        L.xi[m * P + n] = 0.0;
+        E::put_h(L, m, n, hv);
+        E::put_x(L, m, n, make_double2(0.0, 0.0));
     }
     int gt = sched.tile(0, blockIdx.x, gridDim.x);
     double2 pre[PF];
@@ -499,34 +723,29 @@ __global__ __launch_bounds__(64 * NW) void k_tile(TileArgs a) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int it = 0; gt >= 0; ++it) {
         __syncthreads();  // previous tile's readers of X are done
+        const int s = gt / (N * N);
+        int64_t base, sm, sn;
+        int t0, t1;
+        tile_geometry(a.tiling, N, gt % (N * N), base, sm, sn, t0, t1);
+        // D[X=p1][Y=q1][Z] (Z = p2 = q2) from the FP64 stage input of a diagonal strided tile
+        const bool dtile = MODE == 0 && a.D[s] && a.tiling == 0 && t0 == t1;
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
             if (e < N * N) {
                 const int m = e / N, n = e % N;
-                L.xr[m * P + n] = pre[i].x;
-                L.xi[m * P + n] = pre[i].y;
+                E::put_x(L, m, n, pre[i]);
+                if (dtile) a.D[s][((int64_t)m * N + n) * N + t0] = pre[i];
             }
         }
         __syncthreads();
-        const int s = gt / (N * N);
-        int64_t base, sm, sn;
-        int t0, t1;
-        tile_geometry(a.tiling, N, gt % (N * N), base, sm, sn, t0, t1);
         const int next = sched.tile(it + 1, blockIdx.x, gridDim.x);
         if (next >= 0) tile_prefetch<NP, NW>(a, next, pre);  // overlaps the MFMA loop
 
         d4 ar[PW], ai[PW];
-        tile_mfma_fp64<NP, NW>(L, a.hmode, ar, ai, wave, lane);
+        E::compute(L, a.hmode, ar, ai, wave, lane);
 
         if (MODE == 0) {
-            if (a.D[s] && a.tiling == 0 && t0 == t1) {
-                // D[X=p1][Y=q1][Z] with Z = p2 = q2
-                for (int e = threadIdx.x; e < N * N; e += NTHR) {
-                    const int m = e / N, n = e % N;
-                    a.D[s][((int64_t)m * N + n) * N + t0] = make_double2(L.xr[m * P + n], L.xi[m * P + n]);
-                }
-            }
             double2* out = a.out[s];
 #pragma unroll
             for (int u = 0; u < PW; ++u) {
@@ -535,7 +754,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(TileArgs a) {
                 const int n = (t % NT) * 16 + (lane & 15);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int m = (t / NT) * 16 + 4 * j + (lane >> 4);
+                    const int m = (t / NT) * 16 + E::row_of(lane, j);
                     if (m < N && n < N) {
                         const int64_t off = base + m * sm + n * sn;
                         double2 v = make_double2(ar[u][j], ai[u][j]);
@@ -569,7 +788,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(TileArgs a) {
                 const int n = (t % NT) * 16 + (lane & 15);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int m = (t / NT) * 16 + 4 * j + (lane >> 4);
+                    const int m = (t / NT) * 16 + E::row_of(lane, j);
                     if (m >= N || n >= N) continue;
                     const int64_t off = base + m * sm + n * sn;
                     const int64_t mn = (int64_t)m * N + n;
@@ -623,10 +842,10 @@ int tile_waves(int NP) {
     return NP >= 48 ? 16 : 4;  // measured best on MI355X (FP64, N = 32/48/64)
 }
 
-template <int NP, int NW, int MODE>
+template <int NP, int NW, int MODE, int PREC>
 void launch_tile_cfg(const TileArgs& a, hipStream_t s) {
     const int total = a.nspin * a.N * a.N;
-    constexpr int lds = (int)sizeof(TileLds<NP>);
+    constexpr int lds = (int)sizeof(typename Engine<NP, NW, PREC>::Lds);
     int per_cu = (160 * 1024) / lds;
     const int by_waves = 32 / NW;  // at most 32 waves per CU
     if (per_cu > by_waves) per_cu = by_waves;
@@ -634,30 +853,40 @@ void launch_tile_cfg(const TileArgs& a, hipStream_t s) {
     int grid = device_cus() * per_cu;
     if (grid > total) grid = total;
     if (grid >= 8) grid &= ~7;
-    hipLaunchKernelGGL((k_tile<NP, NW, MODE>), dim3(grid), dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
+    hipLaunchKernelGGL((k_tile<NP, NW, MODE, PREC>), dim3(grid), dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
 }
 
-template <int NP, int MODE>
+template <int NP, int MODE, int PREC>
 void launch_tile_np(const TileArgs& a, hipStream_t s) {
     switch (tile_waves(NP)) {
-        case 16: launch_tile_cfg<NP, 16, MODE>(a, s); break;
-        case 8: launch_tile_cfg<NP, 8, MODE>(a, s); break;
-        default: launch_tile_cfg<NP, 4, MODE>(a, s); break;
+        case 16: launch_tile_cfg<NP, 16, MODE, PREC>(a, s); break;
+        case 8: launch_tile_cfg<NP, 8, MODE, PREC>(a, s); break;
+        default: launch_tile_cfg<NP, 4, MODE, PREC>(a, s); break;
     }
 }
 
-template <int MODE>
-void launch_tile(TileArgs a, int nspin, hipStream_t s) {
-    a.nspin = nspin;
+template <int MODE, int PREC>
+void launch_tile_prec(const TileArgs& a, hipStream_t s) {
     const int N = a.N;
     if (N <= 16)
-        launch_tile_np<16, MODE>(a, s);
+        launch_tile_np<16, MODE, PREC>(a, s);
     else if (N <= 32)
-        launch_tile_np<32, MODE>(a, s);
+        launch_tile_np<32, MODE, PREC>(a, s);
     else if (N <= 48)
-        launch_tile_np<48, MODE>(a, s);
+        launch_tile_np<48, MODE, PREC>(a, s);
     else
-        launch_tile_np<64, MODE>(a, s);
+        launch_tile_np<64, MODE, PREC>(a, s);
+}
+
+template <int MODE>
+void launch_tile(TileArgs a, int nspin, int precision, hipStream_t s) {
+    a.nspin = nspin;
+    if (precision == G12_PREC_FP32_3M)
+        launch_tile_prec<MODE, G12_PREC_FP32_3M>(a, s);
+    else if (precision == G12_PREC_BF16X3)
+        launch_tile_prec<MODE, G12_PREC_BF16X3>(a, s);
+    else
+        launch_tile_prec<MODE, G12_PREC_FP64>(a, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -849,7 +1078,6 @@ static TileArgs base_args(const DevState& d, const StageParams& sp) {
 }
 
 void launch_pass_strided(const DevState& d, const StageParams& sp, int precision, hipStream_t s) {
-    (void)precision;
     TileArgs a = base_args(d, sp);
     a.tiling = 0;
     for (int t = 0; t < 2; ++t) {
@@ -857,7 +1085,7 @@ void launch_pass_strided(const DevState& d, const StageParams& sp, int precision
         a.D[t] = d.D[t];
     }
     a.accumulate = 0;
-    launch_tile<0>(a, 2, s);
+    launch_tile<0>(a, 2, precision, s);
 }
 
 void launch_tables(const DevState& d, const StageParams& sp, hipStream_t s) {
@@ -872,7 +1100,6 @@ void launch_g1_rhs(const DevState& d, const StageParams& sp, hipStream_t s) {
 }
 
 void launch_pass_fused(const DevState& d, const StageParams& sp, int precision, hipStream_t s) {
-    (void)precision;
     TileArgs a = base_args(d, sp);
     a.tiling = 1;
     for (int t = 0; t < 2; ++t) {
@@ -892,7 +1119,7 @@ void launch_pass_fused(const DevState& d, const StageParams& sp, int precision, 
     a.dt = sp.dt;
     a.first = sp.first;
     a.last = sp.last;
-    launch_tile<1>(a, 2, s);
+    launch_tile<1>(a, 2, precision, s);
 }
 
 void launch_ref_to_nat(const double2* in, double2* out, int Ns, int Nb, hipStream_t s) {
@@ -907,7 +1134,6 @@ void launch_nat_to_ref(const double2* in, double2* out, int Ns, int Nb, hipStrea
 
 void launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, double2* tmp, int N,
                   int precision, hipStream_t s) {
-    (void)precision;
     (void)tmp;
     TileArgs a{};
     a.S[0] = T;
@@ -923,7 +1149,7 @@ void launch_comm2(const double2* T, const double2* hx, const double2* hy, double
             a.h = side == 1 ? hx : hy;
             a.accumulate = first ? 0 : 1;
             first = 0;
-            launch_tile<0>(a, 1, s);
+            launch_tile<0>(a, 1, precision, s);
         }
 }
 
