@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libg12kba.so")
+LIB_PATH = os.environ.get("G12KBA_LIBRARY") or os.path.join(HERE, "lib", "libg12kba.so")
 
 G12_OK, G12_GUARD = 0, 1
 QUENCH_FULL, QUENCH_PULSE = 0, 1
@@ -89,6 +89,23 @@ def _load():
         "g12_write_outputs": [ctypes.c_char_p, p, i64, i, p, d],
         "g12_save_state": [vp, ctypes.c_char_p],
         "g12_load_state": [vp, ctypes.c_char_p],
+        "g12_configure": [vp, i, i],
+        "g12_batch_create": [ctypes.POINTER(Params), i, ctypes.POINTER(vp)],
+        "g12_batch_destroy": [vp],
+        "g12_batch_size": [vp, ctypes.POINTER(i)],
+        "g12_batch_memory_bytes": [vp, pi64],
+        "g12_batch_set_state": [vp, i, p, p, p],
+        "g12_batch_get_state": [vp, i, p, p, p],
+        "g12_batch_adiabatic_switching": [vp, ctypes.POINTER(i)],
+        "g12_batch_rk4": [vp, ctypes.POINTER(p), i64, pi64],
+        "g12_batch_rk4_advance": [vp, i64],
+        "g12_batch_guard": [vp, ctypes.POINTER(i)],
+        "g12_batch_timing_enable": [vp, i],
+        "g12_batch_timing_get": [vp, i, pi64, p],
+        "g12_batch_configure": [vp, i, i],
+        "g12_batch_tune": [vp],
+        "g12_batch_buckets": [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
+                              ctypes.POINTER(i)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -110,6 +127,11 @@ EXPORTED = [
     "g12_hf_hamiltonian", "g12_h1_commutator", "g12_h2_commutator", "g12_collision",
     "g12_psi_xxxx", "g12_psi_xyxy", "g12_pi", "g12_g1_step", "g12_g2_step", "g12_fermi",
     "g12_dipole", "g12_write_outputs", "g12_save_state", "g12_load_state", "g12_version",
+    "g12_configure", "g12_batch_create", "g12_batch_destroy", "g12_batch_size",
+    "g12_batch_memory_bytes", "g12_batch_set_state", "g12_batch_get_state",
+    "g12_batch_adiabatic_switching", "g12_batch_rk4", "g12_batch_rk4_advance", "g12_batch_guard",
+    "g12_batch_timing_enable", "g12_batch_timing_get", "g12_batch_configure", "g12_batch_tune",
+    "g12_batch_buckets",
 ]
 
 
@@ -412,6 +434,124 @@ class Propagator:
 
     def load(self, path):
         _check(lib.g12_load_state(self.h, path.encode()), "g12_load_state")
+
+    def configure(self, graphs=None, autotune=None):
+        """graphs: replay RK4 steps as hipGraphs; autotune: time tile wave counts."""
+        g = -1 if graphs is None else int(bool(graphs))
+        a = -1 if autotune is None else int(bool(autotune))
+        _check(lib.g12_configure(self.h, g, a), "g12_configure")
+
+
+class Batch:
+    """Many independent trajectories advanced together (one set of launches per
+    RK4 stage for all of them).  `params` is a list of Params that share the
+    time protocol and precision; sizes and physics may differ."""
+
+    def __init__(self, params):
+        self.params = list(params)
+        n = len(self.params)
+        arr = (Params * n)(*self.params)
+        h = ctypes.c_void_p()
+        _check(lib.g12_batch_create(arr, n, ctypes.byref(h)), "g12_batch_create")
+        self.h = h
+        self.n = n
+        self.N = [q.Ns * q.Nb for q in self.params]
+
+    def close(self):
+        if self.h:
+            lib.g12_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self):
+        return self.n
+
+    def set_state(self, i, G1, G2xx=None, G2xy=None):
+        N = self.N[i]
+        G1 = _c(G1, (N, N))
+        Gxx = None if G2xx is None else _c(G2xx, (N ** 2, N ** 2))
+        Gxy = None if G2xy is None else _c(G2xy, (N ** 2, N ** 2))
+        _check(lib.g12_batch_set_state(self.h, i, _p(G1), _p(Gxx), _p(Gxy)), "g12_batch_set_state")
+
+    def get_G1(self, i):
+        N = self.N[i]
+        G1 = np.zeros((N, N), np.complex128)
+        _check(lib.g12_batch_get_state(self.h, i, _p(G1), None, None), "g12_batch_get_state")
+        return G1
+
+    def get_state(self, i):
+        N = self.N[i]
+        G1 = np.zeros((N, N), np.complex128)
+        Gxx = np.zeros((N * N, N * N), np.complex128)
+        Gxy = np.zeros((N * N, N * N), np.complex128)
+        _check(lib.g12_batch_get_state(self.h, i, _p(G1), _p(Gxx), _p(Gxy)), "g12_batch_get_state")
+        return G1, Gxx, Gxy
+
+    def adiabatic_switching(self):
+        steps = (ctypes.c_int * self.n)()
+        _check(lib.g12_batch_adiabatic_switching(self.h, steps), "g12_batch_adiabatic_switching")
+        return list(steps)
+
+    def rk4(self, traj_cap=None, record=True):
+        """Returns (rc, [trajectory_i[steps_i, N_i, N_i]])."""
+        if traj_cap is None:
+            traj_cap = self.params[0].time_steps + 2
+        trajs = [np.zeros((traj_cap, N, N), np.complex128) if record else None for N in self.N]
+        ptrs = (_dp * self.n)(*[_p(t) if t is not None else None for t in trajs])
+        steps = (ctypes.c_int64 * self.n)()
+        rc = _check(lib.g12_batch_rk4(self.h, ptrs if record else None, traj_cap, steps), "g12_batch_rk4")
+        if not record:
+            return rc, list(steps)
+        return rc, [t[: min(s, traj_cap)] for t, s in zip(trajs, steps)]
+
+    def advance(self, nsteps):
+        return _check(lib.g12_batch_rk4_advance(self.h, nsteps), "g12_batch_rk4_advance")
+
+    def guard(self):
+        f = (ctypes.c_int * self.n)()
+        _check(lib.g12_batch_guard(self.h, f), "g12_batch_guard")
+        return list(f)
+
+    def timing(self, on=True):
+        lib.g12_batch_timing_enable(self.h, int(on))
+
+    def timing_get(self, which):
+        n = ctypes.c_int64(0)
+        ms = ctypes.c_double(0)
+        _check(lib.g12_batch_timing_get(self.h, which, ctypes.byref(n), ctypes.byref(ms)),
+               "g12_batch_timing_get")
+        return n.value, ms.value
+
+    def memory_bytes(self):
+        b = ctypes.c_int64(0)
+        lib.g12_batch_memory_bytes(self.h, ctypes.byref(b))
+        return b.value
+
+    def configure(self, graphs=None, autotune=None):
+        g = -1 if graphs is None else int(bool(graphs))
+        a = -1 if autotune is None else int(bool(autotune))
+        _check(lib.g12_batch_configure(self.h, g, a), "g12_batch_configure")
+
+    def tune(self):
+        _check(lib.g12_batch_tune(self.h), "g12_batch_tune")
+
+    def buckets(self):
+        """[(NP, trajectories, waves_strided, waves_fused)] of the tile passes."""
+        cap = 8
+        a, b, c, d = [(ctypes.c_int * cap)() for _ in range(4)]
+        n = _check(lib.g12_batch_buckets(self.h, cap, a, b, c, d), "g12_batch_buckets")
+        return [(a[k], b[k], c[k], d[k]) for k in range(n)]
 
 
 def synchronize():

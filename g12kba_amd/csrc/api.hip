@@ -1,17 +1,27 @@
-// C ABI: contexts, device-resident adiabatic switching and RK4 drivers, and
-// the routine-level entry points (one per reference routine).
+// C ABI: runs (sets of trajectories advanced together; a context is a run of
+// one trajectory), the device-resident adiabatic-switching and RK4 drivers
+// with hipGraph replay of whole RK4 steps, the tile-shape autotuner, and the
+// routine-level entry points (one per reference routine).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
+#include <tuple>
 #include <vector>
 
 #include "g12_internal.h"
 #include "host_io.h"
 
+using g12::BatchDev;
+using g12::Bucket;
 using g12::DevState;
 using g12::StageParams;
+using g12::StageRef;
+using g12::StageShared;
+using g12::TrajParams;
 
 namespace {
 
@@ -55,25 +65,52 @@ int v_profile(const g12_params& p, double* prof) {
 
 bool correlated(const g12_params& p) { return p.U != 0.0 && !p.HF; }
 
-StageParams stage_params(const g12_params& p, const double* prof, int nV, int k, double dt_full, double U_eff,
-                         double wq, int Nq) {
-    StageParams sp{};
+bool valid_geometry(int Ns, int Nb) { return Ns >= 1 && Nb >= 1 && Nb <= G12_MAX_NB && Ns * Nb <= 64; }
+
+// The time protocol (steps, switching, quench ramps) and the precision are
+// shared by every trajectory of a run; the physics (sizes, hoppings,
+// interactions, quench strength and extent) is per trajectory.
+bool same_protocol(const g12_params& a, const g12_params& b) {
+    return a.dt == b.dt && a.time_steps == b.time_steps && a.AS_dt == b.AS_dt && a.AS_rate == b.AS_rate &&
+           a.AS_midpoint == b.AS_midpoint && a.AS_tail == b.AS_tail && a.quench_rate == b.quench_rate &&
+           a.quench_on_midpoint == b.quench_on_midpoint && a.quench_off_midpoint == b.quench_off_midpoint &&
+           a.q_type == b.q_type && a.quench_gate_time == b.quench_gate_time && a.precision == b.precision;
+}
+
+// Length of the adiabatic switching of a trajectory (AS.h:87-214): the loop
+// runs while fewer than AS_tail iterations have begun with U_AS == U exactly,
+// U_AS being the value set by the last stage of the previous iteration (the
+// first iteration never counts).  Depends on U through the rounding of
+// fermi(t) * U.
+int as_step_count(const g12_params& p) {
+    int count = 0, n = 0;
+    bool have = false;
+    double last = 0.0, t = 0.0;
+    while (count < p.AS_tail) {
+        if (have && std::fabs(last - p.U) <= 0) count += 1;
+        for (int k = 0; k < 4; ++k) {
+            const double dtk = (k == 1 || k == 2) ? p.AS_dt / 2 : p.AS_dt;
+            const double ts = k == 0 ? t : t + dtk;
+            last = g12_fermi(ts, p.AS_midpoint, p.AS_rate) * p.U;
+        }
+        have = true;
+        t += p.AS_dt;
+        if (++n > 100000000) break;  // unreachable for finite parameters
+    }
+    return n;
+}
+
+// Scalars of RK4 stage k of a step of length dt_full.
+StageShared stage_block(int k, double dt_full) {
+    StageShared ss{};
     const bool half = (k == 1 || k == 2);
-    sp.c = half ? dt_full / 2 : dt_full;
-    sp.wk = half ? 2.0 : 1.0;
-    sp.dt = dt_full;
-    sp.first = k == 0;
-    sp.last = k == 3;
-    sp.corr = correlated(p);
-    sp.U = U_eff;
-    sp.nV = nV;
-    for (int i = 0; i < nV; ++i) sp.V[i] = U_eff * prof[i];
-    sp.wq = wq;
-    sp.Nq = Nq;
-    sp.t1 = p.t1;
-    sp.t2 = p.t2;
-    for (int i = 0; i < G12_MAX_NB; ++i) sp.eps[i] = i < p.Nb ? p.eps[i] : 0.0;
-    return sp;
+    ss.c = half ? dt_full / 2 : dt_full;
+    ss.wk = half ? 2.0 : 1.0;
+    ss.dt = dt_full;
+    ss.first = k == 0;
+    ss.last = k == 3;
+    ss.lam = 1.0;
+    return ss;
 }
 
 struct Timing {
@@ -115,158 +152,587 @@ struct Timing {
     }
 };
 
+// Device memory of a run is one allocation carved into 256-byte aligned
+// pieces; the first pass (base == nullptr) only measures.
+struct Carver {
+    char* base;
+    int64_t off = 0;
+    template <class T>
+    T* take(int64_t count) {
+        if (count < 1) count = 1;
+        off = (off + 255) & ~int64_t(255);
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += (int64_t)sizeof(T) * count;
+        return p;
+    }
+};
+
 }  // namespace
 
-struct g12_ctx {
-    g12_params p;
-    DevState d;
-    hipStream_t stream = nullptr;
-    double prof[G12_MAX_SHELLS];
-    int nV = 0;
-    double t = 0.0;      // production time
-    int64_t steps = 0;   // production iterations begun
-    Timing timing;
-    int64_t bytes = 0;
-    std::vector<void*> allocs;
+namespace g12 {
 
-    template <class T>
-    int alloc(T** ptr, int64_t count) {
-        void* v = nullptr;
-        if (count <= 0) count = 1;
-        if (hipMalloc(&v, sizeof(T) * count) != hipSuccess) return G12_E_NOMEM;
-        allocs.push_back(v);
-        bytes += sizeof(T) * count;
-        *ptr = static_cast<T*>(v);
-        return 0;
-    }
+// A set of trajectories advanced together.
+struct Run {
+    int ntraj = 0;
+    std::vector<g12_params> p;
+    std::vector<DevState> ds;      // device pointers of each trajectory
+    std::vector<TrajParams> tp;
+    std::vector<int> as_steps;
+    int max_as = 0;
+    int maxN = 0;
+    bool any_corr = false;
+    int precision = 0;
+    DevState* d_ds = nullptr;
+    TrajParams* d_tp = nullptr;
+    int* d_fail = nullptr;          // [ntraj]
+    int* d_counter = nullptr;       // step within the current window
+    StageShared* d_table = nullptr; // [window][4]
+    StageShared* h_table = nullptr; // pinned staging of the table
+    int* h_fail = nullptr;          // pinned copy of the guard flags
+    int window = 256;               // steps per table window = trajectory-ring slots
+    std::vector<Bucket> buckets;
+    std::vector<std::vector<int>> lists;
+    std::vector<std::vector<int64_t>> prefixes;
+    BatchDev b{};
+    hipStream_t stream = nullptr;
+    Timing timing;
+    bool graphs = true;
+    bool autotune = true;
+    bool tuned = false;
+    hipGraphExec_t gx[2] = {nullptr, nullptr};  // one step: 0 adiabatic switching, 1 production
+    void* arena = nullptr;
+    int64_t bytes = 0;
+    double t = 0.0;     // production time (shared by all trajectories)
+    int64_t steps = 0;  // production steps begun
+};
+
+}  // namespace g12
+
+using g12::Run;
+
+struct g12_ctx {
+    Run r;
+};
+
+struct g12_batch {
+    Run r;
 };
 
 namespace {
 
-template <class F>
-void timed(g12_ctx* c, int cls, F&& launch) {
-    if (!c->timing.on) {
-        launch();
-        return;
+// Assigns every device pointer of the run from `base` (nullptr: measure only).
+int64_t layout(Run& r, char* base) {
+    Carver cv{base};
+    for (int i = 0; i < r.ntraj; ++i) {
+        DevState& d = r.ds[i];
+        const bool corr = r.tp[i].corr != 0;
+        const int64_t n4 = corr ? d.n4 : 1, n3 = corr ? d.n3 : 1;
+        d.S1 = cv.take<double2>(d.n2);
+        d.K1 = cv.take<double2>(d.n2);
+        d.A1 = cv.take<double2>(d.n2);
+        d.Y1 = cv.take<double2>(d.n2);
+        d.h = cv.take<double2>(d.n2);
+        d.I = cv.take<double2>(d.n2);
+        d.Wxy = cv.take<double>(d.n2);
+        d.Wxx = cv.take<double>(d.n2);
+        for (int s = 0; s < 2; ++s) {
+            d.S2[s] = cv.take<double2>(d.n4);  // the state is kept even when frozen
+            d.K2[s] = cv.take<double2>(n4);
+            d.A2[s] = cv.take<double2>(n4);
+            d.R2[s] = cv.take<double2>(n4);
+            d.D[s] = cv.take<double2>(n3);
+            d.F[s] = cv.take<double2>(n3);
+            d.Ft[s] = cv.take<double2>(n3);
+            d.Q[s] = cv.take<double2>(n3);
+            d.Qp[s] = cv.take<double2>(n3);
+        }
+        d.traj_slots = r.window;
+        d.traj = cv.take<double2>(d.n2 * r.window);
     }
-    hipEvent_t a = c->timing.get(), b = c->timing.get();
-    hipEventRecord(a, c->stream);
-    launch();
-    hipEventRecord(b, c->stream);
-    c->timing.pending[cls].push_back({a, b});
-    if (c->timing.pending[cls].size() > 4096) c->timing.resolve();
+    r.d_ds = cv.take<DevState>(r.ntraj);
+    r.d_tp = cv.take<TrajParams>(r.ntraj);
+    r.d_fail = cv.take<int>(r.ntraj);
+    for (int i = 0; i < r.ntraj; ++i) r.ds[i].fail = r.d_fail ? r.d_fail + i : nullptr;
+    r.d_counter = cv.take<int>(1);
+    r.d_table = cv.take<StageShared>(4 * (int64_t)r.window);
+    for (size_t k = 0; k < r.buckets.size(); ++k) {
+        r.buckets[k].list = cv.take<int>(r.buckets[k].n);
+        r.buckets[k].prefix = cv.take<int64_t>(r.buckets[k].n + 1);
+    }
+    return cv.off;
 }
 
-// One RK4 stage on the device.
-void run_stage(g12_ctx* c, const StageParams& sp) {
-    const int prec = c->p.precision;
-    hipStream_t s = c->stream;
-    timed(c, 2, [&] { g12::launch_stage_g1(c->d, sp, s); });
-    if (sp.corr) {
-        timed(c, 0, [&] { g12::launch_pass_strided(c->d, sp, prec, s); });
-        timed(c, 2, [&] { g12::launch_tables(c->d, sp, s); });
-    }
-    timed(c, 2, [&] { g12::launch_g1_rhs(c->d, sp, s); });
-    if (sp.corr) timed(c, 1, [&] { g12::launch_pass_fused(c->d, sp, prec, s); });
+void destroy_run(Run& r) {
+    if (r.stream) hipStreamSynchronize(r.stream);
+    r.timing.destroy();
+    for (auto& g : r.gx)
+        if (g) hipGraphExecDestroy(g);
+    if (r.arena) hipFree(r.arena);
+    if (r.h_table) hipHostFree(r.h_table);
+    if (r.h_fail) hipHostFree(r.h_fail);
+    if (r.stream) hipStreamDestroy(r.stream);
 }
 
-// Quench shift and extent for a production stage at time ts (RK_solver.h:141-151).
-void quench_at(const g12_params& p, double ts, double* w, int* nq) {
-    if (ts < p.quench_gate_time) {
-        *w = 0.0;
-        *nq = 0;
-        return;
+int create_run(Run& r, const g12_params* p, int ntraj) {
+    if (!p || ntraj < 1) return G12_E_ARG;
+    for (int i = 0; i < ntraj; ++i) {
+        if (!valid_geometry(p[i].Ns, p[i].Nb)) return G12_E_SIZE;
+        if (!same_protocol(p[0], p[i])) return G12_E_ARG;
     }
-    const double on = g12_fermi(ts, p.quench_on_midpoint, p.quench_rate);
-    if (p.q_type == G12_QUENCH_PULSE)
-        *w = (on * (1 - g12_fermi(ts, p.quench_off_midpoint, p.quench_rate))) * p.quench_strength;
-    else
-        *w = on * p.quench_strength;
-    *nq = p.Nq;
-}
-
-void production_step(g12_ctx* c) {
-    const g12_params& p = c->p;
-    for (int k = 0; k < 4; ++k) {
-        const double dtk = (k == 1 || k == 2) ? p.dt / 2 : p.dt;
-        const double ts = k == 0 ? c->t : c->t + dtk;
-        double wq;
-        int nq;
-        quench_at(p, ts, &wq, &nq);
-        run_stage(c, stage_params(p, c->prof, c->nV, k, p.dt, p.U, wq, nq));
+    if (select_device() < 0) return G12_E_HIP;
+    g12::device_cu_count();  // cached before any graph capture
+    r.ntraj = ntraj;
+    r.p.assign(p, p + ntraj);
+    r.precision = p[0].precision;
+    r.ds.assign(ntraj, DevState{});
+    r.tp.assign(ntraj, TrajParams{});
+    r.as_steps.assign(ntraj, 0);
+    for (int i = 0; i < ntraj; ++i) {
+        const g12_params& q = p[i];
+        DevState& d = r.ds[i];
+        d.Ns = q.Ns;
+        d.Nb = q.Nb;
+        d.N = q.Ns * q.Nb;
+        d.n2 = (int64_t)d.N * d.N;
+        d.n3 = d.n2 * d.N;
+        d.n4 = d.n2 * d.n2;
+        TrajParams& t = r.tp[i];
+        t.Ns = q.Ns;
+        t.Nb = q.Nb;
+        t.N = d.N;
+        t.corr = correlated(q);
+        t.Nq = q.Nq;
+        t.U = q.U;
+        t.t1 = q.t1;
+        t.t2 = q.t2;
+        t.qstr = q.quench_strength;
+        t.nV = v_profile(q, t.Vprof);
+        for (int o = 0; o < G12_MAX_NB; ++o) t.eps[o] = o < q.Nb ? q.eps[o] : 0.0;
+        r.as_steps[i] = as_step_count(q);
+        t.as_steps = r.as_steps[i];
+        r.max_as = std::max(r.max_as, t.as_steps);
+        r.maxN = std::max(r.maxN, d.N);
+        r.any_corr = r.any_corr || t.corr;
     }
-    c->t += p.dt;
-}
-
-int read_fail(g12_ctx* c, int* fail) {
-    HIPCHK(hipMemcpyAsync(fail, c->d.fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    // buckets of correlated trajectories by padded tile size, larger N first
+    for (int NP = 16; NP <= 64; NP += 16) {
+        std::vector<int> l;
+        for (int i = 0; i < ntraj; ++i)
+            if (r.tp[i].corr && (r.tp[i].N + 15) / 16 * 16 == NP) l.push_back(i);
+        if (l.empty()) continue;
+        std::stable_sort(l.begin(), l.end(), [&](int a, int b) { return r.tp[a].N > r.tp[b].N; });
+        std::vector<int64_t> pre(l.size() + 1, 0);
+        for (size_t j = 0; j < l.size(); ++j) pre[j + 1] = pre[j] + 2 * r.ds[l[j]].n2;
+        Bucket k{};
+        k.NP = NP;
+        k.n = (int)l.size();
+        k.tiles = pre.back();
+        k.waves = k.waves_fused = g12::default_waves(NP);
+        r.buckets.push_back(k);
+        r.lists.push_back(l);
+        r.prefixes.push_back(pre);
+    }
+    const int64_t total = layout(r, nullptr);
+    if (hipMalloc(&r.arena, total) != hipSuccess) {
+        r.arena = nullptr;
+        return G12_E_NOMEM;
+    }
+    r.bytes = total;
+    layout(r, static_cast<char*>(r.arena));
+    if (hipHostMalloc((void**)&r.h_table, sizeof(StageShared) * 4 * r.window, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&r.h_fail, sizeof(int) * ntraj, hipHostMallocDefault) != hipSuccess)
+        return G12_E_NOMEM;
+    HIPCHK(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
+    // every buffer starts defined (zero): the first stage reads K2/A2/R2 and
+    // the autotuner runs passes before any stage has written the tables
+    HIPCHK(hipMemsetAsync(r.arena, 0, total, r.stream));
+    HIPCHK(hipMemcpyAsync(r.d_ds, r.ds.data(), sizeof(DevState) * ntraj, hipMemcpyHostToDevice, r.stream));
+    HIPCHK(hipMemcpyAsync(r.d_tp, r.tp.data(), sizeof(TrajParams) * ntraj, hipMemcpyHostToDevice, r.stream));
+    for (size_t k = 0; k < r.buckets.size(); ++k) {
+        HIPCHK(hipMemcpyAsync(const_cast<int*>(r.buckets[k].list), r.lists[k].data(), sizeof(int) * r.lists[k].size(),
+                              hipMemcpyHostToDevice, r.stream));
+        HIPCHK(hipMemcpyAsync(const_cast<int64_t*>(r.buckets[k].prefix), r.prefixes[k].data(),
+                              sizeof(int64_t) * r.prefixes[k].size(), hipMemcpyHostToDevice, r.stream));
+    }
+    HIPCHK(hipStreamSynchronize(r.stream));
+    r.b.ds = r.d_ds;
+    r.b.tp = r.d_tp;
+    r.b.ntraj = ntraj;
+    r.b.maxN = r.maxN;
+    const char* e = std::getenv("G12KBA_GRAPHS");
+    if (e && e[0] == '0') r.graphs = false;
+    e = std::getenv("G12KBA_AUTOTUNE");
+    if (e && e[0] == '0') r.autotune = false;
     return 0;
 }
 
-bool valid_geometry(int Ns, int Nb) { return Ns >= 1 && Nb >= 1 && Nb <= G12_MAX_NB && Ns * Nb <= 64; }
+template <class F>
+void timed(Run& r, int cls, F&& launch) {
+    if (!r.timing.on) {
+        launch();
+        return;
+    }
+    hipEvent_t a = r.timing.get(), b = r.timing.get();
+    hipEventRecord(a, r.stream);
+    launch();
+    hipEventRecord(b, r.stream);
+    r.timing.pending[cls].push_back({a, b});
+    if (r.timing.pending[cls].size() > 4096) r.timing.resolve();
+}
+
+// One RK4 stage of every trajectory (RK_solver.h:120-203 / AS.h:111-200):
+// G1 stage set-up, strided commutator pass (+ D), Q/F tables and collision
+// integral, G1 right-hand side and update, fused commutator + G2 right-hand
+// side and update.
+void run_stage(Run& r, const StageRef& sr) {
+    hipStream_t s = r.stream;
+    timed(r, 2, [&] { g12::launch_b_stage_g1(r.b, sr, s); });
+    for (const Bucket& k : r.buckets) timed(r, 0, [&] { g12::launch_b_tile(r.b, k, sr, 0, r.precision, s); });
+    if (r.any_corr) timed(r, 2, [&] { g12::launch_b_tables(r.b, sr, s); });
+    timed(r, 2, [&] { g12::launch_b_g1_rhs(r.b, sr, s); });
+    for (const Bucket& k : r.buckets) timed(r, 1, [&] { g12::launch_b_tile(r.b, k, sr, 1, r.precision, s); });
+}
+
+// One whole step, reading its stage blocks at d_table[*d_counter] and
+// advancing the counter; production steps begin with the record + guard.
+void enqueue_step(Run& r, int phase) {
+    StageRef sr{};
+    sr.base = r.d_table;
+    sr.counter = r.d_counter;
+    if (phase == 1) timed(r, 2, [&] { g12::launch_b_record(r.b, sr, r.stream); });
+    for (int k = 0; k < 4; ++k) {
+        sr.stage = k;
+        run_stage(r, sr);
+    }
+    timed(r, 2, [&] { g12::launch_b_advance_counter(r.d_counter, r.stream); });
+}
+
+int capture(Run& r, int phase) {
+    if (r.gx[phase]) return 0;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(r.stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(r, phase);
+    HIPCHK(hipStreamEndCapture(r.stream, &g));
+    const hipError_t e = hipGraphInstantiate(&r.gx[phase], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        r.gx[phase] = nullptr;
+        std::fprintf(stderr, "g12kba: hipGraphInstantiate failed: %s\n", hipGetErrorString(e));
+        return G12_E_HIP;
+    }
+    return 0;
+}
+
+void drop_graphs(Run& r) {
+    for (auto& g : r.gx)
+        if (g) {
+            hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// Wave counts per bucket and pass, chosen by timing the compiled candidates on
+// the run's own tile lists.  Both passes are run as stage 0 of a step (the
+// strided pass writes only R2 and D, the fused pass only K2 and A2, all of
+// which the next stage 0 overwrites before reading), so tuning at a step
+// boundary leaves the state unchanged.  Results are cached per process by
+// (tile size, precision, tile count).
+int tune(Run& r) {
+    if (r.tuned || !r.autotune) return 0;
+    static std::map<std::tuple<int, int, int64_t>, std::pair<int, int>> cache;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    StageRef sr{};
+    sr.val = stage_block(0, 0.0);
+    int rc = 0;
+    for (Bucket& k : r.buckets) {
+        const auto key = std::make_tuple(k.NP, r.precision, k.tiles);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            k.waves = it->second.first;
+            k.waves_fused = it->second.second;
+            continue;
+        }
+        int opts[8];
+        const int nopt = g12::tile_wave_options(k.NP, opts);
+        int best[2] = {k.waves, k.waves_fused};
+        for (int fused = 0; fused < 2 && !rc; ++fused) {
+            float best_ms = 1e30f;
+            for (int o = 0; o < nopt; ++o) {
+                Bucket c = k;
+                c.waves = c.waves_fused = opts[o];
+                g12::launch_b_tile(r.b, c, sr, fused, r.precision, r.stream);  // warm-up
+                hipEventRecord(e0, r.stream);
+                for (int rep = 0; rep < 2; ++rep) g12::launch_b_tile(r.b, c, sr, fused, r.precision, r.stream);
+                hipEventRecord(e1, r.stream);
+                if (hipEventSynchronize(e1) != hipSuccess) {
+                    rc = G12_E_HIP;
+                    break;
+                }
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, e0, e1);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best[fused] = opts[o];
+                }
+            }
+        }
+        k.waves = best[0];
+        k.waves_fused = best[1];
+        cache[key] = {best[0], best[1]};
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    r.tuned = true;
+    drop_graphs(r);  // captured launch shapes are stale
+    return hipGetLastError() == hipSuccess ? 0 : G12_E_HIP;
+}
+
+// Runs `nsteps` steps of a phase in windows of r.window steps.  fill(step,
+// slot, blocks) writes the four stage blocks of a step; after(first, n) runs
+// once a window has completed (stream synchronised) and returns < 0 on error,
+// 1 to stop, 0 to go on.
+template <class Fill, class After>
+int run_windows(Run& r, int phase, int64_t nsteps, Fill&& fill, After&& after) {
+    int rc = tune(r);
+    if (rc) return rc;
+    const bool graph = r.graphs && !r.timing.on;
+    if (graph && (rc = capture(r, phase))) return rc;
+    for (int64_t w0 = 0; w0 < nsteps; w0 += r.window) {
+        const int n = (int)std::min<int64_t>(r.window, nsteps - w0);
+        for (int i = 0; i < n; ++i) fill(w0 + i, i, r.h_table + 4 * i);
+        HIPCHK(hipMemcpyAsync(r.d_table, r.h_table, sizeof(StageShared) * 4 * n, hipMemcpyHostToDevice, r.stream));
+        HIPCHK(hipMemsetAsync(r.d_counter, 0, sizeof(int), r.stream));
+        for (int i = 0; i < n; ++i) {
+            if (graph)
+                HIPCHK(hipGraphLaunch(r.gx[phase], r.stream));
+            else
+                enqueue_step(r, phase);
+        }
+        HIPCHK(hipMemcpyAsync(r.h_fail, r.d_fail, sizeof(int) * r.ntraj, hipMemcpyDeviceToHost, r.stream));
+        HIPCHK(hipStreamSynchronize(r.stream));
+        HIPCHK(hipGetLastError());
+        rc = after(w0, n);
+        if (rc) return rc < 0 ? rc : 0;
+    }
+    return 0;
+}
+
+// Adiabatic switching of every trajectory (AS.h:63-214): fixed step AS_dt,
+// U_AS = fermi(t) U per stage with V scaled alike, no quench; trajectory i
+// stops after its own as_steps[i] steps.
+int run_as(Run& r, int* steps) {
+    const g12_params& p = r.p[0];
+    double t = 0.0;
+    int rc = run_windows(
+        r, 0, r.max_as,
+        [&](int64_t n, int, StageShared* blk) {
+            for (int k = 0; k < 4; ++k) {
+                const double dtk = (k == 1 || k == 2) ? p.AS_dt / 2 : p.AS_dt;
+                const double ts = k == 0 ? t : t + dtk;
+                blk[k] = stage_block(k, p.AS_dt);
+                blk[k].lam = g12_fermi(ts, p.AS_midpoint, p.AS_rate);
+                blk[k].as_phase = 1;
+                blk[k].step = (int)n;
+            }
+            t += p.AS_dt;
+        },
+        [&](int64_t, int) { return 0; });
+    r.timing.resolve();
+    if (rc) return rc;
+    if (steps)
+        for (int i = 0; i < r.ntraj; ++i) steps[i] = r.as_steps[i];
+    return 0;
+}
+
+// Stage blocks of production step n at time t (RK_solver.h:120-151): quench
+// shift phi * strength on sites < Nq once the stage time reaches the gate.
+void production_blocks(const g12_params& p, double t, int64_t n, int slot, StageShared* blk) {
+    for (int k = 0; k < 4; ++k) {
+        const double dtk = (k == 1 || k == 2) ? p.dt / 2 : p.dt;
+        const double ts = k == 0 ? t : t + dtk;
+        blk[k] = stage_block(k, p.dt);
+        blk[k].step = (int)n;
+        blk[k].slot = slot;
+        if (ts >= p.quench_gate_time) {
+            const double on = g12_fermi(ts, p.quench_on_midpoint, p.quench_rate);
+            blk[k].gate = 1;
+            blk[k].phi = p.q_type == G12_QUENCH_PULSE ? on * (1 - g12_fermi(ts, p.quench_off_midpoint, p.quench_rate))
+                                                      : on;
+        }
+    }
+}
+
+// Production RK4 of every trajectory (RK_solver.h:64-226), `nsteps` steps from
+// the run's current time.  The G1 matrix before each step goes to traj[i]
+// (capacity cap matrices, from step index `rec0`) when traj is given.
+int run_production(Run& r, int64_t nsteps, double* const* traj, int64_t cap, int64_t rec0) {
+    const g12_params& p = r.p[0];
+    double t = r.t;
+    const int64_t s0 = r.steps;
+    int rc = run_windows(
+        r, 1, nsteps,
+        [&](int64_t i, int slot, StageShared* blk) {
+            production_blocks(p, t, s0 + i, slot, blk);
+            t += p.dt;
+        },
+        [&](int64_t w0, int n) -> int {
+            bool all_failed = true;
+            for (int i = 0; i < r.ntraj; ++i) {
+                const int f = r.h_fail[i];
+                if (!f) all_failed = false;
+                if (!traj || !traj[i]) continue;
+                // steps recorded in this window: up to and including the failing one
+                int64_t hi = s0 + w0 + n;
+                if (f && f < hi) hi = f;
+                const int64_t lo = s0 + w0;
+                const int64_t top = std::min(hi, rec0 + cap);
+                if (top <= lo || lo < rec0) continue;
+                const DevState& d = r.ds[i];
+                HIPCHK(hipMemcpyAsync(traj[i] + (lo - rec0) * d.n2 * 2, d.traj, sizeof(double2) * d.n2 * (top - lo),
+                                      hipMemcpyDeviceToHost, r.stream));
+            }
+            HIPCHK(hipStreamSynchronize(r.stream));
+            return all_failed ? 1 : 0;
+        });
+    r.t = t;
+    r.steps = s0 + nsteps;
+    r.timing.resolve();
+    return rc;
+}
+
+int64_t production_step_count(const g12_params& p) {
+    const double tmax = double(p.time_steps) * p.dt;
+    int64_t n = 0;
+    for (double t = 0.0; t < tmax; t += p.dt) ++n;  // RK_solver.h:96,216
+    return n;
+}
+
+int reset_guard(Run& r) {
+    HIPCHK(hipMemsetAsync(r.d_fail, 0, sizeof(int) * r.ntraj, r.stream));
+    HIPCHK(hipStreamSynchronize(r.stream));
+    return 0;
+}
+
+// Full production run: returns 0, or G12_GUARD if any trajectory's guard
+// tripped; steps[i] = trajectory length (iterations begun).
+int run_rk4(Run& r, double* const* traj, int64_t cap, int64_t* steps) {
+    int rc = reset_guard(r);
+    if (rc) return rc;
+    r.t = 0.0;
+    r.steps = 0;
+    const int64_t n = production_step_count(r.p[0]);
+    rc = run_production(r, n, traj, cap, 0);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(r.h_fail, r.d_fail, sizeof(int) * r.ntraj, hipMemcpyDeviceToHost));
+    bool guard = false;
+    for (int i = 0; i < r.ntraj; ++i) {
+        const int f = r.h_fail[i];
+        if (f) guard = true;
+        if (steps) steps[i] = f ? f : n;
+    }
+    return guard ? G12_GUARD : 0;
+}
+
+int run_advance(Run& r, int64_t nsteps) {
+    int rc = run_production(r, nsteps, nullptr, 0, 0);
+    if (rc) return rc;
+    for (int i = 0; i < r.ntraj; ++i)
+        if (r.h_fail[i]) return G12_GUARD;
+    return 0;
+}
+
+int set_state(Run& r, int i, const double* G1, const double* G2xx, const double* G2xy) {
+    if (i < 0 || i >= r.ntraj || !G1) return G12_E_ARG;
+    const DevState& d = r.ds[i];
+    HIPCHK(hipMemcpyAsync(d.S1, G1, sizeof(double2) * d.n2, hipMemcpyHostToDevice, r.stream));
+    const double* g2[2] = {G2xx, G2xy};
+    double2* tmp = nullptr;
+    if (G2xx || G2xy) HIPCHK(hipMalloc(&tmp, sizeof(double2) * d.n4));
+    for (int s = 0; s < 2; ++s) {
+        if (!g2[s]) {
+            hipMemsetAsync(d.S2[s], 0, sizeof(double2) * d.n4, r.stream);
+            continue;
+        }
+        hipMemcpyAsync(tmp, g2[s], sizeof(double2) * d.n4, hipMemcpyHostToDevice, r.stream);
+        g12::launch_ref_to_nat(tmp, d.S2[s], d.Ns, d.Nb, r.stream);
+    }
+    hipMemsetAsync(d.fail, 0, sizeof(int), r.stream);
+    const hipError_t e = hipStreamSynchronize(r.stream);
+    if (tmp) hipFree(tmp);
+    if (e != hipSuccess) return G12_E_HIP;
+    r.t = 0.0;
+    r.steps = 0;
+    return 0;
+}
+
+int get_state(Run& r, int i, double* G1, double* G2xx, double* G2xy) {
+    if (i < 0 || i >= r.ntraj) return G12_E_ARG;
+    const DevState& d = r.ds[i];
+    if (G1) HIPCHK(hipMemcpyAsync(G1, d.S1, sizeof(double2) * d.n2, hipMemcpyDeviceToHost, r.stream));
+    double* g2[2] = {G2xx, G2xy};
+    double2* tmp = nullptr;
+    if (G2xx || G2xy) HIPCHK(hipMalloc(&tmp, sizeof(double2) * d.n4));
+    for (int s = 0; s < 2; ++s) {
+        if (!g2[s]) continue;
+        g12::launch_nat_to_ref(d.S2[s], tmp, d.Ns, d.Nb, r.stream);
+        hipMemcpyAsync(g2[s], tmp, sizeof(double2) * d.n4, hipMemcpyDeviceToHost, r.stream);
+    }
+    const hipError_t e = hipStreamSynchronize(r.stream);
+    if (tmp) hipFree(tmp);
+    return e == hipSuccess ? 0 : G12_E_HIP;
+}
+
+int timing_enable(Run& r, int on) {
+    r.timing.resolve();
+    r.timing.on = on != 0;
+    for (int i = 0; i < 3; ++i) {
+        r.timing.count[i] = 0;
+        r.timing.ms[i] = 0;
+    }
+    return 0;
+}
+
+int timing_get(Run& r, int which, int64_t* launches, double* total_ms) {
+    if (which < 0 || which > 2) return G12_E_ARG;
+    r.timing.resolve();
+    if (launches) *launches = r.timing.count[which];
+    if (total_ms) *total_ms = r.timing.ms[which];
+    return 0;
+}
+
+int configure(Run& r, int use_graphs, int autotune) {
+    if (use_graphs >= 0) r.graphs = use_graphs != 0;
+    if (autotune >= 0) {
+        const bool want = autotune != 0;
+        if (!want && r.tuned) {  // back to the compiled defaults
+            for (Bucket& k : r.buckets) k.waves = k.waves_fused = g12::default_waves(k.NP);
+            drop_graphs(r);
+        }
+        r.autotune = want;
+        r.tuned = false;
+    }
+    return 0;
+}
 
 }  // namespace
 
 extern "C" {
 
-const char* g12_version(void) { return "g12kba-mi355x 0.1"; }
+const char* g12_version(void) { return "g12kba-mi355x 0.2"; }
 
 int g12_create(const g12_params* p, g12_ctx** out) {
     if (!p || !out) return G12_E_ARG;
     if (!valid_geometry(p->Ns, p->Nb)) return G12_E_SIZE;
-    if (select_device() < 0) return G12_E_HIP;
     g12_ctx* c = new (std::nothrow) g12_ctx();
     if (!c) return G12_E_NOMEM;
-    c->p = *p;
-    c->nV = v_profile(c->p, c->prof);
-    DevState& d = c->d;
-    d.Ns = p->Ns;
-    d.Nb = p->Nb;
-    d.N = p->Ns * p->Nb;
-    d.n2 = (int64_t)d.N * d.N;
-    d.n3 = d.n2 * d.N;
-    d.n4 = d.n2 * d.n2;
-    int rc = 0;
-    rc |= c->alloc(&d.S1, d.n2);
-    rc |= c->alloc(&d.K1, d.n2);
-    rc |= c->alloc(&d.A1, d.n2);
-    rc |= c->alloc(&d.Y1, d.n2);
-    rc |= c->alloc(&d.h, d.n2);
-    rc |= c->alloc(&d.I, d.n2);
-    rc |= c->alloc(&d.Wxy, d.n2);
-    rc |= c->alloc(&d.Wxx, d.n2);
-    const bool corr = correlated(*p);
-    const int64_t n4 = corr ? d.n4 : 1, n3 = corr ? d.n3 : 1;
-    for (int s = 0; s < 2; ++s) {
-        rc |= c->alloc(&d.S2[s], d.n4);  // state is kept even when frozen
-        rc |= c->alloc(&d.K2[s], n4);
-        rc |= c->alloc(&d.A2[s], n4);
-        rc |= c->alloc(&d.R2[s], n4);
-        rc |= c->alloc(&d.D[s], n3);
-        rc |= c->alloc(&d.F[s], n3);
-        rc |= c->alloc(&d.Ft[s], n3);
-        rc |= c->alloc(&d.Q[s], n3);
-        rc |= c->alloc(&d.Qp[s], n3);
-    }
-    rc |= c->alloc(&d.fail, 1);
-    d.traj_slots = 256;
-    rc |= c->alloc(&d.traj, d.n2 * d.traj_slots);
+    const int rc = create_run(c->r, p, 1);
     if (rc) {
-        g12_destroy(c);
-        return G12_E_NOMEM;
-    }
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        g12_destroy(c);
-        return G12_E_HIP;
-    }
-    hipMemsetAsync(d.fail, 0, sizeof(int), c->stream);
-    hipMemsetAsync(d.S1, 0, sizeof(double2) * d.n2, c->stream);
-    hipMemsetAsync(d.I, 0, sizeof(double2) * d.n2, c->stream);
-    for (int s = 0; s < 2; ++s) hipMemsetAsync(d.S2[s], 0, sizeof(double2) * d.n4, c->stream);
-    if (hipStreamSynchronize(c->stream) != hipSuccess) {
-        g12_destroy(c);
-        return G12_E_HIP;
+        destroy_run(c->r);
+        delete c;
+        return rc;
     }
     *out = c;
     return 0;
@@ -274,10 +740,7 @@ int g12_create(const g12_params* p, g12_ctx** out) {
 
 int g12_destroy(g12_ctx* c) {
     if (!c) return 0;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    c->timing.destroy();
-    for (void* v : c->allocs) hipFree(v);
-    if (c->stream) hipStreamDestroy(c->stream);
+    destroy_run(c->r);
     delete c;
     return 0;
 }
@@ -289,169 +752,139 @@ int g12_synchronize(void) {
 
 int g12_memory_bytes(g12_ctx* c, int64_t* bytes) {
     if (!c || !bytes) return G12_E_ARG;
-    *bytes = c->bytes;
+    *bytes = c->r.bytes;
     return 0;
 }
 
 int g12_set_state(g12_ctx* c, const double* G1, const double* G2xx, const double* G2xy) {
-    if (!c || !G1) return G12_E_ARG;
-    DevState& d = c->d;
-    HIPCHK(hipMemcpyAsync(d.S1, G1, sizeof(double2) * d.n2, hipMemcpyHostToDevice, c->stream));
-    const double* g2[2] = {G2xx, G2xy};
-    double2* tmp = nullptr;
-    HIPCHK(hipMalloc(&tmp, sizeof(double2) * d.n4));
-    for (int s = 0; s < 2; ++s) {
-        if (!g2[s]) {
-            hipMemsetAsync(d.S2[s], 0, sizeof(double2) * d.n4, c->stream);
-            continue;
-        }
-        hipMemcpyAsync(tmp, g2[s], sizeof(double2) * d.n4, hipMemcpyHostToDevice, c->stream);
-        g12::launch_ref_to_nat(tmp, d.S2[s], d.Ns, d.Nb, c->stream);
-    }
-    hipMemsetAsync(d.fail, 0, sizeof(int), c->stream);
-    hipError_t e = hipStreamSynchronize(c->stream);
-    hipFree(tmp);
-    if (e != hipSuccess) return G12_E_HIP;
-    c->t = 0.0;
-    c->steps = 0;
-    return 0;
+    if (!c) return G12_E_ARG;
+    return set_state(c->r, 0, G1, G2xx, G2xy);
 }
 
 int g12_get_state(g12_ctx* c, double* G1, double* G2xx, double* G2xy) {
     if (!c) return G12_E_ARG;
-    DevState& d = c->d;
-    if (G1) HIPCHK(hipMemcpyAsync(G1, d.S1, sizeof(double2) * d.n2, hipMemcpyDeviceToHost, c->stream));
-    double* g2[2] = {G2xx, G2xy};
-    double2* tmp = nullptr;
-    if (G2xx || G2xy) HIPCHK(hipMalloc(&tmp, sizeof(double2) * d.n4));
-    for (int s = 0; s < 2; ++s) {
-        if (!g2[s]) continue;
-        g12::launch_nat_to_ref(d.S2[s], tmp, d.Ns, d.Nb, c->stream);
-        hipMemcpyAsync(g2[s], tmp, sizeof(double2) * d.n4, hipMemcpyDeviceToHost, c->stream);
-    }
-    hipError_t e = hipStreamSynchronize(c->stream);
-    if (tmp) hipFree(tmp);
-    return e == hipSuccess ? 0 : G12_E_HIP;
+    return get_state(c->r, 0, G1, G2xx, G2xy);
 }
 
 int g12_get_time(g12_ctx* c, double* t, int64_t* steps) {
     if (!c) return G12_E_ARG;
-    if (t) *t = c->t;
-    if (steps) *steps = c->steps;
+    if (t) *t = c->r.t;
+    if (steps) *steps = c->r.steps;
     return 0;
 }
 
-// Adiabatic switching (AS.h:63-214): fixed step AS_dt, U_AS = fermi(t) U per
-// stage with V scaled alike, no quench, until AS_tail iterations have begun
-// with U_AS == U (the first iteration never counts).
 int g12_adiabatic_switching(g12_ctx* c, int* steps) {
     if (!c) return G12_E_ARG;
-    const g12_params& p = c->p;
-    int count = 0, n = 0;
-    bool have = false;
-    double last = 0.0, t = 0.0;
-    while (count < p.AS_tail) {
-        if (have && std::fabs(last - p.U) <= 0) count += 1;
-        for (int k = 0; k < 4; ++k) {
-            const double dtk = (k == 1 || k == 2) ? p.AS_dt / 2 : p.AS_dt;
-            const double ts = k == 0 ? t : t + dtk;
-            last = g12_fermi(ts, p.AS_midpoint, p.AS_rate) * p.U;
-            run_stage(c, stage_params(p, c->prof, c->nV, k, p.AS_dt, last, 0.0, 0));
-        }
-        have = true;
-        t += p.AS_dt;
-        ++n;
-        if (n > 100000000) break;  // unreachable for finite parameters
-    }
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (steps) *steps = n;
-    return 0;
+    return run_as(c->r, steps);
 }
 
-// Production RK4 (RK_solver.h:64-226).
 int g12_rk4(g12_ctx* c, double* traj, int64_t traj_cap, int64_t* steps) {
     if (!c) return G12_E_ARG;
-    const g12_params& p = c->p;
-    const double tmax = double(p.time_steps) * p.dt;
-    DevState& d = c->d;
-    c->t = 0.0;
-    c->steps = 0;
-    HIPCHK(hipMemsetAsync(d.fail, 0, sizeof(int), c->stream));
-    int64_t n = 0, flushed = 0;
-    int fail = 0;
-    auto flush = [&]() -> int {
-        // copy recorded slots [flushed, n) to the host trajectory
-        if (traj && flushed < n && flushed < traj_cap) {
-            const int64_t hi = n < traj_cap ? n : traj_cap;
-            for (int64_t i = flushed; i < hi; ++i) {
-                const int slot = (int)(i % d.traj_slots);
-                HIPCHK(hipMemcpyAsync(traj + i * d.n2 * 2, d.traj + slot * d.n2, sizeof(double2) * d.n2,
-                                      hipMemcpyDeviceToHost, c->stream));
-            }
-        }
-        flushed = n;
-        return read_fail(c, &fail);
-    };
-    while (c->t < tmax) {
-        g12::launch_record(d, (int)(n % d.traj_slots), n, c->stream);
-        ++n;
-        production_step(c);
-        if (n - flushed >= d.traj_slots) {
-            int rc = flush();
-            if (rc) return rc;
-            if (fail) break;
-        }
-    }
-    int rc = flush();
-    if (rc) return rc;
-    c->timing.resolve();
-    if (fail) {
-        c->steps = fail;
-        if (steps) *steps = fail;
-        return G12_GUARD;
-    }
-    c->steps = n;
-    if (steps) *steps = n;
-    return 0;
+    double* tr[1] = {traj};
+    return run_rk4(c->r, traj ? tr : nullptr, traj_cap, steps);
 }
 
 int g12_rk4_advance(g12_ctx* c, int64_t nsteps, int64_t* done) {
     if (!c || nsteps < 0) return G12_E_ARG;
-    DevState& d = c->d;
-    int fail = 0;
-    const double2* keep = d.traj;
-    for (int64_t i = 0; i < nsteps; ++i) {
-        DevState dd = d;
-        dd.traj = nullptr;  // guard only
-        g12::launch_record(dd, 0, c->steps, c->stream);
-        c->steps += 1;
-        production_step(c);
-    }
-    (void)keep;
-    int rc = read_fail(c, &fail);
-    if (rc) return rc;
-    c->timing.resolve();
+    const int rc = run_advance(c->r, nsteps);
+    if (rc < 0) return rc;
     if (done) *done = nsteps;
-    return fail ? G12_GUARD : 0;
+    return rc;
 }
 
-int g12_timing_enable(g12_ctx* c, int on) {
-    if (!c) return G12_E_ARG;
-    c->timing.resolve();
-    c->timing.on = on != 0;
-    for (int i = 0; i < 3; ++i) {
-        c->timing.count[i] = 0;
-        c->timing.ms[i] = 0;
-    }
-    return 0;
-}
+int g12_timing_enable(g12_ctx* c, int on) { return c ? timing_enable(c->r, on) : G12_E_ARG; }
 
 int g12_timing_get(g12_ctx* c, int which, int64_t* launches, double* total_ms) {
-    if (!c || which < 0 || which > 2) return G12_E_ARG;
-    c->timing.resolve();
-    if (launches) *launches = c->timing.count[which];
-    if (total_ms) *total_ms = c->timing.ms[which];
+    return c ? timing_get(c->r, which, launches, total_ms) : G12_E_ARG;
+}
+
+int g12_configure(g12_ctx* c, int use_graphs, int autotune) {
+    return c ? configure(c->r, use_graphs, autotune) : G12_E_ARG;
+}
+
+// ---- runs of many trajectories ---------------------------------------------
+
+int g12_batch_create(const g12_params* p, int ntraj, g12_batch** out) {
+    if (!p || !out || ntraj < 1) return G12_E_ARG;
+    g12_batch* b = new (std::nothrow) g12_batch();
+    if (!b) return G12_E_NOMEM;
+    const int rc = create_run(b->r, p, ntraj);
+    if (rc) {
+        destroy_run(b->r);
+        delete b;
+        return rc;
+    }
+    *out = b;
     return 0;
+}
+
+int g12_batch_destroy(g12_batch* b) {
+    if (!b) return 0;
+    destroy_run(b->r);
+    delete b;
+    return 0;
+}
+
+int g12_batch_size(g12_batch* b, int* ntraj) {
+    if (!b || !ntraj) return G12_E_ARG;
+    *ntraj = b->r.ntraj;
+    return 0;
+}
+
+int g12_batch_memory_bytes(g12_batch* b, int64_t* bytes) {
+    if (!b || !bytes) return G12_E_ARG;
+    *bytes = b->r.bytes;
+    return 0;
+}
+
+int g12_batch_set_state(g12_batch* b, int i, const double* G1, const double* G2xx, const double* G2xy) {
+    return b ? set_state(b->r, i, G1, G2xx, G2xy) : G12_E_ARG;
+}
+
+int g12_batch_get_state(g12_batch* b, int i, double* G1, double* G2xx, double* G2xy) {
+    return b ? get_state(b->r, i, G1, G2xx, G2xy) : G12_E_ARG;
+}
+
+int g12_batch_adiabatic_switching(g12_batch* b, int* steps) { return b ? run_as(b->r, steps) : G12_E_ARG; }
+
+int g12_batch_rk4(g12_batch* b, double* const* traj, int64_t traj_cap, int64_t* steps) {
+    return b ? run_rk4(b->r, traj, traj_cap, steps) : G12_E_ARG;
+}
+
+int g12_batch_rk4_advance(g12_batch* b, int64_t nsteps) {
+    if (!b || nsteps < 0) return G12_E_ARG;
+    return run_advance(b->r, nsteps);
+}
+
+int g12_batch_guard(g12_batch* b, int* fail) {
+    if (!b || !fail) return G12_E_ARG;
+    HIPCHK(hipMemcpy(fail, b->r.d_fail, sizeof(int) * b->r.ntraj, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int g12_batch_timing_enable(g12_batch* b, int on) { return b ? timing_enable(b->r, on) : G12_E_ARG; }
+
+int g12_batch_timing_get(g12_batch* b, int which, int64_t* launches, double* total_ms) {
+    return b ? timing_get(b->r, which, launches, total_ms) : G12_E_ARG;
+}
+
+int g12_batch_configure(g12_batch* b, int use_graphs, int autotune) {
+    return b ? configure(b->r, use_graphs, autotune) : G12_E_ARG;
+}
+
+int g12_batch_tune(g12_batch* b) { return b ? tune(b->r) : G12_E_ARG; }
+
+int g12_batch_buckets(g12_batch* b, int cap, int* NP, int* count, int* waves_strided, int* waves_fused) {
+    if (!b) return G12_E_ARG;
+    const Run& r = b->r;
+    const int n = (int)r.buckets.size();
+    for (int k = 0; k < n && k < cap; ++k) {
+        if (NP) NP[k] = r.buckets[k].NP;
+        if (count) count[k] = r.buckets[k].n;
+        if (waves_strided) waves_strided[k] = r.buckets[k].waves;
+        if (waves_fused) waves_fused[k] = r.buckets[k].waves_fused;
+    }
+    return n;
 }
 
 }  // extern "C"
@@ -557,7 +990,7 @@ int g12_h2_commutator(const double* G2, const double* h1x, const double* h1y, do
         (rc = up(dhy, h1y, sizeof(double2) * n2)))
         return rc;
     g12::launch_ref_to_nat(draw, dT, Ns, Nb, api_stream());
-    g12::launch_comm2(dT, dhx, dhy, dC, nullptr, N, precision, api_stream());
+    g12::launch_comm2(dT, dhx, dhy, dC, N, precision, api_stream());
     g12::launch_nat_to_ref(dC, draw, Ns, Nb, api_stream());
     return down(comm, draw, sizeof(double2) * n4);
 }
@@ -700,16 +1133,16 @@ int g12_g2_step(const double* comm, const double* psi, const double* pi, double*
 
 int g12_save_state(g12_ctx* c, const char* path) {
     if (!c || !path) return G12_E_ARG;
-    const DevState& d = c->d;
+    const DevState& d = c->r.ds[0];
     std::vector<double> g1(2 * d.n2), gxx(2 * d.n4), gxy(2 * d.n4);
     int rc = g12_get_state(c, g1.data(), gxx.data(), gxy.data());
     if (rc) return rc;
-    return g12io::save_checkpoint(path, d.Ns, d.Nb, c->t, c->steps, g1.data(), gxx.data(), gxy.data());
+    return g12io::save_checkpoint(path, d.Ns, d.Nb, c->r.t, c->r.steps, g1.data(), gxx.data(), gxy.data());
 }
 
 int g12_load_state(g12_ctx* c, const char* path) {
     if (!c || !path) return G12_E_ARG;
-    const DevState& d = c->d;
+    const DevState& d = c->r.ds[0];
     std::vector<double> g1(2 * d.n2), gxx(2 * d.n4), gxy(2 * d.n4);
     double t = 0;
     int64_t steps = 0;
@@ -717,8 +1150,8 @@ int g12_load_state(g12_ctx* c, const char* path) {
     if (rc) return rc;
     rc = g12_set_state(c, g1.data(), gxx.data(), gxy.data());
     if (rc) return rc;
-    c->t = t;
-    c->steps = steps;
+    c->r.t = t;
+    c->r.steps = steps;
     return 0;
 }
 

@@ -8,7 +8,8 @@
 
 namespace g12 {
 
-// Per-stage scalars of one RK4 stage (computed on the host).
+// Scalars of one RK4 stage as the derivative routines see them (built on the
+// device from TrajParams + StageShared, or on the host for routine calls).
 struct StageParams {
     double c;         // Y = S + c * K_prev   (stage 0: K_prev is zero)
     double wk;        // RK weight 1, 2, 2, 1
@@ -25,6 +26,40 @@ struct StageParams {
     double eps[G12_MAX_NB];
 };
 
+// Per-trajectory constants.
+struct TrajParams {
+    int Ns, Nb, N;
+    int corr;          // U != 0 and not HF
+    int as_steps;      // length of this trajectory's adiabatic switching
+    int Nq;
+    int nV;
+    double U, t1, t2;
+    double qstr;       // quench strength
+    double Vprof[G12_MAX_SHELLS];  // V_n = U_eff * Vprof[n-1]
+    double eps[G12_MAX_NB];
+};
+
+// Per-stage scalars shared by every trajectory of a batch.
+struct StageShared {
+    double c, wk, dt;
+    int first, last;
+    double lam;        // adiabatic switching: U_eff = lam * U (lam = fermi(t)); production: U_eff = U
+    double phi;        // quench ramp: shift = phi * qstr on sites < Nq when gate = 1
+    int gate;
+    int as_phase;      // 1 during adiabatic switching
+    int step;          // adiabatic-switching step index (trajectory stops after as_steps)
+    int slot;          // trajectory-record slot of this step (production)
+};
+
+// Where a kernel finds its StageShared: by value, or (graph replay) at
+// base[*counter * 4 + stage] in device memory.
+struct StageRef {
+    StageShared val;
+    const StageShared* base;
+    const int* counter;
+    int stage;
+};
+
 // Device buffers of one trajectory.  G2 tensors are stored in the natural
 // layout T[p1][p2][q1][q2] (pair index p1*N + p2, one-particle index
 // p = site + orb*Ns); index 0 is xxxx, 1 is xyxy.
@@ -38,25 +73,48 @@ struct DevState {
     double2 *F[2], *Ft[2];             // F[X][Y][Z], Ft[Z][Y][X]
     double2 *Q[2], *Qp[2];             // Q[Z][X][Y], Qp[X][Y][Z]
     int* fail;                         // 0 while running, step+1 after the guard trips
-    double2* traj;                     // device trajectory ring (traj_slots x N^2)
+    double2* traj;                     // device trajectory ring (traj_slots x N^2) or null
     int traj_slots;
 };
 
-// kernel launchers (kernels.hip)
-void launch_stage_g1(const DevState& d, const StageParams& sp, hipStream_t s);
-void launch_pass_strided(const DevState& d, const StageParams& sp, int precision, hipStream_t s);
-void launch_tables(const DevState& d, const StageParams& sp, hipStream_t s);
-void launch_g1_rhs(const DevState& d, const StageParams& sp, hipStream_t s);
-void launch_pass_fused(const DevState& d, const StageParams& sp, int precision, hipStream_t s);
-void launch_record(const DevState& d, int slot, int64_t step, hipStream_t s);
+// A set of trajectories advanced together: device arrays of states and
+// constants, and (for the tile passes) buckets of trajectories with the same
+// padded tile size.
+struct BatchDev {
+    const DevState* ds;      // device array [ntraj]
+    const TrajParams* tp;    // device array [ntraj]
+    int ntraj;
+    int maxN;
+};
+
+struct Bucket {
+    int NP;                  // 16, 32, 48 or 64
+    const int* list;         // device array of trajectory indices
+    const int64_t* prefix;   // device array [n + 1]: tiles before each listed trajectory (both spins)
+    int n;
+    int64_t tiles;
+    int waves;               // waves per workgroup, strided pass (autotuned)
+    int waves_fused;         // waves per workgroup, fused pass (autotuned)
+    int wg_per_cu;           // workgroups per CU cap (0: as many as LDS and waves allow)
+};
+
+// batched stage kernels (kernels.hip)
+void launch_b_stage_g1(const BatchDev& b, const StageRef& sr, hipStream_t s);
+void launch_b_tables(const BatchDev& b, const StageRef& sr, hipStream_t s);
+void launch_b_g1_rhs(const BatchDev& b, const StageRef& sr, hipStream_t s);
+void launch_b_record(const BatchDev& b, const StageRef& sr, hipStream_t s);
+void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int fused, int precision, hipStream_t s);
+void launch_b_advance_counter(int* counter, hipStream_t s);
+int device_cu_count();
+int default_waves(int NP);
+int tile_wave_options(int NP, int* opts);  // compiled wave counts for NP; returns their number
 
 // routine-level helpers (kernels.hip)
 void launch_ref_to_nat(const double2* in, double2* out, int Ns, int Nb, hipStream_t s);
 void launch_nat_to_ref(const double2* in, double2* out, int Ns, int Nb, hipStream_t s);
-// out = (accumulate ? out : 0) + [hx.X - X.hy] restricted by hmode bits
-// (1: left term, 2: right term) over all four pair indices.
-void launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, double2* tmp,
-                  int N, int precision, hipStream_t s);
+// out = [hx (x) 1 + 1 (x) hx] T - T [hy (x) 1 + 1 (x) hy] (natural layout)
+void launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
+                  hipStream_t s);
 void launch_hf(double2* h, const double2* G1, int Ns, int Nb, const StageParams& sp, hipStream_t s);
 void launch_comm1(const double2* G, const double2* h, double2* out, int N, hipStream_t s);
 void launch_d_gather(const double2* T, double2* D, int N, hipStream_t s);

@@ -40,6 +40,16 @@ __device__ __forceinline__ double2 cmul_mi(double2 a) { return make_double2(a.y,
 
 __device__ __forceinline__ bool failed(const int* fail) { return fail && *fail != 0; }
 
+// Workgroup barrier that orders LDS only: the tile loop reuses its LDS buffers
+// but never exchanges global memory between waves, so waiting for the
+// scattered global stores of the previous tile (as a full __syncthreads fence
+// would) only stalls the MFMA pipeline.
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---------------------------------------------------------------------------
 // Interaction tables W_xy / W_xx (N x N):  U on one site, V_n at distance
 // n <= nV, 0 beyond; W_xx(X,X) = 0.  (The loops of HF.h, psi.h, pi.h and
@@ -125,9 +135,51 @@ __device__ inline double2 hf_element_diag(const double2* G, const double2* diag,
     return v;
 }
 
-__global__ __launch_bounds__(1024) void k_stage_g1(DevState d, StageParams sp) {
-    if (failed(d.fail)) return;
+// Per-stage scalars of trajectory t from its constants and the shared stage
+// block (the reference's stage set-up, RK_solver.h:120-151 / AS.h:111-141).
+__device__ inline StageShared stage_of(const StageRef& r) {
+    if (r.base) return r.base[(*r.counter) * 4 + r.stage];
+    return r.val;
+}
+
+__device__ inline void fill_sp(StageParams& sp, const TrajParams& t, const StageShared& ss) {
+    sp.c = ss.c;
+    sp.wk = ss.wk;
+    sp.dt = ss.dt;
+    sp.first = ss.first;
+    sp.last = ss.last;
+    sp.corr = t.corr;
+    sp.U = ss.as_phase ? ss.lam * t.U : t.U;  // U_AS = fermi(t) U during switching
+    sp.nV = t.nV;
+    for (int i = 0; i < t.nV; ++i) sp.V[i] = sp.U * t.Vprof[i];
+    sp.wq = ss.gate ? ss.phi * t.qstr : 0.0;
+    sp.Nq = ss.gate ? t.Nq : 0;
+    sp.t1 = t.t1;
+    sp.t2 = t.t2;
+    for (int i = 0; i < G12_MAX_NB; ++i) sp.eps[i] = t.eps[i];
+}
+
+// A trajectory takes part in a stage unless its guard tripped or its own
+// adiabatic switching is already complete.
+__device__ inline bool active(const DevState& d, const TrajParams& t, const StageShared& ss) {
+    if (failed(d.fail)) return false;
+    return !(ss.as_phase && ss.step >= t.as_steps);
+}
+
+// Stage set-up for G1 (one workgroup per trajectory): Y1 = S1 + c K1,
+// h = h_HF[Y1], W tables.
+__global__ __launch_bounds__(1024) void k_stage_g1(BatchDev b, StageRef r) {
+    const int ti = blockIdx.y;
+    const DevState& d = b.ds[ti];
+    const TrajParams& tp = b.tp[ti];
+    const StageShared ss = stage_of(r);
+    if (!active(d, tp, ss)) return;
+    // built once per block in LDS: the V table is indexed at run time and
+    // would otherwise live in per-lane scratch
+    __shared__ StageParams sp;
+    if (threadIdx.x == 0) fill_sp(sp, tp, ss);
     __shared__ double2 diag[G12_MAX_NB * 64];
+    __syncthreads();
     const int N = d.N;
     const int n2 = N * N;
     for (int e = threadIdx.x; e < n2; e += blockDim.x) {
@@ -139,9 +191,7 @@ __global__ __launch_bounds__(1024) void k_stage_g1(DevState d, StageParams sp) {
     __syncthreads();
     for (int e = threadIdx.x; e < n2; e += blockDim.x) {
         const int X = e / N, Y = e % N;
-        const double2 y = d.Y1[e];
         d.h[e] = hf_element_diag(d.Y1, diag, X, Y, d.Ns, d.Nb, sp);
-        (void)y;
         const double w = w_xy(X, Y, d.Ns, sp);
         d.Wxy[e] = w;
         d.Wxx[e] = (X == Y) ? 0.0 : w;
@@ -152,12 +202,10 @@ __global__ __launch_bounds__(1024) void k_stage_g1(DevState d, StageParams sp) {
 // Q^s[Z][X][Y] = sum_w Y1[X][w] W^s[Z][w] Y1[w][Y] and Qp^s[X][Y][Z] = Q^s[Z][X][Y]
 // for s = xx (index 0, W_xx) and xy (index 1, W_xy).  One block per (Z, X),
 // threads over Y.
-__global__ __launch_bounds__(64) void k_q_tables(DevState d) {
-    if (failed(d.fail)) return;
+__device__ inline void q_tables_block(const DevState& d, int Z, int X) {
     __shared__ double2 u[G12_MAX_NB * 64];
     __shared__ double wz[G12_MAX_NB * 64];
     const int N = d.N;
-    const int Z = blockIdx.x / N, X = blockIdx.x % N;
     const double2* G = d.Y1;
     for (int w = threadIdx.x; w < N; w += blockDim.x) {
         wz[w] = d.Wxy[(int64_t)w * N + Z];  // W_xy is symmetric
@@ -187,11 +235,9 @@ __global__ __launch_bounds__(64) void k_q_tables(DevState d) {
 // (Pi_xyxy with the swapped G2 pair, pi.h:25-95 / RK_solver.h:163-164), the
 // transposed copies Ft[Z][Y][X], and I[X][Y] = F^xx[X][Y][X]
 // (collision_int, collision_integral.h:27-69).  One block per (X, Y).
-__global__ __launch_bounds__(64) void k_f_tables(DevState d) {
-    if (failed(d.fail)) return;
+__device__ inline void f_tables_block(const DevState& d, int X, int Y) {
     __shared__ double2 dxx[G12_MAX_NB * 64], dxy[G12_MAX_NB * 64];
     const int N = d.N;
-    const int X = blockIdx.x / N, Y = blockIdx.x % N;
     for (int w = threadIdx.x; w < N; w += blockDim.x) {
         dxx[w] = d.D[0][((int64_t)X * N + Y) * N + w];
         dxy[w] = d.D[1][((int64_t)X * N + Y) * N + w];
@@ -219,14 +265,37 @@ __global__ __launch_bounds__(64) void k_f_tables(DevState d) {
     }
 }
 
+__global__ __launch_bounds__(64) void k_q_tables(BatchDev b, StageRef r) {
+    const int ti = blockIdx.y;
+    const DevState& d = b.ds[ti];
+    if (!b.tp[ti].corr || !active(d, b.tp[ti], stage_of(r))) return;
+    const int N = d.N;
+    if ((int)blockIdx.x >= N * N) return;
+    q_tables_block(d, blockIdx.x / N, blockIdx.x % N);
+}
+
+__global__ __launch_bounds__(64) void k_f_tables(BatchDev b, StageRef r) {
+    const int ti = blockIdx.y;
+    const DevState& d = b.ds[ti];
+    if (!b.tp[ti].corr || !active(d, b.tp[ti], stage_of(r))) return;
+    const int N = d.N;
+    if ((int)blockIdx.x >= N * N) return;
+    f_tables_block(d, blockIdx.x / N, blockIdx.x % N);
+}
+
 // G1 right-hand side: comm = [h, Y1], K = -i(comm + I + I^dag), RK
 // accumulation (G1_step, G1_G2_t_step.h:24-47) and the final update.  One
 // block per row X; the rows X of h and Y1 are staged in LDS.
-__global__ __launch_bounds__(64) void k_g1_rhs(DevState d, StageParams sp) {
-    if (failed(d.fail)) return;
-    __shared__ double2 hrow[G12_MAX_NB * 64], grow[G12_MAX_NB * 64];
+__global__ __launch_bounds__(64) void k_g1_rhs(BatchDev b, StageRef r) {
+    const int ti = blockIdx.y;
+    const DevState& d = b.ds[ti];
+    const TrajParams& tp = b.tp[ti];
+    const StageShared ss = stage_of(r);
+    if (!active(d, tp, ss)) return;
     const int N = d.N;
     const int X = blockIdx.x;
+    if (X >= N) return;
+    __shared__ double2 hrow[G12_MAX_NB * 64], grow[G12_MAX_NB * 64];
     for (int z = threadIdx.x; z < N; z += blockDim.x) {
         hrow[z] = d.h[(int64_t)X * N + z];
         grow[z] = d.Y1[(int64_t)X * N + z];
@@ -240,36 +309,44 @@ __global__ __launch_bounds__(64) void k_g1_rhs(DevState d, StageParams sp) {
             gh = cadd(gh, cmul(grow[z], d.h[(int64_t)z * N + Y]));
         }
         double2 v = csub(hg, gh);
-        if (sp.corr) v = cadd(cadd(v, d.I[e]), cconj(d.I[(int64_t)Y * N + X]));
+        if (tp.corr) v = cadd(cadd(v, d.I[e]), cconj(d.I[(int64_t)Y * N + X]));
         const double2 K = cmul_mi(v);
-        const double2 wK = cscale(K, sp.wk);
-        if (sp.first) {
+        const double2 wK = cscale(K, ss.wk);
+        if (ss.first) {
             d.A1[e] = wK;
             d.K1[e] = K;
-        } else if (!sp.last) {
+        } else if (!ss.last) {
             d.A1[e] = cadd(d.A1[e], wK);
             d.K1[e] = K;
         } else {
             const double2 A = cadd(d.A1[e], wK);
-            d.S1[e] = cadd(d.S1[e], cdiv6(cscale(A, sp.dt)));
+            d.S1[e] = cadd(d.S1[e], cdiv6(cscale(A, ss.dt)));
         }
     }
 }
 
-// Records G1 before a step and applies the density guard
-// (RK_solver.h:100-110): |2 Im G1[0,0]| > 3 or NaN stops the run after the
-// record; fail = step + 1.
-__global__ void k_record(DevState d, int slot, int64_t step) {
+// Records G1 before a production step and applies the density guard
+// (RK_solver.h:100-110): |2 Im G1[0,0]| > 3 or NaN stops the trajectory after
+// the record; fail = step + 1.  Step index and ring slot come from the stage-0
+// block of the step.
+__global__ void k_record(BatchDev b, StageRef r) {
+    const int ti = blockIdx.y;
+    const DevState& d = b.ds[ti];
     if (failed(d.fail)) return;
+    const StageShared ss = stage_of(r);
     const int n2 = d.N * d.N;
-    double2* dst = d.traj ? d.traj + (int64_t)slot * n2 : nullptr;
-    for (int e = threadIdx.x; e < n2; e += blockDim.x)
-        if (dst) dst[e] = d.S1[e];
+    double2* dst = d.traj ? d.traj + (int64_t)ss.slot * n2 : nullptr;
+    if (dst)
+        for (int e = threadIdx.x; e < n2; e += blockDim.x) dst[e] = d.S1[e];
     __syncthreads();
     if (threadIdx.x == 0) {
         const double dens = 2 * d.S1[0].y;
-        if (fabs(dens) > 3 || isnan(dens)) *d.fail = (int)(step + 1);
+        if (fabs(dens) > 3 || isnan(dens)) *d.fail = ss.step + 1;
     }
+}
+
+__global__ void k_advance_counter(int* counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -344,22 +421,26 @@ __device__ __forceinline__ void mfma_step(d4 (&ar)[PW], d4 (&ai)[PW], const doub
         ai[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(NEG ? -a_i[u] : a_i[u], b_r[u], ai[u], 0, 0, 0);
 }
 
+// Lane group kq (= lane / 16) of the 16x16x4 MFMA takes k = kq*NP/4 + j at
+// step j: with the NP+1 pitch this puts the two lane groups that share an LDS
+// cycle 16 rows / columns apart, so A and B fragment reads are conflict-free
+// at NP = 64 (the sum over k does not depend on its order).
 template <int NP, int NW, bool NEG>
 __device__ __forceinline__ void tile_product(const double* ar_, const double* ai_, const double* br_,
                                              const double* bi_, d4 (&ar)[TileCfg<NP, NW>::PW],
                                              d4 (&ai)[TileCfg<NP, NW>::PW], int wave, int lane) {
-    constexpr int PW = TileCfg<NP, NW>::PW;
-    const int r = lane & 15, kq = lane >> 4;
+    constexpr int PW = TileCfg<NP, NW>::PW, KS = NP / 4;
+    const int r = lane & 15, kb = (lane >> 4) * KS;
     double a_r[PW], a_i[PW], b_r[PW], b_i[PW];
     double n_r[PW], n_i[PW], m_r[PW], m_i[PW];
-    load_frags<NP, NW>(ar_, ai_, br_, bi_, kq, wave, r, a_r, a_i, b_r, b_i);
+    load_frags<NP, NW>(ar_, ai_, br_, bi_, kb, wave, r, a_r, a_i, b_r, b_i);
 #pragma unroll
-    for (int k0 = 0; k0 < NP; k0 += 8) {
-        // operands of step k0+4 load while step k0 multiplies, and so on
-        if (k0 + 4 < NP) load_frags<NP, NW>(ar_, ai_, br_, bi_, k0 + 4 + kq, wave, r, n_r, n_i, m_r, m_i);
+    for (int j = 0; j < KS; j += 2) {
+        // operands of step j+1 load while step j multiplies, and so on
+        if (j + 1 < KS) load_frags<NP, NW>(ar_, ai_, br_, bi_, kb + j + 1, wave, r, n_r, n_i, m_r, m_i);
         mfma_step<PW, NEG>(ar, ai, a_r, a_i, b_r, b_i);
-        if (k0 + 4 < NP) {
-            if (k0 + 8 < NP) load_frags<NP, NW>(ar_, ai_, br_, bi_, k0 + 8 + kq, wave, r, a_r, a_i, b_r, b_i);
+        if (j + 1 < KS) {
+            if (j + 2 < KS) load_frags<NP, NW>(ar_, ai_, br_, bi_, kb + j + 2, wave, r, a_r, a_i, b_r, b_i);
             mfma_step<PW, NEG>(ar, ai, n_r, n_i, m_r, m_i);
         }
     }
@@ -440,10 +521,11 @@ struct Engine<NP, NW, G12_PREC_FP32_3M> {
                                    const float* bi_, const float* bs_, f4 (&p1)[PW], f4 (&p2)[PW], f4 (&p3)[PW],
                                    int wave, int lane) {
         constexpr bool FULL = (TT % NW) == 0;
-        const int r = lane & 15, kq = lane >> 4;
+        constexpr int KS = NP / 4;
+        const int r = lane & 15, kb = (lane >> 4) * KS;
 #pragma unroll 4
-        for (int k0 = 0; k0 < NP; k0 += 4) {
-            const int kk = k0 + kq;
+        for (int j = 0; j < KS; ++j) {
+            const int kk = kb + j;  // spread k: conflict-free LDS reads (see tile_product)
             float a1[PW], a2[PW], a3[PW], b1[PW], b2[PW], b3[PW];
 #pragma unroll
             for (int u = 0; u < PW; ++u) {
@@ -602,33 +684,20 @@ struct Engine<NP, NW, G12_PREC_BF16X3> {
     }
 };
 
-struct TileArgs {
-    const double2* S[2];   // state (tile input Y = S + c*Kp)
-    const double2* Kp[2];  // previous stage derivative (may be null)
-    double c;
-    const double2* h;      // N x N
-    int hmode;             // 1: +h.X, 2: -X.h, 3: both
-    int tiling;            // 0: strided (p2,q2) tiles, 1: contiguous (p1,q1) tiles
-    // mode 0: out = (accumulate ? out : 0) + O
-    double2* out[2];
-    int accumulate;
-    double2* D[2];         // optional: D[X][Y][Z] from the diagonal strided tiles
-    // mode 1 (fused right-hand side, contiguous tiling)
-    const double2* R[2];   // strided-pass result
-    double2* Kout[2];
-    double2* A[2];
-    double2* Sout[2];
-    const double2* G1;
-    const double* W[2];    // W_xx, W_xy
-    const double2* Q[2];
-    const double2* Qp[2];
-    const double2* F[2];
-    const double2* Ft[2];
-    double wk, dt;
-    int first, last;
-    int N;
+// Tile-pass arguments.  The tile list of a launch is the concatenation, over
+// the trajectories of one bucket, of nspin * N^2 tiles per trajectory.
+struct BTileArgs {
+    BatchDev b;
+    Bucket k;
+    StageRef r;
+    int tiling;       // 0: strided (p2,q2) tiles, 1: contiguous (p1,q1) tiles
+    int fused;        // 1: fused right-hand side epilogue (contiguous tiling)
+    int hmode;        // 1: +h.X, 2: -X.h, 3: both
+    int accumulate;   // unfused: R += O instead of R = O
     int nspin;
-    const int* fail;
+    int use_kp;       // stage input Y = S + c K_prev (0: Y = S)
+    int extract_d;    // unfused strided pass: D from the diagonal tiles
+    const double2* h_override;  // routine calls: h to use instead of the trajectory's h
 };
 
 // Persistent-tile schedule.  Workgroups are dealt round-robin to the 8 XCDs;
@@ -636,9 +705,10 @@ struct TileArgs {
 // through it together, so tiles that are neighbours in memory (consecutive
 // q2 of the strided pass share cache lines) are in flight on one L2 at once.
 struct TileSched {
-    int total, per_xcd, wg_per_xcd, xcd, local;
+    int64_t total, per_xcd;
+    int wg_per_xcd, xcd, local;
     bool xcd_mode;
-    __device__ TileSched(int total_, int grid, int b) : total(total_) {
+    __device__ TileSched(int64_t total_, int grid, int b) : total(total_) {
         xcd_mode = (total % 8 == 0) && (grid % 8 == 0);
         xcd = b & 7;
         local = b >> 3;
@@ -646,15 +716,124 @@ struct TileSched {
         wg_per_xcd = grid >> 3;
     }
     // tile of iteration `it` for this workgroup, or -1 when done
-    __device__ int tile(int it, int b, int grid) const {
+    __device__ int64_t tile(int it, int b, int grid) const {
         if (xcd_mode) {
-            const int i = it * wg_per_xcd + local;
+            const int64_t i = (int64_t)it * wg_per_xcd + local;
             return i < per_xcd ? xcd * per_xcd + i : -1;
         }
-        const int i = b + it * grid;
+        const int64_t i = b + (int64_t)it * grid;
         return i < total ? i : -1;
     }
 };
+
+// Values that are the same in every lane, moved to scalar registers: the
+// trajectory descriptors are read with vector loads (the kernel writes global
+// memory, so scalar loads of them are not allowed), and left in vector
+// registers they would crowd out the tile data.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uni64(int64_t x) {
+    const uint64_t u = (uint64_t)x;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <class T>
+__device__ __forceinline__ T* uni_ptr(T* p) {
+    return reinterpret_cast<T*>(uni64((int64_t)reinterpret_cast<uintptr_t>(p)));
+}
+__device__ __forceinline__ double uni_d(double x) {
+    return __builtin_bit_cast(double, uni64(__builtin_bit_cast(int64_t, x)));
+}
+__device__ inline StageShared uni_stage(const StageRef& r) {
+    const StageShared v = stage_of(r);
+    StageShared u;
+    u.c = uni_d(v.c);
+    u.wk = uni_d(v.wk);
+    u.dt = uni_d(v.dt);
+    u.first = uni(v.first);
+    u.last = uni(v.last);
+    u.lam = uni_d(v.lam);
+    u.phi = uni_d(v.phi);
+    u.gate = uni(v.gate);
+    u.as_phase = uni(v.as_phase);
+    u.step = uni(v.step);
+    u.slot = uni(v.slot);
+    return u;
+}
+
+// Position of global tiles in the bucket: the listed trajectory that owns a
+// tile, kept with its tile range so that the tiles of one trajectory are
+// located without any memory access.
+struct Cursor {
+    int64_t begin = 0, end = 0;  // global tiles [begin, end) of the trajectory
+    int traj = -1, N = 0;
+    bool on = false;             // the trajectory takes part in this stage
+};
+
+__device__ inline void seek(const BTileArgs& a, const StageShared& ss, Cursor& c, int64_t gt) {
+    if (gt >= c.begin && gt < c.end) return;
+    int lo = 0, hi = a.k.n - 1;
+    while (lo < hi) {  // last listed trajectory with prefix <= gt
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.k.prefix[mid] <= gt) lo = mid; else hi = mid - 1;
+    }
+    c.begin = uni64(a.k.prefix[lo]);
+    c.end = uni64(a.k.prefix[lo + 1]);
+    c.traj = uni(a.k.list[lo]);
+    c.N = uni(a.b.tp[c.traj].N);
+    c.on = uni(active(a.b.ds[c.traj], a.b.tp[c.traj], ss) ? 1 : 0) != 0;
+}
+
+// Tile of a trajectory: spin (0 xxxx, 1 xyxy) and index within the spin.
+struct TileRef {
+    int traj, N, spin, tile;
+};
+
+__device__ inline TileRef ref_of(const Cursor& c, int64_t gt) {
+    TileRef t;
+    t.traj = c.traj;
+    t.N = c.N;
+    const int local = (int)(gt - c.begin);  // < 2 N^2
+    t.spin = local >= c.N * c.N ? 1 : 0;
+    t.tile = local - t.spin * c.N * c.N;
+    return t;
+}
+
+// Buffers of the (trajectory, spin) whose tile is being processed; reloaded
+// only when that pair changes.
+struct View {
+    int traj = -1, spin = -1;
+    const double2* S;   // stage state
+    const double2* Kp;  // previous stage derivative (null: Y = S)
+    double2* R;         // unfused output / strided-pass result
+    double2* D;         // D extraction (unfused strided pass)
+    const double2* G1;  // fused: stage G1
+    const double* W;    // fused: W_xx (spin 0) or W_xy (spin 1)
+    const double2 *Q, *Qp, *F, *Ft;
+    double2 *A, *K;
+};
+
+template <bool FUSED>
+__device__ inline void load_view(View& v, const BTileArgs& a, const StageShared& ss, int traj, int spin) {
+    const DevState& d = a.b.ds[traj];
+    v.traj = traj;
+    v.spin = spin;
+    v.S = uni_ptr(d.S2[spin]);
+    v.Kp = (a.use_kp && !ss.first) ? uni_ptr(d.K2[spin]) : nullptr;
+    v.R = uni_ptr(d.R2[spin]);
+    if constexpr (FUSED) {
+        v.G1 = uni_ptr(d.Y1);
+        v.W = uni_ptr(spin == 0 ? d.Wxx : d.Wxy);
+        v.Q = uni_ptr(d.Q[spin]);
+        v.Qp = uni_ptr(d.Qp[spin]);
+        v.F = uni_ptr(d.F[spin]);
+        v.Ft = uni_ptr(d.Ft[spin]);
+        v.A = uni_ptr(d.A2[spin]);
+        v.K = uni_ptr(d.K2[spin]);
+    } else {
+        v.D = uni_ptr(d.D[spin]);
+    }
+}
 
 __device__ __forceinline__ void tile_geometry(int tiling, int N, int tile, int64_t& base, int64_t& sm, int64_t& sn,
                                               int& t0, int& t1) {
@@ -672,16 +851,17 @@ __device__ __forceinline__ void tile_geometry(int tiling, int N, int tile, int64
     }
 }
 
-template <int NP, int NW>
-__device__ __forceinline__ void tile_prefetch(const TileArgs& a, int gt, double2 (&pre)[TileCfg<NP, NW>::PF]) {
+// Stage input Y = S + c Kp of one tile into registers (TILING: 0 strided,
+// 1 contiguous, -1 given at run time).
+template <int NP, int NW, int TILING>
+__device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const StageShared& ss, const TileRef& t,
+                                              const double2* S, const double2* Kp,
+                                              double2 (&pre)[TileCfg<NP, NW>::PF]) {
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
-    const int N = a.N;
-    const int s = gt / (N * N);
+    const int N = t.N;
     int64_t base, sm, sn;
     int t0, t1;
-    tile_geometry(a.tiling, N, gt % (N * N), base, sm, sn, t0, t1);
-    const double2* S = a.S[s];
-    const double2* Kp = a.Kp[s];
+    tile_geometry(TILING < 0 ? a.tiling : TILING, N, t.tile, base, sm, sn, t0, t1);
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
         const int e = threadIdx.x + NTHR * i;
@@ -690,63 +870,98 @@ __device__ __forceinline__ void tile_prefetch(const TileArgs& a, int gt, double2
             const int m = e / N, n = e % N;
             const int64_t off = base + m * sm + n * sn;
             y = S[off];
-            if (Kp) y = cadd(y, cscale(Kp[off], a.c));
+            if (Kp) y = cadd(y, cscale(Kp[off], ss.c));
         }
         pre[i] = y;
     }
 }
 
-template <int NP, int NW, int MODE, int PREC>
-__global__ __launch_bounds__(64 * NW) void k_tile(TileArgs a) {
-    if (failed(a.fail)) return;
+template <int NP, int NW, int PREC, bool FUSED>
+__global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     typedef Engine<NP, NW, PREC> E;
     __shared__ typename E::Lds L;
     constexpr int NT = TileCfg<NP, NW>::NT, TT = TileCfg<NP, NW>::TT, PW = TileCfg<NP, NW>::PW;
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
-    const int N = a.N;
-    const int64_t N2 = (int64_t)N * N;
-    const int total = a.nspin * N * N;
-    const TileSched sched(total, gridDim.x, blockIdx.x);
-    // h (shared by every tile of the launch) and the zero padding of X
+    constexpr int TILING = FUSED ? 1 : -1;
+    const StageShared ss = uni_stage(a.r);
+    const TileSched sched(a.k.tiles, gridDim.x, blockIdx.x);
     E::zero(L, threadIdx.x, NTHR);
-    __syncthreads();
-    for (int e = threadIdx.x; e < NP * NP; e += NTHR) {
-        const int m = e / NP, n = e % NP;
-        double2 hv = make_double2(0.0, 0.0);
-        if (m < N && n < N) hv = a.h[m * N + n];
-        E::put_h(L, m, n, hv);
-        E::put_x(L, m, n, make_double2(0.0, 0.0));
-    }
-    int gt = sched.tile(0, blockIdx.x, gridDim.x);
+    // next tile of a trajectory that takes part in this stage, or -1
+    Cursor c;
+    auto next_tile = [&](int& it) -> int64_t {
+        for (;;) {
+            const int64_t g = sched.tile(it, blockIdx.x, gridDim.x);
+            if (g < 0) return -1;
+            seek(a, ss, c, g);
+            if (c.on) return g;
+            ++it;
+        }
+    };
+    View v;
+    int it = 0;
+    int64_t gt = next_tile(it);
+    if (gt < 0) return;
+    TileRef tr = ref_of(c, gt);
     double2 pre[PF];
-    if (gt >= 0) tile_prefetch<NP, NW>(a, gt, pre);
+    {
+        const DevState& d = a.b.ds[tr.traj];
+        tile_prefetch<NP, NW, TILING>(a, ss, tr, uni_ptr(d.S2[tr.spin]),
+                                      (a.use_kp && !ss.first) ? uni_ptr(d.K2[tr.spin]) : nullptr, pre);
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int it = 0; gt >= 0; ++it) {
-        __syncthreads();  // previous tile's readers of X are done
-        const int s = gt / (N * N);
+    while (gt >= 0) {
+        const int N = tr.N;
+        const int64_t N2 = (int64_t)N * N;
+        lds_sync();  // previous tile's readers of X (and h) are done
+        if (tr.traj != v.traj || tr.spin != v.spin) {
+            const bool new_h = tr.traj != v.traj;
+            load_view<FUSED>(v, a, ss, tr.traj, tr.spin);
+            if (new_h) {  // h (and the zero padding of X) for this trajectory
+                const double2* h = a.h_override ? a.h_override : a.b.ds[tr.traj].h;
+                for (int e = threadIdx.x; e < NP * NP; e += NTHR) {
+                    const int m = e / NP, n = e % NP;
+                    double2 hv = make_double2(0.0, 0.0);
+                    if (m < N && n < N) hv = h[m * N + n];
+                    E::put_h(L, m, n, hv);
+                    E::put_x(L, m, n, make_double2(0.0, 0.0));
+                }
+                lds_sync();
+            }
+        }
         int64_t base, sm, sn;
         int t0, t1;
-        tile_geometry(a.tiling, N, gt % (N * N), base, sm, sn, t0, t1);
+        tile_geometry(TILING < 0 ? a.tiling : TILING, N, tr.tile, base, sm, sn, t0, t1);
         // D[X=p1][Y=q1][Z] (Z = p2 = q2) from the FP64 stage input of a diagonal strided tile
-        const bool dtile = MODE == 0 && a.D[s] && a.tiling == 0 && t0 == t1;
+        const bool dtile = !FUSED && a.extract_d && a.tiling == 0 && t0 == t1;
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
             if (e < N * N) {
                 const int m = e / N, n = e % N;
                 E::put_x(L, m, n, pre[i]);
-                if (dtile) a.D[s][((int64_t)m * N + n) * N + t0] = pre[i];
+                if (dtile) v.D[((int64_t)m * N + n) * N + t0] = pre[i];
             }
         }
-        __syncthreads();
-        const int next = sched.tile(it + 1, blockIdx.x, gridDim.x);
-        if (next >= 0) tile_prefetch<NP, NW>(a, next, pre);  // overlaps the MFMA loop
+        lds_sync();
+        ++it;
+        const int64_t next = next_tile(it);
+        TileRef nt = tr;
+        if (next >= 0) {  // the next tile's loads overlap the MFMA loop
+            nt = ref_of(c, next);
+            if (nt.traj == v.traj && nt.spin == v.spin) {
+                tile_prefetch<NP, NW, TILING>(a, ss, nt, v.S, v.Kp, pre);
+            } else {
+                const DevState& d = a.b.ds[nt.traj];
+                tile_prefetch<NP, NW, TILING>(a, ss, nt, uni_ptr(d.S2[nt.spin]),
+                                              (a.use_kp && !ss.first) ? uni_ptr(d.K2[nt.spin]) : nullptr, pre);
+            }
+        }
 
         d4 ar[PW], ai[PW];
         E::compute(L, a.hmode, ar, ai, wave, lane);
 
-        if (MODE == 0) {
-            double2* out = a.out[s];
+        if constexpr (!FUSED) {
+            double2* out = v.R;
 #pragma unroll
             for (int u = 0; u < PW; ++u) {
                 const int t = wave + NW * u;
@@ -757,9 +972,9 @@ __global__ __launch_bounds__(64 * NW) void k_tile(TileArgs a) {
                     const int m = (t / NT) * 16 + E::row_of(lane, j);
                     if (m < N && n < N) {
                         const int64_t off = base + m * sm + n * sn;
-                        double2 v = make_double2(ar[u][j], ai[u][j]);
-                        if (a.accumulate) v = cadd(out[off], v);
-                        out[off] = v;
+                        double2 val = make_double2(ar[u][j], ai[u][j]);
+                        if (a.accumulate) val = cadd(out[off], val);
+                        out[off] = val;
                     }
                 }
             }
@@ -772,15 +987,21 @@ __global__ __launch_bounds__(64 * NW) void k_tile(TileArgs a) {
             //   Pi^+ = conj( g(q1,p1) (Ft[q1][p2][q2] - Ft[p1][p2][q2]) )
             //   K    = -i (comm + Psi + Pi - Pi^+)
             const int p1 = t0, q1 = t1;
-            const double2 g11 = a.G1[(int64_t)p1 * N + q1];
-            const double2 gq1p1 = a.G1[(int64_t)q1 * N + p1];
-            const double* W = a.W[s];
-            const double2* Qs = a.Q[s];
-            const double2* Qps = a.Qp[s] + ((int64_t)p1 * N + q1) * N;
-            const double2* Frow = a.F[s] + ((int64_t)p1 * N + q1) * N;
-            const double2* Fts = a.Ft[s];
-            const double2* R = a.R[s];
-            const double2* S = a.S[s];
+            const double2* G1 = v.G1;
+            double2 g11 = G1[(int64_t)p1 * N + q1];
+            double2 gq1p1 = G1[(int64_t)q1 * N + p1];
+            g11 = make_double2(uni_d(g11.x), uni_d(g11.y));  // tile constants: scalar registers
+            gq1p1 = make_double2(uni_d(gq1p1.x), uni_d(gq1p1.y));
+            const double* W = v.W;
+            const double2* Qs = v.Q;
+            const double2* Qps = v.Qp + ((int64_t)p1 * N + q1) * N;
+            const double2* Frow = v.F + ((int64_t)p1 * N + q1) * N;
+            const double2* Fts = v.Ft;
+            const double2* R = v.R;
+            const double2* S = v.S;
+            double2* A2 = v.A;
+            double2* K2 = v.K;
+            double2* S2 = const_cast<double2*>(v.S);
 #pragma unroll
             for (int u = 0; u < PW; ++u) {
                 const int t = wave + NW * u;
@@ -793,7 +1014,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(TileArgs a) {
                     const int64_t off = base + m * sm + n * sn;
                     const int64_t mn = (int64_t)m * N + n;
                     double2 comm = cadd(R[off], make_double2(ar[u][j], ai[u][j]));
-                    const double2 g22 = a.G1[mn];
+                    const double2 g22 = G1[mn];
                     double2 psi = cscale(cmul(g11, g22), W[(int64_t)p1 * N + m] - W[(int64_t)q1 * N + n]);
                     psi = cadd(psi, cmul_i(cmul(g11, csub(Qs[(int64_t)p1 * N2 + mn], Qs[(int64_t)q1 * N2 + mn]))));
                     psi = cadd(psi, cmul_i(cmul(g22, csub(Qps[m], Qps[n]))));
@@ -801,21 +1022,22 @@ __global__ __launch_bounds__(64 * NW) void k_tile(TileArgs a) {
                     const double2 pid =
                         cconj(cmul(gq1p1, csub(Fts[(int64_t)q1 * N2 + mn], Fts[(int64_t)p1 * N2 + mn])));
                     const double2 K = cmul_mi(csub(cadd(cadd(comm, psi), pi), pid));
-                    const double2 wK = cscale(K, a.wk);
-                    if (a.first) {
-                        a.A[s][off] = wK;
-                        a.Kout[s][off] = K;
-                    } else if (!a.last) {
-                        a.A[s][off] = cadd(a.A[s][off], wK);
-                        a.Kout[s][off] = K;
+                    const double2 wK = cscale(K, ss.wk);
+                    if (ss.first) {
+                        A2[off] = wK;
+                        K2[off] = K;
+                    } else if (!ss.last) {
+                        A2[off] = cadd(A2[off], wK);
+                        K2[off] = K;
                     } else {
-                        const double2 A = cadd(a.A[s][off], wK);
-                        a.Sout[s][off] = cadd(S[off], cdiv6(cscale(A, a.dt)));
+                        const double2 A = cadd(A2[off], wK);
+                        S2[off] = cadd(S[off], cdiv6(cscale(A, ss.dt)));
                     }
                 }
             }
         }
         gt = next;
+        tr = nt;
     }
 }
 
@@ -830,63 +1052,58 @@ int device_cus() {
     return cus;
 }
 
-// Waves per tile workgroup: G12KBA_TILE_WAVES (4, 8 or 16) overrides the
-// built-in choice for tuning sweeps.
-int tile_waves(int NP) {
-    static int env = -1;
-    if (env < 0) {
-        const char* v = std::getenv("G12KBA_TILE_WAVES");
-        env = v ? std::atoi(v) : 0;
-    }
-    if (env == 4 || env == 8 || env == 16) return env;
-    return NP >= 48 ? 16 : 4;  // measured best on MI355X (FP64, N = 32/48/64)
-}
-
-template <int NP, int NW, int MODE, int PREC>
-void launch_tile_cfg(const TileArgs& a, hipStream_t s) {
-    const int total = a.nspin * a.N * a.N;
+template <int NP, int NW, int PREC>
+void launch_tile_cfg(const BTileArgs& a, hipStream_t s) {
     constexpr int lds = (int)sizeof(typename Engine<NP, NW, PREC>::Lds);
     int per_cu = (160 * 1024) / lds;
     const int by_waves = 32 / NW;  // at most 32 waves per CU
     if (per_cu > by_waves) per_cu = by_waves;
+    if (a.k.wg_per_cu > 0 && per_cu > a.k.wg_per_cu) per_cu = a.k.wg_per_cu;
     if (per_cu < 1) per_cu = 1;
-    int grid = device_cus() * per_cu;
-    if (grid > total) grid = total;
+    int64_t grid = (int64_t)device_cus() * per_cu;
+    if (grid > a.k.tiles) grid = a.k.tiles;
     if (grid >= 8) grid &= ~7;
-    hipLaunchKernelGGL((k_tile<NP, NW, MODE, PREC>), dim3(grid), dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
+    if (grid < 1) return;
+    if (a.fused)
+        hipLaunchKernelGGL((k_tile<NP, NW, PREC, true>), dim3((unsigned)grid), dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_tile<NP, NW, PREC, false>), dim3((unsigned)grid), dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
 }
 
-template <int NP, int MODE, int PREC>
-void launch_tile_np(const TileArgs& a, hipStream_t s) {
-    switch (tile_waves(NP)) {
-        case 16: launch_tile_cfg<NP, 16, MODE, PREC>(a, s); break;
-        case 8: launch_tile_cfg<NP, 8, MODE, PREC>(a, s); break;
-        default: launch_tile_cfg<NP, 4, MODE, PREC>(a, s); break;
+// Wave counts compiled for a padded tile size NP: W0, 2 W0 and 4 W0 with
+// W0 = 1, 2, 4 for NP = 16, 32, >= 48 (a tile has (NP/16)^2 output blocks of
+// 16x16: one at NP = 16, sixteen at NP = 64).
+constexpr int base_waves(int NP) { return NP == 16 ? 1 : NP == 32 ? 2 : 4; }
+
+template <int NP, int PREC>
+void launch_tile_np(const BTileArgs& a, hipStream_t s) {
+    constexpr int W0 = base_waves(NP);
+    if (a.k.waves >= 4 * W0)
+        launch_tile_cfg<NP, 4 * W0, PREC>(a, s);
+    else if (a.k.waves >= 2 * W0)
+        launch_tile_cfg<NP, 2 * W0, PREC>(a, s);
+    else
+        launch_tile_cfg<NP, W0, PREC>(a, s);
+}
+
+template <int PREC>
+void launch_tile_prec(const BTileArgs& a, hipStream_t s) {
+    switch (a.k.NP) {
+        case 16: launch_tile_np<16, PREC>(a, s); break;
+        case 32: launch_tile_np<32, PREC>(a, s); break;
+        case 48: launch_tile_np<48, PREC>(a, s); break;
+        default: launch_tile_np<64, PREC>(a, s); break;
     }
 }
 
-template <int MODE, int PREC>
-void launch_tile_prec(const TileArgs& a, hipStream_t s) {
-    const int N = a.N;
-    if (N <= 16)
-        launch_tile_np<16, MODE, PREC>(a, s);
-    else if (N <= 32)
-        launch_tile_np<32, MODE, PREC>(a, s);
-    else if (N <= 48)
-        launch_tile_np<48, MODE, PREC>(a, s);
-    else
-        launch_tile_np<64, MODE, PREC>(a, s);
-}
-
-template <int MODE>
-void launch_tile(TileArgs a, int nspin, int precision, hipStream_t s) {
-    a.nspin = nspin;
+void launch_tile(const BTileArgs& a, int precision, hipStream_t s) {
+    if (a.k.tiles <= 0) return;
     if (precision == G12_PREC_FP32_3M)
-        launch_tile_prec<MODE, G12_PREC_FP32_3M>(a, s);
+        launch_tile_prec<G12_PREC_FP32_3M>(a, s);
     else if (precision == G12_PREC_BF16X3)
-        launch_tile_prec<MODE, G12_PREC_BF16X3>(a, s);
+        launch_tile_prec<G12_PREC_BF16X3>(a, s);
     else
-        launch_tile_prec<MODE, G12_PREC_FP64>(a, s);
+        launch_tile_prec<G12_PREC_FP64>(a, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1055,71 +1272,55 @@ inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b
 // ---------------------------------------------------------------------------
 // launchers
 
-void launch_stage_g1(const DevState& d, const StageParams& sp, hipStream_t s) {
-    hipLaunchKernelGGL(k_stage_g1, dim3(1), dim3(1024), 0, s, d, sp);
+void launch_b_stage_g1(const BatchDev& b, const StageRef& r, hipStream_t s) {
+    hipLaunchKernelGGL(k_stage_g1, dim3(1, b.ntraj), dim3(1024), 0, s, b, r);
 }
 
-void launch_record(const DevState& d, int slot, int64_t step, hipStream_t s) {
-    hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, s, d, slot, step);
+void launch_b_tables(const BatchDev& b, const StageRef& r, hipStream_t s) {
+    const unsigned n2 = (unsigned)(b.maxN * b.maxN);
+    hipLaunchKernelGGL(k_q_tables, dim3(n2, b.ntraj), dim3(64), 0, s, b, r);
+    hipLaunchKernelGGL(k_f_tables, dim3(n2, b.ntraj), dim3(64), 0, s, b, r);
 }
 
-static TileArgs base_args(const DevState& d, const StageParams& sp) {
-    TileArgs a{};
-    for (int s = 0; s < 2; ++s) {
-        a.S[s] = d.S2[s];
-        a.Kp[s] = sp.first ? nullptr : d.K2[s];
-    }
-    a.c = sp.c;
-    a.h = d.h;
+void launch_b_g1_rhs(const BatchDev& b, const StageRef& r, hipStream_t s) {
+    hipLaunchKernelGGL(k_g1_rhs, dim3(b.maxN, b.ntraj), dim3(64), 0, s, b, r);
+}
+
+void launch_b_record(const BatchDev& b, const StageRef& r, hipStream_t s) {
+    hipLaunchKernelGGL(k_record, dim3(1, b.ntraj), dim3(256), 0, s, b, r);
+}
+
+void launch_b_advance_counter(int* counter, hipStream_t s) {
+    hipLaunchKernelGGL(k_advance_counter, dim3(1), dim3(64), 0, s, counter);
+}
+
+int device_cu_count() { return device_cus(); }
+
+int default_waves(int NP) { return NP >= 48 ? 16 : 4; }
+
+int tile_wave_options(int NP, int* opts) {
+    const int w0 = base_waves(NP);
+    opts[0] = w0;
+    opts[1] = 2 * w0;
+    opts[2] = 4 * w0;
+    return 3;
+}
+
+void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fused, int precision, hipStream_t s) {
+    BTileArgs a{};
+    a.b = b;
+    a.k = k;
+    if (fused) a.k.waves = k.waves_fused;
+    a.r = r;
+    a.tiling = fused ? 1 : 0;
+    a.fused = fused;
     a.hmode = 3;
-    a.N = d.N;
-    a.fail = d.fail;
-    return a;
-}
-
-void launch_pass_strided(const DevState& d, const StageParams& sp, int precision, hipStream_t s) {
-    TileArgs a = base_args(d, sp);
-    a.tiling = 0;
-    for (int t = 0; t < 2; ++t) {
-        a.out[t] = d.R2[t];
-        a.D[t] = d.D[t];
-    }
     a.accumulate = 0;
-    launch_tile<0>(a, 2, precision, s);
-}
-
-void launch_tables(const DevState& d, const StageParams& sp, hipStream_t s) {
-    (void)sp;
-    const int N = d.N;
-    hipLaunchKernelGGL(k_q_tables, dim3(N * N), dim3(64), 0, s, d);
-    hipLaunchKernelGGL(k_f_tables, dim3(N * N), dim3(64), 0, s, d);
-}
-
-void launch_g1_rhs(const DevState& d, const StageParams& sp, hipStream_t s) {
-    hipLaunchKernelGGL(k_g1_rhs, dim3(d.N), dim3(64), 0, s, d, sp);
-}
-
-void launch_pass_fused(const DevState& d, const StageParams& sp, int precision, hipStream_t s) {
-    TileArgs a = base_args(d, sp);
-    a.tiling = 1;
-    for (int t = 0; t < 2; ++t) {
-        a.R[t] = d.R2[t];
-        a.Kout[t] = d.K2[t];
-        a.A[t] = d.A2[t];
-        a.Sout[t] = d.S2[t];
-        a.Q[t] = d.Q[t];
-        a.Qp[t] = d.Qp[t];
-        a.F[t] = d.F[t];
-        a.Ft[t] = d.Ft[t];
-    }
-    a.W[0] = d.Wxx;
-    a.W[1] = d.Wxy;
-    a.G1 = d.Y1;
-    a.wk = sp.wk;
-    a.dt = sp.dt;
-    a.first = sp.first;
-    a.last = sp.last;
-    launch_tile<1>(a, 2, precision, s);
+    a.nspin = 2;
+    a.use_kp = 1;
+    a.extract_d = fused ? 0 : 1;
+    a.h_override = nullptr;
+    launch_tile(a, precision, s);
 }
 
 void launch_ref_to_nat(const double2* in, double2* out, int Ns, int Nb, hipStream_t s) {
@@ -1132,25 +1333,63 @@ void launch_nat_to_ref(const double2* in, double2* out, int Ns, int Nb, hipStrea
     hipLaunchKernelGGL(k_nat_to_ref, dim3(N * N), dim3(256), 0, s, in, out, Ns, Nb);
 }
 
-void launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, double2* tmp, int N,
-                  int precision, hipStream_t s) {
-    (void)tmp;
-    TileArgs a{};
-    a.S[0] = T;
-    a.Kp[0] = nullptr;
-    a.N = N;
-    a.out[0] = out;
-    // four accumulating launches: (strided, contiguous) x (left hx, right hy)
+void launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
+                  hipStream_t s) {
+    // a batch of one trajectory whose spin-0 state is T and strided-pass
+    // output is `out`; four accumulating launches (strided, contiguous) x
+    // (left hx, right hy)
+    DevState d{};
+    d.N = N;
+    d.S2[0] = const_cast<double2*>(T);
+    d.R2[0] = out;
+    TrajParams tp{};
+    tp.N = N;
+    tp.corr = 1;
+    DevState* dd = nullptr;
+    TrajParams* dt = nullptr;
+    int* dl = nullptr;
+    int64_t* dp = nullptr;
+    if (hipMalloc(&dd, sizeof d) != hipSuccess || hipMalloc(&dt, sizeof tp) != hipSuccess ||
+        hipMalloc(&dl, sizeof(int)) != hipSuccess || hipMalloc(&dp, 2 * sizeof(int64_t)) != hipSuccess)
+        return;
+    const int zero = 0;
+    const int64_t pre[2] = {0, (int64_t)N * N};
+    hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(dt, &tp, sizeof tp, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(dl, &zero, sizeof zero, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(dp, pre, sizeof pre, hipMemcpyHostToDevice, s);
+    BTileArgs a{};
+    a.b.ds = dd;
+    a.b.tp = dt;
+    a.b.ntraj = 1;
+    a.b.maxN = N;
+    a.k.NP = N <= 16 ? 16 : N <= 32 ? 32 : N <= 48 ? 48 : 64;
+    a.k.list = dl;
+    a.k.prefix = dp;
+    a.k.n = 1;
+    a.k.tiles = (int64_t)N * N;
+    a.k.waves = default_waves(a.k.NP);
+    a.k.waves_fused = a.k.waves;
+    a.r.val.first = 1;
+    a.nspin = 1;
+    a.use_kp = 0;
+    a.extract_d = 0;
+    a.fused = 0;
     int first = 1;
     for (int tiling = 0; tiling < 2; ++tiling)
         for (int side = 1; side <= 2; ++side) {
             a.tiling = tiling;
             a.hmode = side;
-            a.h = side == 1 ? hx : hy;
+            a.h_override = side == 1 ? hx : hy;
             a.accumulate = first ? 0 : 1;
             first = 0;
-            launch_tile<0>(a, 1, precision, s);
+            launch_tile(a, precision, s);
         }
+    hipStreamSynchronize(s);
+    hipFree(dd);
+    hipFree(dt);
+    hipFree(dl);
+    hipFree(dp);
 }
 
 void launch_hf(double2* h, const double2* G1, int Ns, int Nb, const StageParams& sp, hipStream_t s) {

@@ -84,6 +84,7 @@ typedef struct g12_params {
 } g12_params;
 
 typedef struct g12_ctx g12_ctx;
+typedef struct g12_batch g12_batch;
 
 /* ---- parameters and input file ----------------------------------------- */
 
@@ -138,11 +139,53 @@ int g12_get_time(g12_ctx* ctx, double* t, int64_t* steps);
 int g12_timing_enable(g12_ctx* ctx, int on);
 int g12_timing_get(g12_ctx* ctx, int which, int64_t* launches, double* total_ms);
 
+/* Execution options of a context: use_graphs = 1 replays each RK4 step as one
+ * captured hipGraph (default; timing mode always launches directly),
+ * autotune = 1 times the compiled wave counts of the tile passes on the
+ * context's own tiles before its first run (default).  A negative value leaves
+ * an option unchanged.  No reference counterpart (launch configuration). */
+int g12_configure(g12_ctx* ctx, int use_graphs, int autotune);
+
 /* Waits for all work on the library's GPU (device-wide synchronisation). */
 int g12_synchronize(void);
 
 /* Device footprint of a context in bytes. */
 int g12_memory_bytes(g12_ctx* ctx, int64_t* bytes);
+
+/* ---- runs of many trajectories -------------------------------------------
+ * A batch advances ntraj independent trajectories together: every RK4 stage is
+ * one set of launches over all of them (the tile passes are persistent
+ * launches over the tiles of every trajectory with the same padded size).
+ * Each trajectory has its own parameter block (sizes, hoppings, U, long-range law,
+ * quench strength and extent, HF flag); the time protocol (dt, time_steps,
+ * AS_*, quench_rate/midpoints/q_type/gate) and the precision must agree, else
+ * G12_E_ARG.  The reference runs one trajectory per process (submit.sh sweeps
+ * U by launching processes); these calls replace that loop.
+ *
+ * g12_batch_adiabatic_switching: steps[i] = switching length of trajectory i
+ * (it depends on U through the exact U_AS == U test of AS.h:96-100).
+ * g12_batch_rk4: traj[i] (or NULL) receives trajectory i's G1 before every
+ * step (capacity traj_cap matrices); steps[i] = its length.  A trajectory
+ * whose density guard trips stops there while the others go on; the call
+ * returns G12_GUARD if any tripped.  g12_batch_guard copies the guard flags
+ * (0, or failing step + 1).  g12_batch_buckets reports, per padded tile size,
+ * the number of trajectories and the wave counts in use; it returns the number
+ * of buckets. */
+int g12_batch_create(const g12_params* p, int ntraj, g12_batch** out);
+int g12_batch_destroy(g12_batch* b);
+int g12_batch_size(g12_batch* b, int* ntraj);
+int g12_batch_memory_bytes(g12_batch* b, int64_t* bytes);
+int g12_batch_set_state(g12_batch* b, int i, const double* G1, const double* G2xx, const double* G2xy);
+int g12_batch_get_state(g12_batch* b, int i, double* G1, double* G2xx, double* G2xy);
+int g12_batch_adiabatic_switching(g12_batch* b, int* steps);
+int g12_batch_rk4(g12_batch* b, double* const* traj, int64_t traj_cap, int64_t* steps);
+int g12_batch_rk4_advance(g12_batch* b, int64_t n);
+int g12_batch_guard(g12_batch* b, int* fail);
+int g12_batch_timing_enable(g12_batch* b, int on);
+int g12_batch_timing_get(g12_batch* b, int which, int64_t* launches, double* total_ms);
+int g12_batch_configure(g12_batch* b, int use_graphs, int autotune);
+int g12_batch_tune(g12_batch* b);
+int g12_batch_buckets(g12_batch* b, int cap, int* NP, int* count, int* waves_strided, int* waves_fused);
 
 /* ---- routine-level entry points ------------------------------------------
  * One entry per reference routine, same argument roles and order, host
