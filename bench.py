@@ -91,6 +91,91 @@ def run_leg(g, precision, G1, Gxx, Gxy, steps, warmup):
     return 1e3 * (t1 - t0) / steps, G1_end, k
 
 
+ENSEMBLE_DRAWS = os.path.join(ROOT, "tests", "golden", "ensemble_config4.json")
+
+
+def ensemble_protocol(precision):
+    """Shared protocol of the 256-trajectory ensemble: dt = 0.02, 4000 steps,
+    default switching (rate 0.5, midpoint 10), full quench ramp (rate 2,
+    midpoint 30), V_n = U/(1+n) where long-range interactions are on."""
+    return dict(dt=0.02, time_steps=4000, V_law="coulomb", q_type="full", quench_rate=2.0,
+                quench_on_midpoint=30.0, AS_rate=0.5, AS_midpoint=10.0, precision=precision)
+
+
+def switching_steps(g, p):
+    """Length of the adiabatic switching (AS.h:87-214), as the library counts it."""
+    count, n, have, last, t = 0, 0, False, 0.0, 0.0
+    while count < p.AS_tail:
+        if have and abs(last - p.U) <= 0:
+            count += 1
+        for k in range(4):
+            dtk = p.AS_dt / 2 if k in (1, 2) else p.AS_dt
+            last = g.fermi(t if k == 0 else t + dtk, p.AS_midpoint, p.AS_rate) * p.U
+        have = True
+        t += p.AS_dt
+        n += 1
+    return n
+
+
+def production_steps(p):
+    n, t = 0, 0.0
+    while t < p.time_steps * p.dt:  # accumulated time (RK_solver.h:96,216)
+        t += p.dt
+        n += 1
+    return n
+
+
+def ensemble_run(g, precision, steps, warmup):
+    """Times `steps` RK4 steps of the whole ensemble (one batch); returns
+    (ms/step, G1 of every trajectory after warmup + steps, batch facts)."""
+    draws = json.load(open(ENSEMBLE_DRAWS))["draws"]
+    ps = g.ensemble_params(draws, **ensemble_protocol(precision))
+    b = g.Batch(ps)
+    for i, p in enumerate(ps):
+        b.set_state(i, g.make_G0(p))
+    b.tune()
+    if warmup:
+        b.advance(warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    rc = b.advance(steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    if rc != 0:
+        raise SystemExit(f"ensemble: density guard tripped ({precision})")
+    G1 = [b.get_G1(i) for i in range(len(ps))]
+    facts = {"buckets": b.buckets(), "memory_GB": round(b.memory_bytes() / 1e9, 2),
+             "as_steps": max(switching_steps(g, p) for p in ps), "rk_steps": production_steps(ps[0]),
+             "N": [p.Ns * p.Nb for p in ps]}
+    b.close()
+    return 1e3 * (t1 - t0) / steps, G1, facts
+
+
+def ensemble_leg(g, steps, warmup, check_fp64=True):
+    import numpy as np
+    ms, G1m, f = ensemble_run(g, "fp32_3m", steps, warmup)
+    per_run = f["as_steps"] + f["rk_steps"]
+    out = {"metric": "trajectories/s", "value": round(256.0 / (per_run * ms * 1e-3), 4),
+           "unit": "trajectories/s", "trajectories": 256, "precision": "fp32_3m",
+           "ms_per_step": round(ms, 3), "steps": steps, "warmup": warmup,
+           "steps_per_trajectory": per_run, "as_steps": f["as_steps"], "rk_steps": f["rk_steps"],
+           "derivation": "value = trajectories / (steps per trajectory x measured ms per ensemble step); "
+                         "every step of a run (switching or production) does the same work",
+           "algorithmic_flop_per_step": float(sum(256.0 * n ** 5 for n in f["N"])),
+           "buckets": [{"NP": k[0], "trajectories": k[1], "waves_strided": k[2], "waves_fused": k[3]}
+                       for k in f["buckets"]],
+           "memory_GB": f["memory_GB"], "config": "256 seeded draws (tests/golden/ensemble_config4.json), "
+                                                  "dt=0.02, 4000 steps, FP32-3M tile products"}
+    out["achieved_tflops"] = round(out["algorithmic_flop_per_step"] / (ms * 1e-3) / 1e12, 3)
+    if check_fp64:
+        ms64, G1d, _ = ensemble_run(g, "fp64", steps, warmup)
+        errs = [float(np.abs(np.diagonal(a).imag - np.diagonal(b).imag).max()) for a, b in zip(G1m, G1d)]
+        out["fp64_ms_per_step"] = round(ms64, 3)
+        out["max_density_err_vs_fp64"] = max(errs)
+        out["err_horizon_steps"] = warmup + steps
+    return out
+
+
 def roofline(N, precision, k, steps):
     # dominant kernel: the tile pass with the larger summed time; one launch of
     # either pass applies two of the four one-particle terms of [H2,G2] to both
@@ -121,6 +206,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-trajectory ensemble leg")
+    ap.add_argument("--ensemble-steps", type=int, default=3)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {a.gpus} is not supported "
@@ -151,6 +238,7 @@ def main():
                                "max_density_err_vs_fp64": float(np.abs(np.diagonal(G1m).imag - d64).max()),
                                "err_horizon_steps": a.warmup + a.steps,
                                "roofline": roofline(N, prec, km, a.steps)}
+    ensemble = None if a.no_ensemble else ensemble_leg(g, a.ensemble_steps, 1)
     cpu = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
@@ -165,7 +253,7 @@ def main():
            "config": {"workload": "Ns=32 Nb=2 (N=64) EHM V=U/(1+n) U=1.5 t1=1 t2=0.5 full quench 1.0 on 8 sites "
                                   "dt=0.01", "N": N, "precision": a.precision,
                       "algorithmic_flop_per_step": 256.0 * N ** 5},
-           "roofline": roofline(N, a.precision, k_head, a.steps), "mixed": mixed, "cpu_baseline": cpu}
+           "roofline": roofline(N, a.precision, k_head, a.steps), "mixed": mixed, "ensemble": ensemble, "cpu_baseline": cpu}
     print(json.dumps(out))
 
 

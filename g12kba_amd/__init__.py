@@ -204,6 +204,18 @@ def make_params(**kw):
     return p
 
 
+def ensemble_params(draws, **protocol):
+    """Params of each trajectory of an ensemble: physics from each draw (dicts
+    of Params fields, e.g. Ns, Nb, U, t1, t2, quench_strength, Nq, EHM), the
+    time protocol and precision shared (keyword arguments)."""
+    out = []
+    for d in draws:
+        kw = dict(protocol)
+        kw.update(d)
+        out.append(make_params(**kw))
+    return out
+
+
 def parse_input(path):
     p = default_params()
     _check(lib.g12_parse_input(path.encode(), ctypes.byref(p)), "g12_parse_input")

@@ -709,17 +709,18 @@ struct TileSched {
     int wg_per_xcd, xcd, local;
     bool xcd_mode;
     __device__ TileSched(int64_t total_, int grid, int b) : total(total_) {
-        xcd_mode = (total % 8 == 0) && (grid % 8 == 0);
+        xcd_mode = (grid % 8) == 0;
         xcd = b & 7;
         local = b >> 3;
-        per_xcd = total >> 3;
+        per_xcd = (total + 7) >> 3;  // the last XCD's range may be shorter
         wg_per_xcd = grid >> 3;
     }
     // tile of iteration `it` for this workgroup, or -1 when done
     __device__ int64_t tile(int it, int b, int grid) const {
         if (xcd_mode) {
             const int64_t i = (int64_t)it * wg_per_xcd + local;
-            return i < per_xcd ? xcd * per_xcd + i : -1;
+            const int64_t g = xcd * per_xcd + i;
+            return (i < per_xcd && g < total) ? g : -1;
         }
         const int64_t i = b + (int64_t)it * grid;
         return i < total ? i : -1;
