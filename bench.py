@@ -176,6 +176,39 @@ def ensemble_leg(g, steps, warmup, check_fp64=True):
     return out
 
 
+def extrapolator_leg(g, total=2000, prefix=400):
+    """Learned extrapolator on the N = 48 workload over a shortened horizon
+    (quench onset at t = 8, inside it), judged by plain RK4."""
+    import numpy as np
+    from g12kba_amd import extrapolator as xp
+    if not os.path.exists(xp.DEFAULT_MODEL):
+        return {"error": "trained network missing"}
+    p = xp.config5_params(time_steps=total)
+    p.quench_gate_time = p.quench_on_midpoint = 8.0
+    judge, t_judge = xp.rk4_run(p, total=total)
+    out = {"workload": f"Ns=24 Nb=2 (N=48) EHM V=U/(1+n) U=2 t1=1 t2=0.4 full quench 1.2 on 6 sites dt=0.01, "
+                       f"{total} steps (RK4 prefix {prefix}), quench onset at t=8",
+           "judge_rk4_seconds": round(t_judge, 3)}
+    with xp.Extrapolator() as m:
+        m.load(xp.DEFAULT_MODEL)
+        for name, kw in (("strict", dict(trace_tol=1e-6, herm_tol=1e-5)),
+                         ("projected", dict(trace_tol=1e-3, herm_tol=1e-3, project=True))):
+            r = xp.extrapolated_run(p, m, prefix=prefix, total=total, **kw)
+            dens = np.abs(np.diagonal(r["traj"] - judge, axis1=1, axis2=2).imag).max()
+            out[name] = {"thresholds": [kw["trace_tol"], kw["herm_tol"]], "rk4_steps": r["rk_steps"],
+                         "network_steps": int(r["by_net"].sum()), "rejected_blocks": r["rejected_blocks"],
+                         "seconds": round(r["seconds"], 3), "speedup_vs_rk4": round(t_judge / r["seconds"], 3),
+                         "max_density_err_vs_rk4": float(dens)}
+        # network throughput alone: 64 blocks fed back from the judge's history
+        w = xp.quench_shift(p, np.arange(prefix - m.hist, prefix + 64 * m.horizon))
+        m.rollout(judge[prefix - m.hist:prefix], p.Ns, p.Nq, w, 2)
+        t0 = time.perf_counter()
+        m.rollout(judge[prefix - m.hist:prefix], p.Ns, p.Nq, w, 64, project=True)
+        out["rollout_steps_per_s"] = round(64 * m.horizon / (time.perf_counter() - t0), 1)
+    out["rk4_steps_per_s"] = round((total + 304) / t_judge, 1)
+    return out
+
+
 def roofline(N, precision, k, steps):
     # dominant kernel: the tile pass with the larger summed time; one launch of
     # either pass applies two of the four one-particle terms of [H2,G2] to both
@@ -208,6 +241,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-trajectory ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=3)
+    ap.add_argument("--no-extrapolator", action="store_true", help="skip the learned-extrapolator leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {a.gpus} is not supported "
@@ -239,6 +273,7 @@ def main():
                                "err_horizon_steps": a.warmup + a.steps,
                                "roofline": roofline(N, prec, km, a.steps)}
     ensemble = None if a.no_ensemble else ensemble_leg(g, a.ensemble_steps, 1)
+    extrap = None if a.no_extrapolator else extrapolator_leg(g)
     cpu = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
@@ -253,7 +288,8 @@ def main():
            "config": {"workload": "Ns=32 Nb=2 (N=64) EHM V=U/(1+n) U=1.5 t1=1 t2=0.5 full quench 1.0 on 8 sites "
                                   "dt=0.01", "N": N, "precision": a.precision,
                       "algorithmic_flop_per_step": 256.0 * N ** 5},
-           "roofline": roofline(N, a.precision, k_head, a.steps), "mixed": mixed, "ensemble": ensemble, "cpu_baseline": cpu}
+           "roofline": roofline(N, a.precision, k_head, a.steps), "mixed": mixed, "ensemble": ensemble, "extrapolator": extrap,
+           "cpu_baseline": cpu}
     print(json.dumps(out))
 
 

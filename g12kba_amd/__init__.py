@@ -51,6 +51,11 @@ class Params(ctypes.Structure):
     ]
 
 
+class XPConfig(ctypes.Structure):
+    _fields_ = [("hist", ctypes.c_int), ("horizon", ctypes.c_int), ("hidden", ctypes.c_int),
+                ("dt", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -71,6 +76,7 @@ def _load():
         "g12_adiabatic_switching": [vp, ctypes.POINTER(i)],
         "g12_rk4": [vp, p, i64, pi64],
         "g12_rk4_advance": [vp, i64, pi64],
+        "g12_rk4_record": [vp, i64, p, i64, pi64],
         "g12_get_time": [vp, p, pi64],
         "g12_timing_enable": [vp, i],
         "g12_timing_get": [vp, i, pi64, p],
@@ -99,6 +105,7 @@ def _load():
         "g12_batch_adiabatic_switching": [vp, ctypes.POINTER(i)],
         "g12_batch_rk4": [vp, ctypes.POINTER(p), i64, pi64],
         "g12_batch_rk4_advance": [vp, i64],
+        "g12_batch_rk4_record": [vp, i64, ctypes.POINTER(p), i64],
         "g12_batch_guard": [vp, ctypes.POINTER(i)],
         "g12_batch_timing_enable": [vp, i],
         "g12_batch_timing_get": [vp, i, pi64, p],
@@ -107,6 +114,24 @@ def _load():
         "g12_batch_buckets": [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
                               ctypes.POINTER(i)],
     }
+    fp = ctypes.POINTER(ctypes.c_float)
+    xcfg = ctypes.POINTER(XPConfig)
+    sig.update({
+        "g12_xp_default_config": [xcfg],
+        "g12_xp_create": [xcfg, ctypes.POINTER(vp)],
+        "g12_xp_destroy": [vp],
+        "g12_xp_param_count": [vp, pi64],
+        "g12_xp_get_params": [vp, fp],
+        "g12_xp_set_params": [vp, fp],
+        "g12_xp_save": [vp, ctypes.c_char_p],
+        "g12_xp_load": [vp, ctypes.c_char_p],
+        "g12_xp_set_data": [vp, fp, fp, fp, i64, i],
+        "g12_xp_train": [vp, i, i, d, fp],
+        "g12_xp_loss_grad": [vp, fp, fp, i, p, fp],
+        "g12_xp_forward": [vp, fp, i, fp],
+        "g12_xp_features": [vp, p, i, i, i, p, fp, p, p],
+        "g12_xp_rollout": [vp, p, i, i, i, p, p, i, i, p, p],
+    })
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -123,15 +148,18 @@ lib = _load()
 EXPORTED = [
     "g12_params_default", "g12_parse_input", "g12_make_g0", "g12_create", "g12_destroy",
     "g12_set_state", "g12_get_state", "g12_adiabatic_switching", "g12_rk4", "g12_rk4_advance",
-    "g12_get_time", "g12_timing_enable", "g12_timing_get", "g12_memory_bytes", "g12_synchronize",
+    "g12_get_time", "g12_rk4_record", "g12_timing_enable", "g12_timing_get", "g12_memory_bytes", "g12_synchronize",
     "g12_hf_hamiltonian", "g12_h1_commutator", "g12_h2_commutator", "g12_collision",
     "g12_psi_xxxx", "g12_psi_xyxy", "g12_pi", "g12_g1_step", "g12_g2_step", "g12_fermi",
     "g12_dipole", "g12_write_outputs", "g12_save_state", "g12_load_state", "g12_version",
     "g12_configure", "g12_batch_create", "g12_batch_destroy", "g12_batch_size",
     "g12_batch_memory_bytes", "g12_batch_set_state", "g12_batch_get_state",
-    "g12_batch_adiabatic_switching", "g12_batch_rk4", "g12_batch_rk4_advance", "g12_batch_guard",
+    "g12_batch_adiabatic_switching", "g12_batch_rk4", "g12_batch_rk4_advance", "g12_batch_rk4_record", "g12_batch_guard",
     "g12_batch_timing_enable", "g12_batch_timing_get", "g12_batch_configure", "g12_batch_tune",
-    "g12_batch_buckets",
+    "g12_batch_buckets", "g12_xp_default_config", "g12_xp_create", "g12_xp_destroy",
+    "g12_xp_param_count", "g12_xp_get_params", "g12_xp_set_params", "g12_xp_save", "g12_xp_load",
+    "g12_xp_set_data", "g12_xp_train", "g12_xp_loss_grad", "g12_xp_forward", "g12_xp_features",
+    "g12_xp_rollout",
 ]
 
 
@@ -202,6 +230,27 @@ def make_params(**kw):
             v = int(bool(v))
         setattr(p, k, v)
     return p
+
+
+def draw_ensemble(seed=20261203, count=256, max_n=None):
+    """Seeded parameter draws of an ensemble: Ns uniform in {6..20}, Nb uniform
+    in {1,2,3}, U uniform in [0.5, 3], t2 uniform in [0.1, 0.8], quench strength
+    uniform in [0.1, 2] on Nq = Ns // 4 sites, long-range interactions with
+    probability 0.5, t1 = 1 (numpy default_rng, drawn in that order; draws with
+    Ns * Nb > max_n are skipped)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        Ns = int(rng.integers(6, 21))
+        Nb = int(rng.integers(1, 4))
+        U = float(rng.uniform(0.5, 3.0))
+        t2 = float(rng.uniform(0.1, 0.8))
+        s = float(rng.uniform(0.1, 2.0))
+        ehm = bool(rng.random() < 0.5)
+        if max_n is not None and Ns * Nb > max_n:
+            continue
+        out.append(dict(Ns=Ns, Nb=Nb, U=U, t1=1.0, t2=t2, quench_strength=s, Nq=Ns // 4, EHM=ehm))
+    return out
 
 
 def ensemble_params(draws, **protocol):
@@ -421,6 +470,13 @@ class Propagator:
         n = ctypes.c_int64(0)
         return _check(lib.g12_rk4_advance(self.h, nsteps, ctypes.byref(n)), "g12_rk4_advance")
 
+    def record(self, nsteps):
+        """Advance nsteps from the current time; returns (rc, G1 before each step)."""
+        traj = np.zeros((max(nsteps, 1), self.N, self.N), np.complex128)
+        n = ctypes.c_int64(0)
+        rc = _check(lib.g12_rk4_record(self.h, nsteps, _p(traj), nsteps, ctypes.byref(n)), "g12_rk4_record")
+        return rc, traj[:nsteps]
+
     def time(self):
         t = ctypes.c_double(0)
         n = ctypes.c_int64(0)
@@ -529,6 +585,16 @@ class Batch:
 
     def advance(self, nsteps):
         return _check(lib.g12_batch_rk4_advance(self.h, nsteps), "g12_batch_rk4_advance")
+
+    def record(self, nsteps, out=None, offset=0):
+        """nsteps more production steps; G1 before each goes to out[i][offset:]
+        (new arrays when out is None).  Returns (rc, out)."""
+        if out is None:
+            out = [np.zeros((nsteps, N, N), np.complex128) for N in self.N]
+            offset = 0
+        ptrs = (_dp * self.n)(*[_p(o[offset:]) for o in out])
+        rc = _check(lib.g12_batch_rk4_record(self.h, nsteps, ptrs, nsteps), "g12_batch_rk4_record")
+        return rc, out
 
     def guard(self):
         f = (ctypes.c_int * self.n)()

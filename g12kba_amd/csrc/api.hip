@@ -792,6 +792,17 @@ int g12_rk4_advance(g12_ctx* c, int64_t nsteps, int64_t* done) {
     return rc;
 }
 
+int g12_rk4_record(g12_ctx* c, int64_t nsteps, double* traj, int64_t traj_cap, int64_t* done) {
+    if (!c || nsteps < 0) return G12_E_ARG;
+    Run& r = c->r;
+    double* tr[1] = {traj};
+    const int64_t s0 = r.steps;
+    const int rc = run_production(r, nsteps, traj ? tr : nullptr, traj_cap, s0);
+    if (rc) return rc;
+    if (done) *done = nsteps;
+    return r.h_fail[0] ? G12_GUARD : 0;
+}
+
 int g12_timing_enable(g12_ctx* c, int on) { return c ? timing_enable(c->r, on) : G12_E_ARG; }
 
 int g12_timing_get(g12_ctx* c, int which, int64_t* launches, double* total_ms) {
@@ -854,6 +865,16 @@ int g12_batch_rk4(g12_batch* b, double* const* traj, int64_t traj_cap, int64_t* 
 int g12_batch_rk4_advance(g12_batch* b, int64_t nsteps) {
     if (!b || nsteps < 0) return G12_E_ARG;
     return run_advance(b->r, nsteps);
+}
+
+int g12_batch_rk4_record(g12_batch* b, int64_t nsteps, double* const* traj, int64_t traj_cap) {
+    if (!b || nsteps < 0) return G12_E_ARG;
+    Run& r = b->r;
+    const int rc = run_production(r, nsteps, traj, traj_cap, r.steps);
+    if (rc) return rc;
+    for (int i = 0; i < r.ntraj; ++i)
+        if (r.h_fail[i]) return G12_GUARD;
+    return 0;
 }
 
 int g12_batch_guard(g12_batch* b, int* fail) {

@@ -1,0 +1,788 @@
+// Learned G1 extrapolator: a temporal MLP applied to every upper-triangle
+// element of G1 (with a linear skip path), trained and run with hand-written
+// BF16 MFMA GEMMs (FP32 accumulation, FP32 master weights, Adam).
+//
+// Per element (p, q), p <= q, a window of `hist` steps h_0..h_{hist-1} and the
+// drive difference d_k = w(t_k) (m_p - m_q) over hist + horizon steps
+// (w: quench shift, m: 1 on quenched sites) give the features
+//   ref = h_{hist-1},  s = max( max_k |h_k - ref|, 4 dt max_k |d_k| |ref|, 1e-6 max(1, |ref|) )
+//   x[2k], x[2k+1]                 = Re, Im (h_k - ref) / s            k < hist
+//   x[2 hist + 2k], x[2 hist+2k+1] = Re, Im (-i d_k dt ref) / s       k < hist + horizon
+// and the network predicts y[2j], y[2j+1] = Re, Im (g_j - ref) / s for the
+// next `horizon` steps g_j.  Layers: A1 = tanh(X W1 + b1), A2 = tanh(A1 W2 + b2),
+// A3 = tanh(A2 W3 + b3), Y = A3 W4 + b4 + X Ws.  Training loss per output with
+// residual r = y - target: r^2 for |r| <= 1, 2|r| - 1 beyond (a Huber loss that
+// keeps windows whose future leaves the history's range from dominating).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/g12kba.h"
+
+namespace {
+
+#define XCHK(x)                                                                        \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) {                                                        \
+            std::fprintf(stderr, "g12kba: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+            return G12_E_HIP;                                                          \
+        }                                                                              \
+    } while (0)
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// GEMM  C[M][N] (ldc) = epilogue( sum_k op(A)[m][k] op(B)[k][n] )
+// op(A)[m][k] = TA ? A[k * lda + m] : A[m * lda + k]
+// op(B)[k][n] = TB ? B[n * ldb + k] : B[k * ldb + n]
+// Operands are rounded to BF16 when staged in LDS; products accumulate in FP32
+// on v_mfma_f32_16x16x32_bf16.  64 x 64 output tile per workgroup, 4 waves of
+// 32 x 32 (2 x 2 blocks of 16 x 16), k steps of 32.
+enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_TANH = 2, EPI_DTANH = 3, EPI_ADD = 4 };
+
+constexpr int GT = 64, GK = 32, GKP = GK + 8;  // tile, k step, LDS pitch (bf16)
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, int lda, const float* __restrict__ B,
+                                              int ldb, float* C, int ldc, int M, int N, int K, int epi,
+                                              const float* __restrict__ bias, const float* aux) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[GT * GKP];  // [m][k]
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[GT * GKP];  // [n][k]
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f4 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += GK) {
+        for (int e = tid; e < GT * GK; e += 256) {
+            int m, k;
+            if (TA) {
+                k = e / GT;
+                m = e % GT;
+            } else {
+                m = e / GK;
+                k = e % GK;
+            }
+            const int gm = m0 + m, gk = k0 + k;
+            float v = 0.f;
+            if (gm < M && gk < K) v = TA ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk];
+            As[m * GKP + k] = (__bf16)v;
+        }
+        for (int e = tid; e < GT * GK; e += 256) {
+            int n, k;
+            if (TB) {
+                n = e / GK;
+                k = e % GK;
+            } else {
+                k = e / GT;
+                n = e % GT;
+            }
+            const int gn = n0 + n, gk = k0 + k;
+            float v = 0.f;
+            if (gn < N && gk < K) v = TB ? B[(int64_t)gn * ldb + gk] : B[(int64_t)gk * ldb + gn];
+            Bs[n * GKP + k] = (__bf16)v;
+        }
+        __syncthreads();
+        const int r = lane & 15, kq = (lane >> 4) * 8;
+        bf8 a[2], b[2];
+        for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf8*>(&As[(wm + 16 * i + r) * GKP + kq]);
+        for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf8*>(&Bs[(wn + 16 * j + r) * GKP + kq]);
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+    // result register t of lane l: row 4 (l / 16) + t, column l % 16 of the block
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int t = 0; t < 4; ++t) {
+                const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + t;
+                const int n = n0 + wn + 16 * j + (lane & 15);
+                if (m >= M || n >= N) continue;
+                float v = acc[i][j][t];
+                float* c = C + (int64_t)m * ldc + n;
+                switch (epi) {
+                    case EPI_BIAS: v += bias[n]; break;
+                    case EPI_BIAS_TANH: v = tanhf(v + bias[n]); break;
+                    case EPI_DTANH: {
+                        const float a_ = aux[(int64_t)m * ldc + n];
+                        v *= 1.f - a_ * a_;
+                        break;
+                    }
+                    case EPI_ADD: v += *c; break;
+                    default: break;
+                }
+                *c = v;
+            }
+}
+
+// db[n] = sum_m dZ[m][n]
+__global__ void k_colsum(const float* __restrict__ dZ, int M, int N, float* db) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.f;
+    for (int m = 0; m < M; ++m) s += dZ[(int64_t)m * N + n];
+    db[n] = s;
+}
+
+// Loss r^2 (|r| <= 1) or 2|r| - 1 per output, r = Y - T, averaged over count:
+// dY = 2 clamp(r, -1, 1) / count and per-block partial sums of the loss.
+__global__ void k_mse(const float* __restrict__ Y, const float* __restrict__ T, int64_t count, float* dY,
+                      float* partial) {
+    __shared__ float red[256];
+    float s = 0.f;
+    const float sc = 2.f / (float)count;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        const float d = Y[i] - T[i];
+        const float a = fabsf(d);
+        dY[i] = sc * fminf(fmaxf(d, -1.f), 1.f);
+        s += a <= 1.f ? d * d : 2.f * a - 1.f;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_adam(float* p, const float* __restrict__ g, float* m, float* v, int64_t n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// features (shared by the training gather and the rollout)
+
+struct FeatDims {
+    int hist, horizon, din, dout;
+};
+
+// h: hist complex values (oldest first); d: hist + horizon drive differences.
+// Writes x[din]; returns ref and s.
+__device__ inline void make_features(const FeatDims& fd, double dt, const double2* h, const double* d, float* x,
+                                     double2* ref_out, double* s_out) {
+    const double2 ref = h[fd.hist - 1];
+    double var = 0.0, dmax = 0.0;
+    for (int k = 0; k < fd.hist; ++k) {
+        const double a = h[k].x - ref.x, b = h[k].y - ref.y;
+        var = fmax(var, sqrt(a * a + b * b));
+    }
+    for (int k = 0; k < fd.hist + fd.horizon; ++k) dmax = fmax(dmax, fabs(d[k]));
+    const double aref = sqrt(ref.x * ref.x + ref.y * ref.y);
+    const double s = fmax(fmax(var, 4.0 * dt * dmax * aref), 1e-6 * fmax(1.0, aref));
+    for (int k = 0; k < fd.hist; ++k) {
+        x[2 * k] = (float)((h[k].x - ref.x) / s);
+        x[2 * k + 1] = (float)((h[k].y - ref.y) / s);
+    }
+    for (int k = 0; k < fd.hist + fd.horizon; ++k) {
+        // -i d dt ref = d dt (Im ref, -Re ref)
+        x[2 * fd.hist + 2 * k] = (float)(d[k] * dt * ref.y / s);
+        x[2 * fd.hist + 2 * k + 1] = (float)(-d[k] * dt * ref.x / s);
+    }
+    *ref_out = ref;
+    *s_out = s;
+}
+
+__device__ inline uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Training minibatch: sample i draws series s and window start t0 from a hash
+// of (seed, step, i); the window and the next `horizon` steps come from the
+// complex64 series, the drive from coef[s] * drive[t].
+__global__ void k_gather(FeatDims fd, double dt, const float2* __restrict__ series, const float* __restrict__ coef,
+                         const float* __restrict__ drive, int64_t n_series, int T, uint64_t seed, int64_t step,
+                         int M, float* X, float* Y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const uint64_t r = mix64(seed ^ mix64((uint64_t)step * 0x100000001B3ull + (uint64_t)i));
+    const int64_t s = (int64_t)(r % (uint64_t)n_series);
+    const int span = fd.hist + fd.horizon;
+    const int t0 = (int)(mix64(r) % (uint64_t)(T - span + 1));
+    double2 h[128];
+    double d[160];
+    const float2* src = series + s * T + t0;
+    for (int k = 0; k < fd.hist; ++k) h[k] = make_double2(src[k].x, src[k].y);
+    for (int k = 0; k < span; ++k) d[k] = (double)coef[s] * (double)drive[t0 + k];
+    double2 ref;
+    double sc;
+    make_features(fd, dt, h, d, X + (int64_t)i * fd.din, &ref, &sc);
+    float* y = Y + (int64_t)i * fd.dout;
+    for (int j = 0; j < fd.horizon; ++j) {
+        const float2 g = src[fd.hist + j];
+        y[2 * j] = (float)(((double)g.x - ref.x) / sc);
+        y[2 * j + 1] = (float)(((double)g.y - ref.y) / sc);
+    }
+}
+
+// Rollout input: element e of the upper triangle (row-major over p <= q),
+// history from the ring of G1 matrices (slot of step k: (first + k) % cap).
+__global__ void k_roll_in(FeatDims fd, double dt, const double2* __restrict__ ring, int cap, int first, int N,
+                          int Ns, int Nq, const double* __restrict__ w, float* X, double2* ref, double* scale) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_el = N * (N + 1) / 2;
+    if (e >= n_el) return;
+    int p = 0, rem = e;
+    while (rem >= N - p) {
+        rem -= N - p;
+        ++p;
+    }
+    const int q = p + rem;
+    const double mp = (p % Ns) < Nq ? 1.0 : 0.0, mq = (q % Ns) < Nq ? 1.0 : 0.0;
+    double2 h[128];
+    double d[160];
+    for (int k = 0; k < fd.hist; ++k) h[k] = ring[(int64_t)((first + k) % cap) * N * N + p * N + q];
+    for (int k = 0; k < fd.hist + fd.horizon; ++k) d[k] = w[k] * (mp - mq);
+    make_features(fd, dt, h, d, X + (int64_t)e * fd.din, &ref[e], &scale[e]);
+}
+
+// Rollout output: predicted steps written to the ring (slots first + hist + j)
+// and to `out` (horizon x N x N), lower triangle by G1^dagger = -G1.
+__global__ void k_roll_out(FeatDims fd, const float* __restrict__ Y, const double2* __restrict__ ref,
+                           const double* __restrict__ scale, double2* ring, int cap, int first, int N, double2* out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_el = N * (N + 1) / 2;
+    if (e >= n_el) return;
+    int p = 0, rem = e;
+    while (rem >= N - p) {
+        rem -= N - p;
+        ++p;
+    }
+    const int q = p + rem;
+    const double2 r = ref[e];
+    const double s = scale[e];
+    const float* y = Y + (int64_t)e * fd.dout;
+    for (int j = 0; j < fd.horizon; ++j) {
+        const double2 g = make_double2(r.x + s * (double)y[2 * j], r.y + s * (double)y[2 * j + 1]);
+        double2* dst = ring + (int64_t)((first + fd.hist + j) % cap) * N * N;
+        dst[p * N + q] = g;
+        if (p != q) dst[q * N + p] = make_double2(-g.x, g.y);
+        if (out) {
+            double2* o = out + (int64_t)j * N * N;
+            o[p * N + q] = g;
+            if (p != q) o[q * N + p] = make_double2(-g.x, g.y);
+        }
+    }
+}
+
+// Checks of the `horizon` predicted matrices: max |tr G - tr0| and the
+// anti-hermiticity error max |G + G^dagger| (= 2 max |Re G_ii| here).
+__global__ void k_roll_check(const double2* __restrict__ ring, int cap, int first, int hist, int horizon, int N,
+                             double2 tr0, double* res) {
+    __shared__ double dr[256], hr[256];
+    double drift = 0.0, herm = 0.0;
+    for (int j = 0; j < horizon; ++j) {
+        const double2* G = ring + (int64_t)((first + hist + j) % cap) * N * N;
+        // trace: one thread sums (N <= 64), the others idle
+        if (threadIdx.x == 0) {
+            double tx = 0.0, ty = 0.0;
+            for (int i = 0; i < N; ++i) {
+                tx += G[i * N + i].x;
+                ty += G[i * N + i].y;
+            }
+            drift = fmax(drift, sqrt((tx - tr0.x) * (tx - tr0.x) + (ty - tr0.y) * (ty - tr0.y)));
+        }
+        for (int i = threadIdx.x; i < N; i += blockDim.x) herm = fmax(herm, 2.0 * fabs(G[i * N + i].x));
+    }
+    dr[threadIdx.x] = drift;
+    hr[threadIdx.x] = herm;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            dr[threadIdx.x] = fmax(dr[threadIdx.x], dr[threadIdx.x + w]);
+            hr[threadIdx.x] = fmax(hr[threadIdx.x], hr[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        res[0] = dr[0];
+        res[1] = hr[0];
+    }
+}
+
+// Projection of the `horizon` predicted matrices onto the constraints the
+// dynamics conserve: Re G_ii = 0 (G1^dagger = -G1) and Im tr G = Im tr0, the
+// trace difference spread evenly over the diagonal.  Ring and out alike.
+__global__ void k_roll_project(double2* ring, int cap, int first, int hist, int horizon, int N, double2 tr0,
+                               double2* out) {
+    const int j = blockIdx.x;
+    if (j >= horizon) return;
+    double2* G = ring + (int64_t)((first + hist + j) % cap) * N * N;
+    __shared__ double tr_im;
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < N; ++i) t += G[i * N + i].y;
+        tr_im = t;
+    }
+    __syncthreads();
+    const double shift = (tr0.y - tr_im) / N;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const double2 v = make_double2(0.0, G[i * N + i].y + shift);
+        G[i * N + i] = v;
+        if (out) out[(int64_t)j * N * N + i * N + i] = v;
+    }
+}
+
+unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+int select_xp_device() {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return -1;
+    const char* s = std::getenv("G12KBA_DEVICE");
+    int dev = s ? std::atoi(s) : 0;
+    if (dev < 0 || dev >= count) dev = 0;
+    return hipSetDevice(dev) == hipSuccess ? dev : -1;
+}
+
+}  // namespace
+
+struct g12_xp {
+    g12_xp_config cfg;
+    FeatDims fd;
+    int H;
+    int64_t np = 0;
+    // offsets of the parameter blocks (floats)
+    int64_t oW1, ob1, oW2, ob2, oW3, ob3, oW4, ob4, oWs;
+    float *P = nullptr, *G = nullptr, *Mm = nullptr, *Vv = nullptr;  // params, grads, Adam moments
+    int64_t adam_t = 0;
+    // activations (capacity cap_m rows)
+    int cap_m = 0;
+    float *X = nullptr, *A1 = nullptr, *A2 = nullptr, *A3 = nullptr, *Y = nullptr, *T = nullptr, *dY = nullptr;
+    float *D1 = nullptr, *D2 = nullptr, *D3 = nullptr, *partial = nullptr;
+    // dataset
+    float2* series = nullptr;
+    float* coef = nullptr;
+    float* drive = nullptr;
+    int64_t n_series = 0;
+    int T_len = 0;
+    // rollout
+    double2 *ring = nullptr, *rout = nullptr, *ref = nullptr;
+    double *scale = nullptr, *w = nullptr, *chk = nullptr;
+    int ring_N = 0, ring_cap = 0;
+    hipStream_t s = nullptr;
+};
+
+namespace {
+
+void gemm(g12_xp* x, bool ta, bool tb, const float* A, int lda, const float* B, int ldb, float* C, int ldc, int M,
+          int N, int K, int epi, const float* bias = nullptr, const float* aux = nullptr) {
+    dim3 grid(blocks(N, GT), blocks(M, GT));
+    if (!ta && !tb)
+        hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(256), 0, x->s, A, lda, B, ldb, C, ldc, M, N, K, epi, bias, aux);
+    else if (ta && !tb)
+        hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(256), 0, x->s, A, lda, B, ldb, C, ldc, M, N, K, epi, bias, aux);
+    else if (!ta && tb)
+        hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(256), 0, x->s, A, lda, B, ldb, C, ldc, M, N, K, epi, bias, aux);
+    else
+        hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(256), 0, x->s, A, lda, B, ldb, C, ldc, M, N, K, epi, bias, aux);
+}
+
+int ensure_rows(g12_xp* x, int M) {
+    if (M <= x->cap_m) return 0;
+    hipStreamSynchronize(x->s);
+    float** bufs[] = {&x->X, &x->A1, &x->A2, &x->A3, &x->Y, &x->T, &x->dY, &x->D1, &x->D2, &x->D3};
+    for (auto b : bufs) {
+        if (*b) hipFree(*b);
+        *b = nullptr;
+    }
+    const int64_t din = x->fd.din, dout = x->fd.dout, H = x->H;
+    const int64_t sizes[] = {din, H, H, H, dout, dout, dout, H, H, H};
+    for (int i = 0; i < 10; ++i)
+        if (hipMalloc(bufs[i], sizeof(float) * sizes[i] * M) != hipSuccess) return G12_E_NOMEM;
+    x->cap_m = M;
+    return 0;
+}
+
+// forward pass on rows X[0..M): Y = A3 W4 + b4 + X Ws
+void forward(g12_xp* x, int M) {
+    const int din = x->fd.din, dout = x->fd.dout, H = x->H;
+    float* P = x->P;
+    gemm(x, false, false, x->X, din, P + x->oW1, H, x->A1, H, M, H, din, EPI_BIAS_TANH, P + x->ob1);
+    gemm(x, false, false, x->A1, H, P + x->oW2, H, x->A2, H, M, H, H, EPI_BIAS_TANH, P + x->ob2);
+    gemm(x, false, false, x->A2, H, P + x->oW3, H, x->A3, H, M, H, H, EPI_BIAS_TANH, P + x->ob3);
+    gemm(x, false, false, x->A3, H, P + x->oW4, dout, x->Y, dout, M, dout, H, EPI_BIAS, P + x->ob4);
+    gemm(x, false, false, x->X, din, P + x->oWs, dout, x->Y, dout, M, dout, din, EPI_ADD);
+}
+
+// loss and gradient (into G) for rows X, targets T
+void backward(g12_xp* x, int M) {
+    const int din = x->fd.din, dout = x->fd.dout, H = x->H;
+    float *P = x->P, *G = x->G;
+    const int nb = 64;
+    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, x->s, x->Y, x->T, (int64_t)M * dout, x->dY, x->partial);
+    // output layer and skip path
+    gemm(x, true, false, x->A3, H, x->dY, dout, G + x->oW4, dout, H, dout, M, EPI_STORE);
+    gemm(x, true, false, x->X, din, x->dY, dout, G + x->oWs, dout, din, dout, M, EPI_STORE);
+    hipLaunchKernelGGL(k_colsum, dim3(blocks(dout, 256)), dim3(256), 0, x->s, x->dY, M, dout, G + x->ob4);
+    // dZ3 = dY W4^T (1 - A3^2)
+    gemm(x, false, true, x->dY, dout, P + x->oW4, dout, x->D3, H, M, H, dout, EPI_DTANH, nullptr, x->A3);
+    gemm(x, true, false, x->A2, H, x->D3, H, G + x->oW3, H, H, H, M, EPI_STORE);
+    hipLaunchKernelGGL(k_colsum, dim3(blocks(H, 256)), dim3(256), 0, x->s, x->D3, M, H, G + x->ob3);
+    gemm(x, false, true, x->D3, H, P + x->oW3, H, x->D2, H, M, H, H, EPI_DTANH, nullptr, x->A2);
+    gemm(x, true, false, x->A1, H, x->D2, H, G + x->oW2, H, H, H, M, EPI_STORE);
+    hipLaunchKernelGGL(k_colsum, dim3(blocks(H, 256)), dim3(256), 0, x->s, x->D2, M, H, G + x->ob2);
+    gemm(x, false, true, x->D2, H, P + x->oW2, H, x->D1, H, M, H, H, EPI_DTANH, nullptr, x->A1);
+    gemm(x, true, false, x->X, din, x->D1, H, G + x->oW1, H, din, H, M, EPI_STORE);
+    hipLaunchKernelGGL(k_colsum, dim3(blocks(H, 256)), dim3(256), 0, x->s, x->D1, M, H, G + x->ob1);
+}
+
+double read_loss(g12_xp* x, int64_t count) {
+    float part[64];
+    if (hipMemcpyAsync(part, x->partial, sizeof part, hipMemcpyDeviceToHost, x->s) != hipSuccess ||
+        hipStreamSynchronize(x->s) != hipSuccess)
+        return NAN;
+    double sum = 0.0;
+    for (float v : part) sum += v;
+    return sum / (double)count;
+}
+
+// Glorot-uniform weights from a seeded splitmix64 stream, zero biases.
+void init_params(std::vector<float>& p, const g12_xp* x, uint64_t seed) {
+    uint64_t st = seed;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto fill = [&](int64_t off, int fin, int fout, double gain) {
+        const double a = gain * std::sqrt(6.0 / (fin + fout));
+        for (int64_t i = 0; i < (int64_t)fin * fout; ++i) p[off + i] = (float)((2.0 * next() - 1.0) * a);
+    };
+    const int din = x->fd.din, dout = x->fd.dout, H = x->H;
+    std::fill(p.begin(), p.end(), 0.f);
+    fill(x->oW1, din, H, 1.0);
+    fill(x->oW2, H, H, 1.0);
+    fill(x->oW3, H, H, 1.0);
+    fill(x->oW4, H, dout, 0.1);
+    fill(x->oWs, din, dout, 0.1);
+}
+
+}  // namespace
+
+extern "C" {
+
+int g12_xp_default_config(g12_xp_config* c) {
+    if (!c) return G12_E_ARG;
+    c->hist = 64;
+    c->horizon = 16;
+    c->hidden = 320;
+    c->dt = 0.01;
+    c->seed = 20261203ull;
+    return 0;
+}
+
+int g12_xp_create(const g12_xp_config* c, g12_xp** out) {
+    if (!c || !out) return G12_E_ARG;
+    if (c->hist < 2 || c->hist > 128 || c->horizon < 1 || c->horizon > 32 || c->hist + c->horizon > 160 ||
+        c->hidden < 16 || c->hidden > 1024 || !(c->dt > 0))
+        return G12_E_SIZE;
+    if (select_xp_device() < 0) return G12_E_HIP;
+    g12_xp* x = new (std::nothrow) g12_xp();
+    if (!x) return G12_E_NOMEM;
+    x->cfg = *c;
+    x->fd.hist = c->hist;
+    x->fd.horizon = c->horizon;
+    x->fd.din = 2 * c->hist + 2 * (c->hist + c->horizon);
+    x->fd.dout = 2 * c->horizon;
+    x->H = c->hidden;
+    const int64_t din = x->fd.din, dout = x->fd.dout, H = x->H;
+    int64_t o = 0;
+    x->oW1 = o; o += din * H;
+    x->ob1 = o; o += H;
+    x->oW2 = o; o += H * H;
+    x->ob2 = o; o += H;
+    x->oW3 = o; o += H * H;
+    x->ob3 = o; o += H;
+    x->oW4 = o; o += H * dout;
+    x->ob4 = o; o += dout;
+    x->oWs = o; o += din * dout;
+    x->np = o;
+    int rc = 0;
+    if (hipMalloc(&x->P, sizeof(float) * o) != hipSuccess || hipMalloc(&x->G, sizeof(float) * o) != hipSuccess ||
+        hipMalloc(&x->Mm, sizeof(float) * o) != hipSuccess || hipMalloc(&x->Vv, sizeof(float) * o) != hipSuccess ||
+        hipMalloc(&x->partial, sizeof(float) * 64) != hipSuccess)
+        rc = G12_E_NOMEM;
+    if (!rc && hipStreamCreateWithFlags(&x->s, hipStreamNonBlocking) != hipSuccess) rc = G12_E_HIP;
+    if (!rc) {
+        std::vector<float> p(o);
+        init_params(p, x, c->seed);
+        if (hipMemcpy(x->P, p.data(), sizeof(float) * o, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(x->Mm, 0, sizeof(float) * o) != hipSuccess || hipMemset(x->Vv, 0, sizeof(float) * o) != hipSuccess)
+            rc = G12_E_HIP;
+    }
+    if (rc) {
+        g12_xp_destroy(x);
+        return rc;
+    }
+    *out = x;
+    return 0;
+}
+
+int g12_xp_destroy(g12_xp* x) {
+    if (!x) return 0;
+    if (x->s) hipStreamSynchronize(x->s);
+    void* bufs[] = {x->P, x->G, x->Mm, x->Vv, x->X, x->A1, x->A2, x->A3, x->Y, x->T, x->dY, x->D1, x->D2, x->D3,
+                    x->partial, x->series, x->coef, x->drive, x->ring, x->rout, x->ref, x->scale, x->w, x->chk};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    if (x->s) hipStreamDestroy(x->s);
+    delete x;
+    return 0;
+}
+
+int g12_xp_param_count(g12_xp* x, int64_t* n) {
+    if (!x || !n) return G12_E_ARG;
+    *n = x->np;
+    return 0;
+}
+
+int g12_xp_get_params(g12_xp* x, float* p) {
+    if (!x || !p) return G12_E_ARG;
+    XCHK(hipMemcpyAsync(p, x->P, sizeof(float) * x->np, hipMemcpyDeviceToHost, x->s));
+    XCHK(hipStreamSynchronize(x->s));
+    return 0;
+}
+
+int g12_xp_set_params(g12_xp* x, const float* p) {
+    if (!x || !p) return G12_E_ARG;
+    XCHK(hipMemcpyAsync(x->P, p, sizeof(float) * x->np, hipMemcpyHostToDevice, x->s));
+    XCHK(hipMemsetAsync(x->Mm, 0, sizeof(float) * x->np, x->s));
+    XCHK(hipMemsetAsync(x->Vv, 0, sizeof(float) * x->np, x->s));
+    XCHK(hipStreamSynchronize(x->s));
+    x->adam_t = 0;
+    return 0;
+}
+
+// File: magic "G12XP1\0\0", hist, horizon, hidden (int32), dt (double), count
+// (int64), FP32 parameters.
+int g12_xp_save(g12_xp* x, const char* path) {
+    if (!x || !path) return G12_E_ARG;
+    std::vector<float> p(x->np);
+    int rc = g12_xp_get_params(x, p.data());
+    if (rc) return rc;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return G12_E_IO;
+    const char magic[8] = {'G', '1', '2', 'X', 'P', '1', 0, 0};
+    const int32_t dims[3] = {x->cfg.hist, x->cfg.horizon, x->cfg.hidden};
+    bool ok = std::fwrite(magic, 8, 1, f) == 1 && std::fwrite(dims, sizeof dims, 1, f) == 1 &&
+              std::fwrite(&x->cfg.dt, sizeof(double), 1, f) == 1 && std::fwrite(&x->np, sizeof(int64_t), 1, f) == 1 &&
+              std::fwrite(p.data(), sizeof(float), p.size(), f) == p.size();
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? 0 : G12_E_IO;
+}
+
+int g12_xp_load(g12_xp* x, const char* path) {
+    if (!x || !path) return G12_E_ARG;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return G12_E_IO;
+    char magic[8];
+    int32_t dims[3];
+    double dt = 0;
+    int64_t n = 0;
+    bool ok = std::fread(magic, 8, 1, f) == 1 && std::memcmp(magic, "G12XP1", 6) == 0 &&
+              std::fread(dims, sizeof dims, 1, f) == 1 && std::fread(&dt, sizeof dt, 1, f) == 1 &&
+              std::fread(&n, sizeof n, 1, f) == 1;
+    if (ok && (dims[0] != x->cfg.hist || dims[1] != x->cfg.horizon || dims[2] != x->cfg.hidden || dt != x->cfg.dt ||
+               n != x->np)) {
+        std::fclose(f);
+        return G12_E_SIZE;
+    }
+    std::vector<float> p(ok ? n : 0);
+    ok = ok && std::fread(p.data(), sizeof(float), p.size(), f) == p.size();
+    std::fclose(f);
+    if (!ok) return G12_E_IO;
+    return g12_xp_set_params(x, p.data());
+}
+
+int g12_xp_set_data(g12_xp* x, const float* series, const float* coef, const float* drive, int64_t n_series, int T) {
+    if (!x || !series || !coef || !drive || n_series < 1 || T < x->fd.hist + x->fd.horizon) return G12_E_ARG;
+    XCHK(hipStreamSynchronize(x->s));
+    if (x->series) hipFree(x->series);
+    if (x->coef) hipFree(x->coef);
+    if (x->drive) hipFree(x->drive);
+    x->series = nullptr;
+    x->coef = x->drive = nullptr;
+    if (hipMalloc(&x->series, sizeof(float2) * n_series * T) != hipSuccess ||
+        hipMalloc(&x->coef, sizeof(float) * n_series) != hipSuccess || hipMalloc(&x->drive, sizeof(float) * T) != hipSuccess)
+        return G12_E_NOMEM;
+    XCHK(hipMemcpyAsync(x->series, series, sizeof(float2) * n_series * T, hipMemcpyHostToDevice, x->s));
+    XCHK(hipMemcpyAsync(x->coef, coef, sizeof(float) * n_series, hipMemcpyHostToDevice, x->s));
+    XCHK(hipMemcpyAsync(x->drive, drive, sizeof(float) * T, hipMemcpyHostToDevice, x->s));
+    XCHK(hipStreamSynchronize(x->s));
+    x->n_series = n_series;
+    x->T_len = T;
+    return 0;
+}
+
+// `steps` Adam steps on minibatches of `batch` windows drawn on the device;
+// loss[i] (if given) receives the minibatch loss of step i.
+int g12_xp_train(g12_xp* x, int steps, int batch, double lr, float* loss) {
+    if (!x || steps < 0 || batch < 1 || !x->series) return G12_E_ARG;
+    int rc = ensure_rows(x, batch);
+    if (rc) return rc;
+    const float b1 = 0.9f, b2 = 0.999f;
+    for (int i = 0; i < steps; ++i) {
+        hipLaunchKernelGGL(k_gather, dim3(blocks(batch, 64)), dim3(64), 0, x->s, x->fd, x->cfg.dt, x->series, x->coef,
+                           x->drive, x->n_series, x->T_len, (uint64_t)x->cfg.seed, (int64_t)x->adam_t, batch, x->X,
+                           x->T);
+        forward(x, batch);
+        backward(x, batch);
+        x->adam_t += 1;
+        const float c1 = 1.f - std::pow(b1, (float)x->adam_t), c2 = 1.f - std::pow(b2, (float)x->adam_t);
+        hipLaunchKernelGGL(k_adam, dim3(256), dim3(256), 0, x->s, x->P, x->G, x->Mm, x->Vv, x->np, (float)lr, b1, b2,
+                           1e-8f, c1, c2);
+        if (loss) loss[i] = (float)read_loss(x, (int64_t)batch * x->fd.dout);
+    }
+    XCHK(hipStreamSynchronize(x->s));
+    XCHK(hipGetLastError());
+    return 0;
+}
+
+// Loss (averaged over M x dout outputs, as in training) and its gradient with respect
+// to every parameter (layout of g12_xp_get_params) for given features and targets.
+int g12_xp_loss_grad(g12_xp* x, const float* X, const float* T, int M, double* loss, float* grad) {
+    if (!x || !X || !T || M < 1) return G12_E_ARG;
+    int rc = ensure_rows(x, M);
+    if (rc) return rc;
+    XCHK(hipMemcpyAsync(x->X, X, sizeof(float) * x->fd.din * M, hipMemcpyHostToDevice, x->s));
+    XCHK(hipMemcpyAsync(x->T, T, sizeof(float) * x->fd.dout * M, hipMemcpyHostToDevice, x->s));
+    forward(x, M);
+    backward(x, M);
+    const double l = read_loss(x, (int64_t)M * x->fd.dout);
+    if (loss) *loss = l;
+    if (grad) XCHK(hipMemcpyAsync(grad, x->G, sizeof(float) * x->np, hipMemcpyDeviceToHost, x->s));
+    XCHK(hipStreamSynchronize(x->s));
+    return 0;
+}
+
+int g12_xp_forward(g12_xp* x, const float* X, int M, float* Y) {
+    if (!x || !X || !Y || M < 1) return G12_E_ARG;
+    int rc = ensure_rows(x, M);
+    if (rc) return rc;
+    XCHK(hipMemcpyAsync(x->X, X, sizeof(float) * x->fd.din * M, hipMemcpyHostToDevice, x->s));
+    forward(x, M);
+    XCHK(hipMemcpyAsync(Y, x->Y, sizeof(float) * x->fd.dout * M, hipMemcpyDeviceToHost, x->s));
+    XCHK(hipStreamSynchronize(x->s));
+    return 0;
+}
+
+// Rollout of `nblocks` blocks of `horizon` steps from `hist` G1 matrices
+// (N x N complex, oldest first).  w holds the quench shift w(t) for the hist
+// steps and every predicted step (hist + nblocks * horizon values); sites
+// s < Nq (site = index % Ns) carry it.  out receives nblocks * horizon
+// matrices; checks[2 b], checks[2 b + 1] the largest trace drift (against
+// tr_ref, the conserved trace as (re, im), or the last history matrix when
+// tr_ref is null) and anti-hermiticity error |G + G^dagger| of block b, both
+// of the network's own output.  With `project`, each block is then projected
+// (Re G_ii = 0, Im tr G = Im tr_ref) before it is fed back, so the checks
+// measure the drift a single block produces.
+int g12_xp_rollout(g12_xp* x, const double* hist, int N, int Ns, int Nq, const double* w, const double* tr_ref,
+                   int project, int nblocks, double* out, double* checks) {
+    if (!x || !hist || !w || !out || N < 1 || N > 64 || Ns < 1 || nblocks < 1) return G12_E_ARG;
+    const int H = x->fd.hist, F = x->fd.horizon;
+    const int n_el = N * (N + 1) / 2;
+    int rc = ensure_rows(x, n_el);
+    if (rc) return rc;
+    const int cap = H + F;
+    if (x->ring_N != N || x->ring_cap != cap) {
+        XCHK(hipStreamSynchronize(x->s));
+        void* bufs[] = {x->ring, x->rout, x->ref, x->scale, x->w, x->chk};
+        for (void* b : bufs)
+            if (b) hipFree(b);
+        if (hipMalloc(&x->ring, sizeof(double2) * cap * N * N) != hipSuccess ||
+            hipMalloc(&x->rout, sizeof(double2) * F * N * N) != hipSuccess ||
+            hipMalloc(&x->ref, sizeof(double2) * n_el) != hipSuccess || hipMalloc(&x->scale, sizeof(double) * n_el) != hipSuccess ||
+            hipMalloc(&x->w, sizeof(double) * 4096) != hipSuccess || hipMalloc(&x->chk, sizeof(double) * 2) != hipSuccess) {
+            x->ring_N = 0;
+            return G12_E_NOMEM;
+        }
+        x->ring_N = N;
+        x->ring_cap = cap;
+    }
+    const int64_t n2 = (int64_t)N * N;
+    XCHK(hipMemcpyAsync(x->ring, hist, sizeof(double2) * H * n2, hipMemcpyHostToDevice, x->s));
+    double2 tr_last = make_double2(0.0, 0.0);
+    for (int i = 0; i < N; ++i) {
+        tr_last.x += hist[2 * ((H - 1) * n2 + i * N + i)];
+        tr_last.y += hist[2 * ((H - 1) * n2 + i * N + i) + 1];
+    }
+    // conserved trace: checks compare against it; with projection every block
+    // starts from a matrix that has it, so a check sees that block's own drift
+    const double2 tr0 = tr_ref ? make_double2(tr_ref[0], tr_ref[1]) : tr_last;
+    int first = 0;  // ring slot of the oldest history step
+    for (int b = 0; b < nblocks; ++b) {
+        XCHK(hipMemcpyAsync(x->w, w + (int64_t)b * F, sizeof(double) * (H + F), hipMemcpyHostToDevice, x->s));
+        hipLaunchKernelGGL(k_roll_in, dim3(blocks(n_el, 64)), dim3(64), 0, x->s, x->fd, x->cfg.dt, x->ring, cap, first,
+                           N, Ns, Nq, x->w, x->X, x->ref, x->scale);
+        forward(x, n_el);
+        hipLaunchKernelGGL(k_roll_out, dim3(blocks(n_el, 64)), dim3(64), 0, x->s, x->fd, x->Y, x->ref, x->scale,
+                           x->ring, cap, first, N, x->rout);
+        hipLaunchKernelGGL(k_roll_check, dim3(1), dim3(256), 0, x->s, x->ring, cap, first, H, F, N, tr0, x->chk);
+        if (project)
+            hipLaunchKernelGGL(k_roll_project, dim3(F), dim3(64), 0, x->s, x->ring, cap, first, H, F, N, tr0,
+                               x->rout);
+        XCHK(hipMemcpyAsync(out + (int64_t)b * F * n2 * 2, x->rout, sizeof(double2) * F * n2, hipMemcpyDeviceToHost,
+                            x->s));
+        if (checks) XCHK(hipMemcpyAsync(checks + 2 * b, x->chk, sizeof(double) * 2, hipMemcpyDeviceToHost, x->s));
+        XCHK(hipStreamSynchronize(x->s));  // w and the pageable copies
+        first = (first + F) % cap;
+    }
+    XCHK(hipGetLastError());
+    return 0;
+}
+
+// Features of the rollout input for the last `hist` matrices (tests): X
+// (n_el x din, upper-triangle order), ref and scale per element.
+int g12_xp_features(g12_xp* x, const double* hist, int N, int Ns, int Nq, const double* w, float* X, double* ref,
+                    double* scale) {
+    if (!x || !hist || !w || !X || N < 1 || N > 64 || Ns < 1) return G12_E_ARG;
+    const int H = x->fd.hist, F = x->fd.horizon, n_el = N * (N + 1) / 2;
+    int rc = ensure_rows(x, n_el);
+    if (rc) return rc;
+    double2 *ring = nullptr, *dref = nullptr;
+    double *dsc = nullptr, *dw = nullptr;
+    const int64_t n2 = (int64_t)N * N;
+    if (hipMalloc(&ring, sizeof(double2) * H * n2) != hipSuccess || hipMalloc(&dref, sizeof(double2) * n_el) != hipSuccess ||
+        hipMalloc(&dsc, sizeof(double) * n_el) != hipSuccess || hipMalloc(&dw, sizeof(double) * (H + F)) != hipSuccess)
+        rc = G12_E_NOMEM;
+    if (!rc) {
+        hipMemcpyAsync(ring, hist, sizeof(double2) * H * n2, hipMemcpyHostToDevice, x->s);
+        hipMemcpyAsync(dw, w, sizeof(double) * (H + F), hipMemcpyHostToDevice, x->s);
+        hipLaunchKernelGGL(k_roll_in, dim3(blocks(n_el, 64)), dim3(64), 0, x->s, x->fd, x->cfg.dt, ring, H, 0, N, Ns,
+                           Nq, dw, x->X, dref, dsc);
+        hipMemcpyAsync(X, x->X, sizeof(float) * x->fd.din * n_el, hipMemcpyDeviceToHost, x->s);
+        if (ref) hipMemcpyAsync(ref, dref, sizeof(double2) * n_el, hipMemcpyDeviceToHost, x->s);
+        if (scale) hipMemcpyAsync(scale, dsc, sizeof(double) * n_el, hipMemcpyDeviceToHost, x->s);
+        if (hipStreamSynchronize(x->s) != hipSuccess) rc = G12_E_HIP;
+    }
+    if (ring) hipFree(ring);
+    if (dref) hipFree(dref);
+    if (dsc) hipFree(dsc);
+    if (dw) hipFree(dw);
+    return rc;
+}
+
+}  // extern "C"

@@ -128,6 +128,13 @@ int g12_rk4(g12_ctx* ctx, double* traj, int64_t traj_cap, int64_t* steps);
  * time without recording (benchmarks, checkpoints).  Same guard semantics. */
 int g12_rk4_advance(g12_ctx* ctx, int64_t n, int64_t* steps_done);
 
+/* Advance n production steps from the context's current time, writing the G1
+ * matrix before each step to traj (capacity traj_cap matrices, may be NULL).
+ * Same guard semantics as g12_rk4.  No reference counterpart: the reference's
+ * RK4 always starts at t = 0; this resumes a run (checkpoints, extrapolation
+ * fallback). */
+int g12_rk4_record(g12_ctx* ctx, int64_t n, double* traj, int64_t traj_cap, int64_t* steps_done);
+
 /* Current production time and step count of the context. */
 int g12_get_time(g12_ctx* ctx, double* t, int64_t* steps);
 
@@ -180,12 +187,60 @@ int g12_batch_get_state(g12_batch* b, int i, double* G1, double* G2xx, double* G
 int g12_batch_adiabatic_switching(g12_batch* b, int* steps);
 int g12_batch_rk4(g12_batch* b, double* const* traj, int64_t traj_cap, int64_t* steps);
 int g12_batch_rk4_advance(g12_batch* b, int64_t n);
+/* n more production steps from the batch's current time; traj[i] (or NULL)
+ * receives trajectory i's G1 before each of them (capacity traj_cap). */
+int g12_batch_rk4_record(g12_batch* b, int64_t n, double* const* traj, int64_t traj_cap);
 int g12_batch_guard(g12_batch* b, int* fail);
 int g12_batch_timing_enable(g12_batch* b, int on);
 int g12_batch_timing_get(g12_batch* b, int which, int64_t* launches, double* total_ms);
 int g12_batch_configure(g12_batch* b, int use_graphs, int autotune);
 int g12_batch_tune(g12_batch* b);
 int g12_batch_buckets(g12_batch* b, int cap, int* NP, int* count, int* waves_strided, int* waves_fused);
+
+/* ---- learned G1 extrapolator ------------------------------------------------
+ * No reference counterpart (the reference's README names extrapolation of the
+ * dynamics as a direction; nothing in its source implements one).  A temporal
+ * MLP applied to every upper-triangle element of G1: from `hist` steps of the
+ * element and the quench drive it predicts the next `horizon` steps.  Four
+ * layers (tanh, `hidden` wide) plus a linear skip path; BF16 MFMA products
+ * with FP32 accumulation, FP32 master weights, Adam.  Features and the exact
+ * network are described in g12kba_amd/csrc/extrap.hip.
+ *
+ * Training data: `n_series` complex64 series of length T (interleaved re/im),
+ * the drive coefficient of each series (quench strength x (m_p - m_q)) and the
+ * shared drive profile (T values); minibatches are drawn on the device.
+ * g12_xp_loss_grad / g12_xp_forward / g12_xp_features expose one pass on given
+ * data (gradient and feature checks).  g12_xp_rollout predicts nblocks blocks
+ * from hist matrices, feeding predictions back, and reports per block the
+ * largest trace drift (against tr_ref, the conserved trace, or the last history
+ * matrix's when NULL) and anti-hermiticity error of the network's output;
+ * project = 1 then restores Re G_ii = 0 and the conserved Im tr G in each
+ * block before feeding it back. */
+typedef struct g12_xp g12_xp;
+typedef struct g12_xp_config {
+  int hist;        /* history steps (64) */
+  int horizon;     /* predicted steps per block (16) */
+  int hidden;      /* hidden width (320) */
+  double dt;       /* time step of the trajectories (0.01) */
+  uint64_t seed;   /* weight initialisation and minibatch draws */
+} g12_xp_config;
+
+int g12_xp_default_config(g12_xp_config* c);
+int g12_xp_create(const g12_xp_config* c, g12_xp** out);
+int g12_xp_destroy(g12_xp* x);
+int g12_xp_param_count(g12_xp* x, int64_t* n);
+int g12_xp_get_params(g12_xp* x, float* params);
+int g12_xp_set_params(g12_xp* x, const float* params);
+int g12_xp_save(g12_xp* x, const char* path);
+int g12_xp_load(g12_xp* x, const char* path);
+int g12_xp_set_data(g12_xp* x, const float* series, const float* coef, const float* drive, int64_t n_series, int T);
+int g12_xp_train(g12_xp* x, int steps, int batch, double lr, float* loss);
+int g12_xp_loss_grad(g12_xp* x, const float* X, const float* T, int M, double* loss, float* grad);  /* loss: r^2 up to |r| = 1, 2|r| - 1 beyond, averaged */
+int g12_xp_forward(g12_xp* x, const float* X, int M, float* Y);
+int g12_xp_features(g12_xp* x, const double* hist, int N, int Ns, int Nq, const double* w, float* X, double* ref,
+                    double* scale);
+int g12_xp_rollout(g12_xp* x, const double* hist, int N, int Ns, int Nq, const double* w, const double* tr_ref,
+                   int project, int nblocks, double* out, double* checks);
 
 /* ---- routine-level entry points ------------------------------------------
  * One entry per reference routine, same argument roles and order, host
