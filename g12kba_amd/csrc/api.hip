@@ -107,6 +107,7 @@ StageShared stage_block(int k, double dt_full) {
     ss.c = half ? dt_full / 2 : dt_full;
     ss.wk = half ? 2.0 : 1.0;
     ss.dt = dt_full;
+    ss.cn = (k == 0 || k == 1) ? dt_full / 2 : dt_full;  // input of stage k + 1 (unused at k = 3)
     ss.first = k == 0;
     ss.last = k == 3;
     ss.lam = 1.0;
@@ -237,7 +238,7 @@ int64_t layout(Run& r, char* base) {
         d.Wxx = cv.take<double>(d.n2);
         for (int s = 0; s < 2; ++s) {
             d.S2[s] = cv.take<double2>(d.n4);  // the state is kept even when frozen
-            d.K2[s] = cv.take<double2>(n4);
+            d.Y2[s] = cv.take<double2>(n4);
             d.A2[s] = cv.take<double2>(n4);
             d.R2[s] = cv.take<double2>(n4);
             d.D[s] = cv.take<double2>(n3);
@@ -343,8 +344,8 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
         hipHostMalloc((void**)&r.h_fail, sizeof(int) * ntraj, hipHostMallocDefault) != hipSuccess)
         return G12_E_NOMEM;
     HIPCHK(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
-    // every buffer starts defined (zero): the first stage reads K2/A2/R2 and
-    // the autotuner runs passes before any stage has written the tables
+    // every buffer starts defined (zero): the autotuner runs passes before any
+    // stage has written the tables, R2 or Y2
     HIPCHK(hipMemsetAsync(r.arena, 0, total, r.stream));
     HIPCHK(hipMemcpyAsync(r.d_ds, r.ds.data(), sizeof(DevState) * ntraj, hipMemcpyHostToDevice, r.stream));
     HIPCHK(hipMemcpyAsync(r.d_tp, r.tp.data(), sizeof(TrajParams) * ntraj, hipMemcpyHostToDevice, r.stream));
@@ -382,14 +383,15 @@ void timed(Run& r, int cls, F&& launch) {
 
 // One RK4 stage of every trajectory (RK_solver.h:120-203 / AS.h:111-200):
 // G1 stage set-up, strided commutator pass (+ D), Q/F tables and collision
-// integral, G1 right-hand side and update, fused commutator + G2 right-hand
-// side and update.
+// integral, G1 right-hand side and update, G2 sources (Psi, Pi, Pi^+) into the
+// strided-pass result, fused commutator + G2 update.
 void run_stage(Run& r, const StageRef& sr) {
     hipStream_t s = r.stream;
     timed(r, 2, [&] { g12::launch_b_stage_g1(r.b, sr, s); });
     for (const Bucket& k : r.buckets) timed(r, 0, [&] { g12::launch_b_tile(r.b, k, sr, 0, r.precision, s); });
     if (r.any_corr) timed(r, 2, [&] { g12::launch_b_tables(r.b, sr, s); });
     timed(r, 2, [&] { g12::launch_b_g1_rhs(r.b, sr, s); });
+    for (const Bucket& k : r.buckets) timed(r, 2, [&] { g12::launch_b_sources(r.b, k, sr, s); });
     for (const Bucket& k : r.buckets) timed(r, 1, [&] { g12::launch_b_tile(r.b, k, sr, 1, r.precision, s); });
 }
 
@@ -433,7 +435,7 @@ void drop_graphs(Run& r) {
 
 // Wave counts per bucket and pass, chosen by timing the compiled candidates on
 // the run's own tile lists.  Both passes are run as stage 0 of a step (the
-// strided pass writes only R2 and D, the fused pass only K2 and A2, all of
+// strided pass writes only R2 and D, the fused pass only Y2 and A2, all of
 // which the next stage 0 overwrites before reading), so tuning at a step
 // boundary leaves the state unchanged.  Results are cached per process by
 // (tile size, precision, tile count).

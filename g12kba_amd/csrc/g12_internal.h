@@ -42,6 +42,7 @@ struct TrajParams {
 // Per-stage scalars shared by every trajectory of a batch.
 struct StageShared {
     double c, wk, dt;
+    double cn;         // next stage's input coefficient: Y = S + cn K (stages 0-2)
     int first, last;
     double lam;        // adiabatic switching: U_eff = lam * U (lam = fermi(t)); production: U_eff = U
     double phi;        // quench ramp: shift = phi * qstr on sites < Nq when gate = 1
@@ -68,7 +69,7 @@ struct DevState {
     int64_t n2, n3, n4;
     double2 *S1, *K1, *A1, *Y1, *h, *I;
     double *Wxy, *Wxx;                 // N x N interaction tables (xy, xx)
-    double2 *S2[2], *K2[2], *A2[2], *R2[2];
+    double2 *S2[2], *Y2[2], *A2[2], *R2[2];  // state, next stage input, RK accumulator, strided-pass result
     double2 *D[2];                     // D[X][Y][Z] = T[X][Z][Y][Z]
     double2 *F[2], *Ft[2];             // F[X][Y][Z], Ft[Z][Y][X]
     double2 *Q[2], *Qp[2];             // Q[Z][X][Y], Qp[X][Y][Z]
@@ -104,6 +105,7 @@ void launch_b_tables(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_record(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int fused, int precision, hipStream_t s);
+void launch_b_sources(const BatchDev& b, const Bucket& k, const StageRef& sr, hipStream_t s);
 void launch_b_advance_counter(int* counter, hipStream_t s);
 int device_cu_count();
 int default_waves(int NP);

@@ -13,9 +13,10 @@
 //                   -> the terms that act on p1 and q1;
 //   contiguous tiles X[p2][q2] = T[p1][p2][q1][q2] at fixed (p1,q1)
 //                   -> the terms that act on p2 and q2.
-// The contiguous pass also evaluates the rest of the G2 right-hand side
-// (Psi, Pi, Pi-dagger, reference G1_G2_t_step.h:55-86) and the RK4 update for
-// its tile, so Psi and Pi are never stored.
+// Between the two passes a streaming kernel adds the rest of the G2
+// right-hand side (Psi, Pi, Pi-dagger, reference G1_G2_t_step.h:55-86) into
+// the strided-pass result, so Psi and Pi are never stored on their own; the
+// contiguous pass then finishes K and applies the RK4 update for its tile.
 #include "g12_internal.h"
 
 #include <cmath>
@@ -39,6 +40,24 @@ __device__ __forceinline__ double2 cmul_i(double2 a) { return make_double2(-a.y,
 __device__ __forceinline__ double2 cmul_mi(double2 a) { return make_double2(a.y, -a.x); }
 
 __device__ __forceinline__ bool failed(const int* fail) { return fail && *fail != 0; }
+
+// Explicit global-address-space accesses: pointers read from the trajectory
+// descriptors are generic, and generic (flat) memory operations are also
+// counted by the LDS wait counter, so in the tile kernels every LDS wait of the
+// MFMA loop would wait for the next tile's prefetch and every LDS barrier for
+// the previous tile's stores.
+#define GLOBAL __attribute__((address_space(1)))
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double2 gload(const double2* p, int64_t i) {
+    const dv2 v = ((const GLOBAL dv2*)p)[i];
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void gstore(double2* p, int64_t i, double2 v) {
+    ((GLOBAL dv2*)p)[i] = dv2{v.x, v.y};
+}
+__device__ __forceinline__ double gload1(const double* p, int64_t i) { return ((const GLOBAL double*)p)[i]; }
+
 
 // Workgroup barrier that orders LDS only: the tile loop reuses its LDS buffers
 // but never exchanges global memory between waves, so waiting for the
@@ -142,21 +161,26 @@ __device__ inline StageShared stage_of(const StageRef& r) {
     return r.val;
 }
 
+// Stage parameters of trajectory t, filled by the threads of a block
+// together (thread i writes shell i).  Callers synchronise before use.
 __device__ inline void fill_sp(StageParams& sp, const TrajParams& t, const StageShared& ss) {
-    sp.c = ss.c;
-    sp.wk = ss.wk;
-    sp.dt = ss.dt;
-    sp.first = ss.first;
-    sp.last = ss.last;
-    sp.corr = t.corr;
-    sp.U = ss.as_phase ? ss.lam * t.U : t.U;  // U_AS = fermi(t) U during switching
-    sp.nV = t.nV;
-    for (int i = 0; i < t.nV; ++i) sp.V[i] = sp.U * t.Vprof[i];
-    sp.wq = ss.gate ? ss.phi * t.qstr : 0.0;
-    sp.Nq = ss.gate ? t.Nq : 0;
-    sp.t1 = t.t1;
-    sp.t2 = t.t2;
-    for (int i = 0; i < G12_MAX_NB; ++i) sp.eps[i] = t.eps[i];
+    const double U = ss.as_phase ? ss.lam * t.U : t.U;  // U_AS = fermi(t) U during switching
+    for (int i = threadIdx.x; i < t.nV; i += blockDim.x) sp.V[i] = U * t.Vprof[i];
+    if (threadIdx.x == 0) {
+        sp.c = ss.c;
+        sp.wk = ss.wk;
+        sp.dt = ss.dt;
+        sp.first = ss.first;
+        sp.last = ss.last;
+        sp.corr = t.corr;
+        sp.U = U;
+        sp.nV = t.nV;
+        sp.wq = ss.gate ? ss.phi * t.qstr : 0.0;
+        sp.Nq = ss.gate ? t.Nq : 0;
+        sp.t1 = t.t1;
+        sp.t2 = t.t2;
+    }
+    if (threadIdx.x < G12_MAX_NB) sp.eps[threadIdx.x] = t.eps[threadIdx.x];
 }
 
 // A trajectory takes part in a stage unless its guard tripped or its own
@@ -177,7 +201,7 @@ __global__ __launch_bounds__(1024) void k_stage_g1(BatchDev b, StageRef r) {
     // built once per block in LDS: the V table is indexed at run time and
     // would otherwise live in per-lane scratch
     __shared__ StageParams sp;
-    if (threadIdx.x == 0) fill_sp(sp, tp, ss);
+    fill_sp(sp, tp, ss);
     __shared__ double2 diag[G12_MAX_NB * 64];
     __syncthreads();
     const int N = d.N;
@@ -285,8 +309,9 @@ __global__ __launch_bounds__(64) void k_f_tables(BatchDev b, StageRef r) {
 
 // G1 right-hand side: comm = [h, Y1], K = -i(comm + I + I^dag), RK
 // accumulation (G1_step, G1_G2_t_step.h:24-47) and the final update.  One
-// block per row X; the rows X of h and Y1 are staged in LDS.
-__global__ __launch_bounds__(64) void k_g1_rhs(BatchDev b, StageRef r) {
+// block per row X: the rows X of h and Y1 are staged in LDS, and four groups
+// of 64 threads each sum a quarter of the z range for the columns Y they own.
+__global__ __launch_bounds__(256) void k_g1_rhs(BatchDev b, StageRef r) {
     const int ti = blockIdx.y;
     const DevState& d = b.ds[ti];
     const TrajParams& tp = b.tp[ti];
@@ -296,32 +321,46 @@ __global__ __launch_bounds__(64) void k_g1_rhs(BatchDev b, StageRef r) {
     const int X = blockIdx.x;
     if (X >= N) return;
     __shared__ double2 hrow[G12_MAX_NB * 64], grow[G12_MAX_NB * 64];
+    __shared__ double2 part[4][64];
+    const double2* h = d.h;
+    const double2* Y1 = d.Y1;
     for (int z = threadIdx.x; z < N; z += blockDim.x) {
-        hrow[z] = d.h[(int64_t)X * N + z];
-        grow[z] = d.Y1[(int64_t)X * N + z];
+        hrow[z] = gload(h, (int64_t)X * N + z);
+        grow[z] = gload(Y1, (int64_t)X * N + z);
     }
     __syncthreads();
-    for (int Y = threadIdx.x; Y < N; Y += blockDim.x) {
-        const int64_t e = (int64_t)X * N + Y;
-        double2 hg = make_double2(0.0, 0.0), gh = make_double2(0.0, 0.0);
-        for (int z = 0; z < N; ++z) {
-            hg = cadd(hg, cmul(hrow[z], d.Y1[(int64_t)z * N + Y]));
-            gh = cadd(gh, cmul(grow[z], d.h[(int64_t)z * N + Y]));
+    const int grp = threadIdx.x >> 6, col = threadIdx.x & 63;
+    const int zq = (N + 3) / 4, z0 = grp * zq, z1 = min(N, z0 + zq);
+    for (int Y0 = 0; Y0 < N; Y0 += 64) {
+        const int Y = Y0 + col;
+        double2 acc = make_double2(0.0, 0.0);
+        if (Y < N) {
+#pragma unroll 4
+            for (int z = z0; z < z1; ++z) {
+                acc = cadd(acc, cmul(hrow[z], gload(Y1, (int64_t)z * N + Y)));
+                acc = csub(acc, cmul(grow[z], gload(h, (int64_t)z * N + Y)));
+            }
         }
-        double2 v = csub(hg, gh);
-        if (tp.corr) v = cadd(cadd(v, d.I[e]), cconj(d.I[(int64_t)Y * N + X]));
-        const double2 K = cmul_mi(v);
-        const double2 wK = cscale(K, ss.wk);
-        if (ss.first) {
-            d.A1[e] = wK;
-            d.K1[e] = K;
-        } else if (!ss.last) {
-            d.A1[e] = cadd(d.A1[e], wK);
-            d.K1[e] = K;
-        } else {
-            const double2 A = cadd(d.A1[e], wK);
-            d.S1[e] = cadd(d.S1[e], cdiv6(cscale(A, ss.dt)));
+        part[grp][col] = acc;
+        __syncthreads();
+        if (grp == 0 && Y < N) {
+            const int64_t e = (int64_t)X * N + Y;
+            double2 v = cadd(cadd(part[0][col], part[1][col]), cadd(part[2][col], part[3][col]));
+            if (tp.corr) v = cadd(cadd(v, d.I[e]), cconj(d.I[(int64_t)Y * N + X]));
+            const double2 K = cmul_mi(v);
+            const double2 wK = cscale(K, ss.wk);
+            if (ss.first) {
+                d.A1[e] = wK;
+                d.K1[e] = K;
+            } else if (!ss.last) {
+                d.A1[e] = cadd(d.A1[e], wK);
+                d.K1[e] = K;
+            } else {
+                const double2 A = cadd(d.A1[e], wK);
+                d.S1[e] = cadd(d.S1[e], cdiv6(cscale(A, ss.dt)));
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -695,7 +734,7 @@ struct BTileArgs {
     int hmode;        // 1: +h.X, 2: -X.h, 3: both
     int accumulate;   // unfused: R += O instead of R = O
     int nspin;
-    int use_kp;       // stage input Y = S + c K_prev (0: Y = S)
+    int use_y;        // stage input from the Y buffer after stage 0 (0: always S)
     int extract_d;    // unfused strided pass: D from the diagonal tiles
     const double2* h_override;  // routine calls: h to use instead of the trajectory's h
 };
@@ -749,6 +788,7 @@ __device__ inline StageShared uni_stage(const StageRef& r) {
     const StageShared v = stage_of(r);
     StageShared u;
     u.c = uni_d(v.c);
+    u.cn = uni_d(v.cn);
     u.wk = uni_d(v.wk);
     u.dt = uni_d(v.dt);
     u.first = uni(v.first);
@@ -804,33 +844,28 @@ __device__ inline TileRef ref_of(const Cursor& c, int64_t gt) {
 // only when that pair changes.
 struct View {
     int traj = -1, spin = -1;
-    const double2* S;   // stage state
-    const double2* Kp;  // previous stage derivative (null: Y = S)
+    const double2* X;   // stage input: S at stage 0, else Y = S + c K_prev
+    double2* S;         // stage state (fused: read for Y, updated at the last stage)
     double2* R;         // unfused output / strided-pass result
     double2* D;         // D extraction (unfused strided pass)
-    const double2* G1;  // fused: stage G1
-    const double* W;    // fused: W_xx (spin 0) or W_xy (spin 1)
-    const double2 *Q, *Qp, *F, *Ft;
-    double2 *A, *K;
+    double2 *A, *Y;     // fused: RK accumulator, next stage input
 };
+
+__device__ inline const double2* stage_input(const DevState& d, const BTileArgs& a, const StageShared& ss, int spin) {
+    return (a.use_y && !ss.first) ? d.Y2[spin] : d.S2[spin];
+}
 
 template <bool FUSED>
 __device__ inline void load_view(View& v, const BTileArgs& a, const StageShared& ss, int traj, int spin) {
     const DevState& d = a.b.ds[traj];
     v.traj = traj;
     v.spin = spin;
+    v.X = uni_ptr(stage_input(d, a, ss, spin));
     v.S = uni_ptr(d.S2[spin]);
-    v.Kp = (a.use_kp && !ss.first) ? uni_ptr(d.K2[spin]) : nullptr;
     v.R = uni_ptr(d.R2[spin]);
     if constexpr (FUSED) {
-        v.G1 = uni_ptr(d.Y1);
-        v.W = uni_ptr(spin == 0 ? d.Wxx : d.Wxy);
-        v.Q = uni_ptr(d.Q[spin]);
-        v.Qp = uni_ptr(d.Qp[spin]);
-        v.F = uni_ptr(d.F[spin]);
-        v.Ft = uni_ptr(d.Ft[spin]);
         v.A = uni_ptr(d.A2[spin]);
-        v.K = uni_ptr(d.K2[spin]);
+        v.Y = uni_ptr(d.Y2[spin]);
     } else {
         v.D = uni_ptr(d.D[spin]);
     }
@@ -852,11 +887,11 @@ __device__ __forceinline__ void tile_geometry(int tiling, int N, int tile, int64
     }
 }
 
-// Stage input Y = S + c Kp of one tile into registers (TILING: 0 strided,
-// 1 contiguous, -1 given at run time).
+// Stage input of one tile into registers (TILING: 0 strided, 1 contiguous,
+// -1 given at run time).  Plain loads: nothing waits for them before the
+// MFMA loop.
 template <int NP, int NW, int TILING>
-__device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const StageShared& ss, const TileRef& t,
-                                              const double2* S, const double2* Kp,
+__device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef& t, const double2* X,
                                               double2 (&pre)[TileCfg<NP, NW>::PF]) {
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
     const int N = t.N;
@@ -869,9 +904,7 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const StageSha
         double2 y = make_double2(0.0, 0.0);
         if (e < N * N) {
             const int m = e / N, n = e % N;
-            const int64_t off = base + m * sm + n * sn;
-            y = S[off];
-            if (Kp) y = cadd(y, cscale(Kp[off], ss.c));
+            y = gload(X, base + m * sm + n * sn);
         }
         pre[i] = y;
     }
@@ -904,11 +937,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     if (gt < 0) return;
     TileRef tr = ref_of(c, gt);
     double2 pre[PF];
-    {
-        const DevState& d = a.b.ds[tr.traj];
-        tile_prefetch<NP, NW, TILING>(a, ss, tr, uni_ptr(d.S2[tr.spin]),
-                                      (a.use_kp && !ss.first) ? uni_ptr(d.K2[tr.spin]) : nullptr, pre);
-    }
+    tile_prefetch<NP, NW, TILING>(a, tr, uni_ptr(stage_input(a.b.ds[tr.traj], a, ss, tr.spin)), pre);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     while (gt >= 0) {
         const int N = tr.N;
@@ -940,7 +969,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
             if (e < N * N) {
                 const int m = e / N, n = e % N;
                 E::put_x(L, m, n, pre[i]);
-                if (dtile) v.D[((int64_t)m * N + n) * N + t0] = pre[i];
+                if (dtile) gstore(v.D, ((int64_t)m * N + n) * N + t0, pre[i]);
             }
         }
         lds_sync();
@@ -949,13 +978,10 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         TileRef nt = tr;
         if (next >= 0) {  // the next tile's loads overlap the MFMA loop
             nt = ref_of(c, next);
-            if (nt.traj == v.traj && nt.spin == v.spin) {
-                tile_prefetch<NP, NW, TILING>(a, ss, nt, v.S, v.Kp, pre);
-            } else {
-                const DevState& d = a.b.ds[nt.traj];
-                tile_prefetch<NP, NW, TILING>(a, ss, nt, uni_ptr(d.S2[nt.spin]),
-                                              (a.use_kp && !ss.first) ? uni_ptr(d.K2[nt.spin]) : nullptr, pre);
-            }
+            const double2* X = (nt.traj == v.traj && nt.spin == v.spin)
+                                   ? v.X
+                                   : uni_ptr(stage_input(a.b.ds[nt.traj], a, ss, nt.spin));
+            tile_prefetch<NP, NW, TILING>(a, nt, X, pre);
         }
 
         d4 ar[PW], ai[PW];
@@ -974,35 +1000,16 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                     if (m < N && n < N) {
                         const int64_t off = base + m * sm + n * sn;
                         double2 val = make_double2(ar[u][j], ai[u][j]);
-                        if (a.accumulate) val = cadd(out[off], val);
-                        out[off] = val;
+                        if (a.accumulate) val = cadd(gload(out, off), val);
+                        gstore(out, off, val);
                     }
                 }
             }
         } else {
-            // fused right-hand side for tile (p1, q1) = (t0, t1), element (p2, q2) = (m, n):
-            //   comm = R + O
-            //   Psi  = g11 g22 (W[p1][p2] - W[q1][q2]) + i g11 (Q[p1][p2][q2] - Q[q1][p2][q2])
-            //          + i g22 (Qp[p1][q1][p2] - Qp[p1][q1][q2])
-            //   Pi   = g22 (F[p1][q1][p2] - F[p1][q1][q2])
-            //   Pi^+ = conj( g(q1,p1) (Ft[q1][p2][q2] - Ft[p1][p2][q2]) )
-            //   K    = -i (comm + Psi + Pi - Pi^+)
-            const int p1 = t0, q1 = t1;
-            const double2* G1 = v.G1;
-            double2 g11 = G1[(int64_t)p1 * N + q1];
-            double2 gq1p1 = G1[(int64_t)q1 * N + p1];
-            g11 = make_double2(uni_d(g11.x), uni_d(g11.y));  // tile constants: scalar registers
-            gq1p1 = make_double2(uni_d(gq1p1.x), uni_d(gq1p1.y));
-            const double* W = v.W;
-            const double2* Qs = v.Q;
-            const double2* Qps = v.Qp + ((int64_t)p1 * N + q1) * N;
-            const double2* Frow = v.F + ((int64_t)p1 * N + q1) * N;
-            const double2* Fts = v.Ft;
-            const double2* R = v.R;
-            const double2* S = v.S;
-            double2* A2 = v.A;
-            double2* K2 = v.K;
-            double2* S2 = const_cast<double2*>(v.S);
+            // fused update for tile (p1, q1) = (t0, t1), element (p2, q2) = (m, n):
+            // R already holds the strided-pass terms plus the sources (k_sources),
+            // K = -i (R + O); then A += w K and the next stage input
+            // Y = S + c_next K, or at the last stage S += dt (A + w K) / 6.
 #pragma unroll
             for (int u = 0; u < PW; ++u) {
                 const int t = wave + NW * u;
@@ -1013,32 +1020,99 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                     const int m = (t / NT) * 16 + E::row_of(lane, j);
                     if (m >= N || n >= N) continue;
                     const int64_t off = base + m * sm + n * sn;
-                    const int64_t mn = (int64_t)m * N + n;
-                    double2 comm = cadd(R[off], make_double2(ar[u][j], ai[u][j]));
-                    const double2 g22 = G1[mn];
-                    double2 psi = cscale(cmul(g11, g22), W[(int64_t)p1 * N + m] - W[(int64_t)q1 * N + n]);
-                    psi = cadd(psi, cmul_i(cmul(g11, csub(Qs[(int64_t)p1 * N2 + mn], Qs[(int64_t)q1 * N2 + mn]))));
-                    psi = cadd(psi, cmul_i(cmul(g22, csub(Qps[m], Qps[n]))));
-                    const double2 pi = cmul(g22, csub(Frow[m], Frow[n]));
-                    const double2 pid =
-                        cconj(cmul(gq1p1, csub(Fts[(int64_t)q1 * N2 + mn], Fts[(int64_t)p1 * N2 + mn])));
-                    const double2 K = cmul_mi(csub(cadd(cadd(comm, psi), pi), pid));
+                    const double2 K = cmul_mi(cadd(gload(v.R, off), make_double2(ar[u][j], ai[u][j])));
                     const double2 wK = cscale(K, ss.wk);
-                    if (ss.first) {
-                        A2[off] = wK;
-                        K2[off] = K;
-                    } else if (!ss.last) {
-                        A2[off] = cadd(A2[off], wK);
-                        K2[off] = K;
+                    const double2 s = gload(v.S, off);
+                    if (!ss.last) {
+                        gstore(v.A, off, ss.first ? wK : cadd(gload(v.A, off), wK));
+                        gstore(v.Y, off, cadd(s, cscale(K, ss.cn)));
                     } else {
-                        const double2 A = cadd(A2[off], wK);
-                        S2[off] = cadd(S[off], cdiv6(cscale(A, ss.dt)));
+                        const double2 A = cadd(gload(v.A, off), wK);
+                        gstore(v.S, off, cadd(s, cdiv6(cscale(A, ss.dt))));
                     }
                 }
             }
         }
         gt = next;
         tr = nt;
+    }
+}
+
+// Sources of the G2 equation added to the strided-pass result, for the slice
+// (p1, q1) of both spins of one trajectory (element (p2, q2) = (m, n)):
+//   Psi  = g11 g22 (W[p1][p2] - W[q1][q2]) + i g11 (Q[p1][p2][q2] - Q[q1][p2][q2])
+//          + i g22 (Qp[p1][q1][p2] - Qp[p1][q1][q2])
+//   Pi   = g22 (F[p1][q1][p2] - F[p1][q1][q2])
+//   Pi^+ = conj( g(q1,p1) (Ft[q1][p2][q2] - Ft[p1][p2][q2]) )
+//   R   += Psi + Pi - Pi^+
+// with g = the stage G1, g11 = g(p1,q1), g22 = g(p2,q2), W = W_xx (spin 0)
+// or W_xy (spin 1) (psi.h:26-277, pi.h:25-95, G1_G2_t_step.h:55-86 through
+// the partial-trace tables).  A streaming kernel: its table and state loads
+// are hidden by occupancy instead of stalling the MFMA pass.  The slice's
+// vectors (W rows, Qp and F rows) are staged in LDS; G1 and W are shared by
+// the two spins.  One block per slice; block b is the bucket's tile b (spin 0
+// tiles only).
+__global__ __launch_bounds__(256) void k_sources(BTileArgs a) {
+    __shared__ double wp[G12_MAX_NB * 64], wq[G12_MAX_NB * 64];
+    __shared__ double2 qv[2][G12_MAX_NB * 64], fv[2][G12_MAX_NB * 64];
+    const StageShared ss = uni_stage(a.r);
+    // slice b of the bucket: listed trajectory j, tile t < N^2 (spin 0)
+    int lo = 0, hi = a.k.n - 1;
+    const int64_t b = blockIdx.x;
+    while (lo < hi) {  // last listed trajectory whose slices start at or before b
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.k.prefix[mid] / 2 <= b) lo = mid; else hi = mid - 1;
+    }
+    const int traj = uni(a.k.list[lo]);
+    const DevState& d = a.b.ds[traj];
+    if (!active(d, a.b.tp[traj], ss)) return;
+    const int N = a.b.tp[traj].N;
+    const int tile = (int)(b - a.k.prefix[lo] / 2);
+    if (tile >= N * N) return;
+    const int64_t N2 = (int64_t)N * N;
+    const int p1 = tile / N, q1 = tile % N;
+    const double2* G1 = uni_ptr(d.Y1);
+    const double* Wxy = uni_ptr(d.Wxy);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        wp[i] = gload1(Wxy, (int64_t)p1 * N + i);
+        wq[i] = gload1(Wxy, (int64_t)q1 * N + i);
+        for (int s = 0; s < 2; ++s) {
+            qv[s][i] = gload(d.Qp[s], ((int64_t)p1 * N + q1) * N + i);
+            fv[s][i] = gload(d.F[s], ((int64_t)p1 * N + q1) * N + i);
+        }
+    }
+    __syncthreads();
+    const double2 g11 = gload(G1, (int64_t)p1 * N + q1);
+    const double2 gq1p1 = gload(G1, (int64_t)q1 * N + p1);
+    const double2* Q0 = uni_ptr(d.Q[0]);
+    const double2* Q1 = uni_ptr(d.Q[1]);
+    const double2* T0 = uni_ptr(d.Ft[0]);
+    const double2* T1 = uni_ptr(d.Ft[1]);
+    double2* R0 = uni_ptr(d.R2[0]) + (int64_t)p1 * N2 * N + (int64_t)q1 * N;
+    double2* R1 = uni_ptr(d.R2[1]) + (int64_t)p1 * N2 * N + (int64_t)q1 * N;
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        const int m = e / N, n = e % N;
+        const int64_t off = (int64_t)m * N2 + n;
+        const double2 r0 = gload(R0, off), r1 = gload(R1, off);
+        const double2 g22 = gload(G1, e);
+        const double2 gg = cmul(g11, g22);
+        // W_xx differs from W_xy only on its diagonal (zero)
+        const double wxy = wp[m] - wq[n];
+        const double wxx = (m == p1 ? 0.0 : wp[m]) - (n == q1 ? 0.0 : wq[n]);
+        const double2* Qs[2] = {Q0, Q1};
+        const double2* Ts[2] = {T0, T1};
+        double2* Rs[2] = {R0, R1};
+        const double2 rs[2] = {r0, r1};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            double2 psi = cscale(gg, s == 0 ? wxx : wxy);
+            psi = cadd(psi, cmul_i(cmul(g11, csub(gload(Qs[s], (int64_t)p1 * N2 + e), gload(Qs[s], (int64_t)q1 * N2 + e)))));
+            psi = cadd(psi, cmul_i(cmul(g22, csub(qv[s][m], qv[s][n]))));
+            const double2 pi = cmul(g22, csub(fv[s][m], fv[s][n]));
+            const double2 pid =
+                cconj(cmul(gq1p1, csub(gload(Ts[s], (int64_t)q1 * N2 + e), gload(Ts[s], (int64_t)p1 * N2 + e))));
+            gstore(Rs[s], off, cadd(rs[s], csub(cadd(psi, pi), pid)));
+        }
     }
 }
 
@@ -1284,7 +1358,7 @@ void launch_b_tables(const BatchDev& b, const StageRef& r, hipStream_t s) {
 }
 
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& r, hipStream_t s) {
-    hipLaunchKernelGGL(k_g1_rhs, dim3(b.maxN, b.ntraj), dim3(64), 0, s, b, r);
+    hipLaunchKernelGGL(k_g1_rhs, dim3(b.maxN, b.ntraj), dim3(256), 0, s, b, r);
 }
 
 void launch_b_record(const BatchDev& b, const StageRef& r, hipStream_t s) {
@@ -1307,6 +1381,16 @@ int tile_wave_options(int NP, int* opts) {
     return 3;
 }
 
+void launch_b_sources(const BatchDev& b, const Bucket& k, const StageRef& r, hipStream_t s) {
+    if (k.tiles <= 0) return;
+    BTileArgs a{};
+    a.b = b;
+    a.k = k;
+    a.r = r;
+    // one block per (p1, q1) slice: half the bucket's tiles (two spins each)
+    hipLaunchKernelGGL(k_sources, dim3((unsigned)(k.tiles / 2)), dim3(256), 0, s, a);
+}
+
 void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fused, int precision, hipStream_t s) {
     BTileArgs a{};
     a.b = b;
@@ -1318,7 +1402,7 @@ void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fu
     a.hmode = 3;
     a.accumulate = 0;
     a.nspin = 2;
-    a.use_kp = 1;
+    a.use_y = 1;
     a.extract_d = fused ? 0 : 1;
     a.h_override = nullptr;
     launch_tile(a, precision, s);
@@ -1373,7 +1457,7 @@ void launch_comm2(const double2* T, const double2* hx, const double2* hy, double
     a.k.waves_fused = a.k.waves;
     a.r.val.first = 1;
     a.nspin = 1;
-    a.use_kp = 0;
+    a.use_y = 0;
     a.extract_d = 0;
     a.fused = 0;
     int first = 1;
