@@ -126,3 +126,19 @@ def test_g0_projector(Ns, Nb, t2):
 def test_g0_filling_override():
     p = g.make_params(Ns=6, Nb=2, t1=1.0, t2=0.5, p_number=6)
     assert abs(np.trace(-1j * g.make_G0(p)) - 6) < 1e-12
+
+
+@pytest.mark.parametrize("Ns,Nb,t2", [(16, 2, 0.3), (7, 3, 0.8)])
+def test_g0_fp32_emulation(Ns, Nb, t2):
+    """The FP32-eigensolver emulation (the reference's LAPACKE_cheev precision,
+    init_G.h:31-38) gives the same projector to FP32 accuracy on a
+    non-degenerate Fermi level, and still a Hermitian one with trace p."""
+    p = g.make_params(Ns=Ns, Nb=Nb, t1=1.0, t2=t2)
+    G64 = g.make_G0(p)
+    G32 = g.make_G0(p, fp32_emulation=True)
+    P = -1j * G32
+    assert np.abs(P - P.conj().T).max() < 1e-12
+    assert abs(np.trace(P) - Ns // 2) < 1e-5
+    assert np.abs(P @ P - P).max() < 1e-5
+    d = np.abs(G32 - G64).max()
+    assert 0 < d < 1e-5  # rounded through float, and only that far
