@@ -58,6 +58,25 @@ __device__ __forceinline__ void gstore(double2* p, int64_t i, double2 v) {
 }
 __device__ __forceinline__ double gload1(const double* p, int64_t i) { return ((const GLOBAL double*)p)[i]; }
 
+// Values that are the same in every lane, moved to scalar registers: the
+// trajectory descriptors are read with vector loads (the kernel writes global
+// memory, so scalar loads of them are not allowed), and left in vector
+// registers they would crowd out the tile data.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uni64(int64_t x) {
+    const uint64_t u = (uint64_t)x;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <class T>
+__device__ __forceinline__ T* uni_ptr(T* p) {
+    return reinterpret_cast<T*>(uni64((int64_t)reinterpret_cast<uintptr_t>(p)));
+}
+__device__ __forceinline__ double uni_d(double x) {
+    return __builtin_bit_cast(double, uni64(__builtin_bit_cast(int64_t, x)));
+}
+
 
 // Workgroup barrier that orders LDS only: the tile loop reuses its LDS buffers
 // but never exchanges global memory between waves, so waiting for the
@@ -119,9 +138,11 @@ __device__ inline double2 hf_element(const double2* G, int X, int Y, int Ns, int
 }
 
 // ---------------------------------------------------------------------------
-// Stage set-up for G1 (one workgroup): Y1 = S1 + c K1, h = h_HF[Y1],
-// W tables.  The Hartree sums read the diagonal of Y1 from LDS.
-__device__ inline double2 hf_element_diag(const double2* G, const double2* diag, int X, int Y, int Ns, int Nb,
+// h_HF for one element (X, Y) during the stage set-up: the Hartree sums read
+// the diagonal of the stage G1 from LDS, the Fock terms take G1 elements from
+// gv(index).
+template <class GV>
+__device__ inline double2 hf_element_with(GV gv, const double2* diag, int X, int Y, int Ns, int Nb,
                                           const StageParams& sp) {
     const int N = Ns * Nb;
     const int sx = X % Ns, ox = X / Ns, sy = Y % Ns, oy = Y / Ns;
@@ -146,9 +167,9 @@ __device__ inline double2 hf_element_diag(const double2* G, const double2* diag,
             int dorb = ox - oy;
             dorb = dorb < 0 ? -dorb : dorb;
             if (dorb == 1) v.x -= sp.t2;
-            v = cadd(v, cmul_i(cscale(G[(int64_t)Y * N + X], sp.U)));
+            v = cadd(v, cmul_i(cscale(gv((int64_t)Y * N + X), sp.U)));
         } else if (ds <= sp.nV) {
-            v = cadd(v, cmul_i(cscale(G[(int64_t)X * N + Y], sp.V[ds - 1])));
+            v = cadd(v, cmul_i(cscale(gv((int64_t)X * N + Y), sp.V[ds - 1])));
         }
     }
     return v;
@@ -190,35 +211,44 @@ __device__ inline bool active(const DevState& d, const TrajParams& t, const Stag
     return !(ss.as_phase && ss.step >= t.as_steps);
 }
 
-// Stage set-up for G1 (one workgroup per trajectory): Y1 = S1 + c K1,
-// h = h_HF[Y1], W tables.
-__global__ __launch_bounds__(1024) void k_stage_g1(BatchDev b, StageRef r) {
+// Stage set-up for G1: Y1 = S1 + c K1, h = h_HF[Y1], W tables.  Blocks of
+// 256 elements per trajectory; each block forms the diagonal of Y1 itself (for
+// the Hartree sums) and takes the Y1 elements of its Fock terms from S1 and
+// K1, so the blocks need no ordering among themselves.
+__global__ __launch_bounds__(256) void k_stage_g1(BatchDev b, StageRef r) {
     const int ti = blockIdx.y;
     const DevState& d = b.ds[ti];
     const TrajParams& tp = b.tp[ti];
     const StageShared ss = stage_of(r);
     if (!active(d, tp, ss)) return;
-    // built once per block in LDS: the V table is indexed at run time and
-    // would otherwise live in per-lane scratch
-    __shared__ StageParams sp;
-    fill_sp(sp, tp, ss);
-    __shared__ double2 diag[G12_MAX_NB * 64];
-    __syncthreads();
     const int N = d.N;
     const int n2 = N * N;
-    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
-        double2 y = d.S1[e];
-        if (!sp.first) y = cadd(y, cscale(d.K1[e], sp.c));
-        d.Y1[e] = y;
-        if (e / N == e % N) diag[e / N] = y;
-    }
+    const int e0 = blockIdx.x * blockDim.x;
+    if (e0 >= n2) return;
+    // built per block in LDS: the V table is indexed at run time and would
+    // otherwise live in per-lane scratch
+    __shared__ StageParams sp;
+    __shared__ double2 diag[G12_MAX_NB * 64];
+    fill_sp(sp, tp, ss);
+    const double2* S1 = uni_ptr(d.S1);
+    const double2* K1 = uni_ptr(d.K1);
+    const bool first = ss.first != 0;
+    const double c = ss.c;
+    auto yv = [&](int64_t i) {
+        double2 y = gload(S1, i);
+        if (!first) y = cadd(y, cscale(gload(K1, i), c));
+        return y;
+    };
+    for (int X = threadIdx.x; X < N; X += blockDim.x) diag[X] = yv((int64_t)X * N + X);
     __syncthreads();
-    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+    const int e = e0 + threadIdx.x;
+    if (e < n2) {
         const int X = e / N, Y = e % N;
-        d.h[e] = hf_element_diag(d.Y1, diag, X, Y, d.Ns, d.Nb, sp);
+        gstore(d.Y1, e, yv(e));
+        gstore(d.h, e, hf_element_with(yv, diag, X, Y, d.Ns, d.Nb, sp));
         const double w = w_xy(X, Y, d.Ns, sp);
-        d.Wxy[e] = w;
-        d.Wxx[e] = (X == Y) ? 0.0 : w;
+        ((GLOBAL double*)d.Wxy)[e] = w;
+        ((GLOBAL double*)d.Wxx)[e] = (X == Y) ? 0.0 : w;
     }
 }
 
@@ -766,24 +796,6 @@ struct TileSched {
     }
 };
 
-// Values that are the same in every lane, moved to scalar registers: the
-// trajectory descriptors are read with vector loads (the kernel writes global
-// memory, so scalar loads of them are not allowed), and left in vector
-// registers they would crowd out the tile data.
-__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ int64_t uni64(int64_t x) {
-    const uint64_t u = (uint64_t)x;
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-template <class T>
-__device__ __forceinline__ T* uni_ptr(T* p) {
-    return reinterpret_cast<T*>(uni64((int64_t)reinterpret_cast<uintptr_t>(p)));
-}
-__device__ __forceinline__ double uni_d(double x) {
-    return __builtin_bit_cast(double, uni64(__builtin_bit_cast(int64_t, x)));
-}
 __device__ inline StageShared uni_stage(const StageRef& r) {
     const StageShared v = stage_of(r);
     StageShared u;
@@ -984,6 +996,26 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
             tile_prefetch<NP, NW, TILING>(a, nt, X, pre);
         }
 
+        // fused pass: this lane's R and S elements load while the MFMA loop runs
+        double2 rpre[FUSED ? PW : 1][4], spre[FUSED ? PW : 1][4];
+        if constexpr (FUSED) {
+#pragma unroll
+            for (int u = 0; u < PW; ++u) {
+                const int t = wave + NW * u;
+                const int n = (t % NT) * 16 + (lane & 15);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int m = (t / NT) * 16 + E::row_of(lane, j);
+                    rpre[u][j] = spre[u][j] = make_double2(0.0, 0.0);
+                    if (t < TT && m < N && n < N) {
+                        const int64_t off = base + m * sm + n * sn;
+                        rpre[u][j] = gload(v.R, off);
+                        spre[u][j] = gload(v.S, off);
+                    }
+                }
+            }
+        }
+
         d4 ar[PW], ai[PW];
         E::compute(L, a.hmode, ar, ai, wave, lane);
 
@@ -1020,9 +1052,9 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                     const int m = (t / NT) * 16 + E::row_of(lane, j);
                     if (m >= N || n >= N) continue;
                     const int64_t off = base + m * sm + n * sn;
-                    const double2 K = cmul_mi(cadd(gload(v.R, off), make_double2(ar[u][j], ai[u][j])));
+                    const double2 K = cmul_mi(cadd(rpre[u][j], make_double2(ar[u][j], ai[u][j])));
                     const double2 wK = cscale(K, ss.wk);
-                    const double2 s = gload(v.S, off);
+                    const double2 s = spre[u][j];
                     if (!ss.last) {
                         gstore(v.A, off, ss.first ? wK : cadd(gload(v.A, off), wK));
                         gstore(v.Y, off, cadd(s, cscale(K, ss.cn)));
@@ -1056,9 +1088,13 @@ __global__ __launch_bounds__(256) void k_sources(BTileArgs a) {
     __shared__ double wp[G12_MAX_NB * 64], wq[G12_MAX_NB * 64];
     __shared__ double2 qv[2][G12_MAX_NB * 64], fv[2][G12_MAX_NB * 64];
     const StageShared ss = uni_stage(a.r);
-    // slice b of the bucket: listed trajectory j, tile t < N^2 (spin 0)
+    // slice b of the bucket (listed trajectory j, tile t < N^2 of spin 0);
+    // each XCD takes a contiguous range of slices, so the slices that share
+    // p1 (and with it the Q and Ft rows of p1) stay in one L2
+    const int64_t nsl = a.k.tiles / 2, per_xcd = (nsl + 7) / 8;
+    const int64_t b = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (b >= nsl) return;
     int lo = 0, hi = a.k.n - 1;
-    const int64_t b = blockIdx.x;
     while (lo < hi) {  // last listed trajectory whose slices start at or before b
         const int mid = (lo + hi + 1) >> 1;
         if (a.k.prefix[mid] / 2 <= b) lo = mid; else hi = mid - 1;
@@ -1348,7 +1384,7 @@ inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b
 // launchers
 
 void launch_b_stage_g1(const BatchDev& b, const StageRef& r, hipStream_t s) {
-    hipLaunchKernelGGL(k_stage_g1, dim3(1, b.ntraj), dim3(1024), 0, s, b, r);
+    hipLaunchKernelGGL(k_stage_g1, dim3((b.maxN * b.maxN + 255) / 256, b.ntraj), dim3(256), 0, s, b, r);
 }
 
 void launch_b_tables(const BatchDev& b, const StageRef& r, hipStream_t s) {
@@ -1387,8 +1423,10 @@ void launch_b_sources(const BatchDev& b, const Bucket& k, const StageRef& r, hip
     a.b = b;
     a.k = k;
     a.r = r;
-    // one block per (p1, q1) slice: half the bucket's tiles (two spins each)
-    hipLaunchKernelGGL(k_sources, dim3((unsigned)(k.tiles / 2)), dim3(256), 0, s, a);
+    // one block per (p1, q1) slice: half the bucket's tiles (two spins each),
+    // rounded up to whole XCD ranges
+    const int64_t nsl = k.tiles / 2, per_xcd = (nsl + 7) / 8;
+    hipLaunchKernelGGL(k_sources, dim3((unsigned)(8 * per_xcd)), dim3(256), 0, s, a);
 }
 
 void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fused, int precision, hipStream_t s) {
