@@ -236,11 +236,13 @@ int64_t layout(Run& r, char* base) {
         d.I = cv.take<double2>(d.n2);
         d.Wxy = cv.take<double>(d.n2);
         d.Wxx = cv.take<double>(d.n2);
+        // mixed-precision runs keep Y2 and R2 as complex64 (half the elements' bytes)
+        const int64_t n4c = r.precision != G12_PREC_FP64 ? (n4 + 1) / 2 : n4;
         for (int s = 0; s < 2; ++s) {
             d.S2[s] = cv.take<double2>(d.n4);  // the state is kept even when frozen
-            d.Y2[s] = cv.take<double2>(n4);
+            d.Y2[s] = cv.take<double2>(n4c);
             d.A2[s] = cv.take<double2>(n4);
-            d.R2[s] = cv.take<double2>(n4);
+            d.R2[s] = cv.take<double2>(n4c);
             d.D[s] = cv.take<double2>(n3);
             d.F[s] = cv.take<double2>(n3);
             d.Ft[s] = cv.take<double2>(n3);
@@ -391,7 +393,7 @@ void run_stage(Run& r, const StageRef& sr) {
     for (const Bucket& k : r.buckets) timed(r, 0, [&] { g12::launch_b_tile(r.b, k, sr, 0, r.precision, s); });
     if (r.any_corr) timed(r, 2, [&] { g12::launch_b_tables(r.b, sr, s); });
     timed(r, 2, [&] { g12::launch_b_g1_rhs(r.b, sr, s); });
-    for (const Bucket& k : r.buckets) timed(r, 2, [&] { g12::launch_b_sources(r.b, k, sr, s); });
+    for (const Bucket& k : r.buckets) timed(r, 2, [&] { g12::launch_b_sources(r.b, k, sr, r.precision, s); });
     for (const Bucket& k : r.buckets) timed(r, 1, [&] { g12::launch_b_tile(r.b, k, sr, 1, r.precision, s); });
 }
 
@@ -435,9 +437,10 @@ void drop_graphs(Run& r) {
 
 // Wave counts per bucket and pass, chosen by timing the compiled candidates on
 // the run's own tile lists.  Both passes are run as stage 0 of a step (the
-// strided pass writes only R2 and D, the fused pass only Y2 and A2, all of
-// which the next stage 0 overwrites before reading), so tuning at a step
-// boundary leaves the state unchanged.  Results are cached per process by
+// strided pass writes only R2, the fused pass only Y2, A2 and D; the next
+// stage 0 overwrites R2, Y2 and A2 before reading them and every run starts by
+// rebuilding D from S), so tuning at a step boundary leaves the state
+// unchanged.  Results are cached per process by
 // (tile size, precision, tile count).
 int tune(Run& r) {
     if (r.tuned || !r.autotune) return 0;
@@ -502,6 +505,9 @@ int run_windows(Run& r, int phase, int64_t nsteps, Fill&& fill, After&& after) {
     if (rc) return rc;
     const bool graph = r.graphs && !r.timing.on;
     if (graph && (rc = capture(r, phase))) return rc;
+    // partial traces of the first stage's input (S); every fused pass then
+    // writes those of the next stage input
+    for (const Bucket& k : r.buckets) g12::launch_b_d_from_state(r.b, k, r.stream);
     for (int64_t w0 = 0; w0 < nsteps; w0 += r.window) {
         const int n = (int)std::min<int64_t>(r.window, nsteps - w0);
         for (int i = 0; i < n; ++i) fill(w0 + i, i, r.h_table + 4 * i);

@@ -21,6 +21,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 
 namespace g12 {
 namespace {
@@ -57,6 +58,23 @@ __device__ __forceinline__ void gstore(double2* p, int64_t i, double2 v) {
     ((GLOBAL dv2*)p)[i] = dv2{v.x, v.y};
 }
 __device__ __forceinline__ double gload1(const double* p, int64_t i) { return ((const GLOBAL double*)p)[i]; }
+
+// Complex elements stored as complex128 or, in the compact (complex64) form
+// of mixed-precision runs, as complex64; arithmetic stays FP64.
+typedef float fv2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 cload(const void* p, int64_t i, bool compact) {
+    if (compact) {
+        const fv2 v = ((const GLOBAL fv2*)p)[i];
+        return make_double2(v.x, v.y);
+    }
+    return gload((const double2*)p, i);
+}
+__device__ __forceinline__ void cstore(void* p, int64_t i, double2 v, bool compact) {
+    if (compact)
+        ((GLOBAL fv2*)p)[i] = fv2{(float)v.x, (float)v.y};
+    else
+        gstore((double2*)p, i, v);
+}
 
 // Values that are the same in every lane, moved to scalar registers: the
 // trajectory descriptors are read with vector loads (the kernel writes global
@@ -765,7 +783,7 @@ struct BTileArgs {
     int accumulate;   // unfused: R += O instead of R = O
     int nspin;
     int use_y;        // stage input from the Y buffer after stage 0 (0: always S)
-    int extract_d;    // unfused strided pass: D from the diagonal tiles
+    int compact;      // R and Y held as complex64 (mixed-precision runs)
     const double2* h_override;  // routine calls: h to use instead of the trajectory's h
 };
 
@@ -857,14 +875,19 @@ __device__ inline TileRef ref_of(const Cursor& c, int64_t gt) {
 struct View {
     int traj = -1, spin = -1;
     const double2* X;   // stage input: S at stage 0, else Y = S + c K_prev
+    bool xc;            // X held compact
     double2* S;         // stage state (fused: read for Y, updated at the last stage)
-    double2* R;         // unfused output / strided-pass result
-    double2* D;         // D extraction (unfused strided pass)
-    double2 *A, *Y;     // fused: RK accumulator, next stage input
+    double2* R;         // unfused output / strided-pass result (compact in mixed runs)
+    double2* D;         // fused: partial traces of the next stage input
+    double2 *A, *Y;     // fused: RK accumulator, next stage input (compact in mixed runs)
 };
 
 __device__ inline const double2* stage_input(const DevState& d, const BTileArgs& a, const StageShared& ss, int spin) {
     return (a.use_y && !ss.first) ? d.Y2[spin] : d.S2[spin];
+}
+// the stage input is compact when it is the Y buffer of a mixed-precision run
+__device__ inline bool input_compact(const BTileArgs& a, const StageShared& ss) {
+    return a.compact && a.use_y && !ss.first;
 }
 
 template <bool FUSED>
@@ -873,12 +896,12 @@ __device__ inline void load_view(View& v, const BTileArgs& a, const StageShared&
     v.traj = traj;
     v.spin = spin;
     v.X = uni_ptr(stage_input(d, a, ss, spin));
+    v.xc = input_compact(a, ss);
     v.S = uni_ptr(d.S2[spin]);
     v.R = uni_ptr(d.R2[spin]);
     if constexpr (FUSED) {
         v.A = uni_ptr(d.A2[spin]);
         v.Y = uni_ptr(d.Y2[spin]);
-    } else {
         v.D = uni_ptr(d.D[spin]);
     }
 }
@@ -903,22 +926,30 @@ __device__ __forceinline__ void tile_geometry(int tiling, int N, int tile, int64
 // -1 given at run time).  Plain loads: nothing waits for them before the
 // MFMA loop.
 template <int NP, int NW, int TILING>
-__device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef& t, const double2* X,
+__device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef& t, const double2* X, bool xc,
                                               double2 (&pre)[TileCfg<NP, NW>::PF]) {
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
     const int N = t.N;
     int64_t base, sm, sn;
     int t0, t1;
     tile_geometry(TILING < 0 ? a.tiling : TILING, N, t.tile, base, sm, sn, t0, t1);
+    // one branch on the storage form, outside the unrolled loads
+    if (xc) {
 #pragma unroll
-    for (int i = 0; i < PF; ++i) {
-        const int e = threadIdx.x + NTHR * i;
-        double2 y = make_double2(0.0, 0.0);
-        if (e < N * N) {
-            const int m = e / N, n = e % N;
-            y = gload(X, base + m * sm + n * sn);
+        for (int i = 0; i < PF; ++i) {
+            const int e = threadIdx.x + NTHR * i;
+            double2 y = make_double2(0.0, 0.0);
+            if (e < N * N) y = cload(X, base + (e / N) * sm + (e % N) * sn, true);
+            pre[i] = y;
         }
-        pre[i] = y;
+    } else {
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int e = threadIdx.x + NTHR * i;
+            double2 y = make_double2(0.0, 0.0);
+            if (e < N * N) y = gload(X, base + (e / N) * sm + (e % N) * sn);
+            pre[i] = y;
+        }
     }
 }
 
@@ -949,7 +980,8 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     if (gt < 0) return;
     TileRef tr = ref_of(c, gt);
     double2 pre[PF];
-    tile_prefetch<NP, NW, TILING>(a, tr, uni_ptr(stage_input(a.b.ds[tr.traj], a, ss, tr.spin)), pre);
+    tile_prefetch<NP, NW, TILING>(a, tr, uni_ptr(stage_input(a.b.ds[tr.traj], a, ss, tr.spin)),
+                                  input_compact(a, ss), pre);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     while (gt >= 0) {
         const int N = tr.N;
@@ -973,16 +1005,10 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         int64_t base, sm, sn;
         int t0, t1;
         tile_geometry(TILING < 0 ? a.tiling : TILING, N, tr.tile, base, sm, sn, t0, t1);
-        // D[X=p1][Y=q1][Z] (Z = p2 = q2) from the FP64 stage input of a diagonal strided tile
-        const bool dtile = !FUSED && a.extract_d && a.tiling == 0 && t0 == t1;
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
-            if (e < N * N) {
-                const int m = e / N, n = e % N;
-                E::put_x(L, m, n, pre[i]);
-                if (dtile) gstore(v.D, ((int64_t)m * N + n) * N + t0, pre[i]);
-            }
+            if (e < N * N) E::put_x(L, e / N, e % N, pre[i]);
         }
         lds_sync();
         ++it;
@@ -993,27 +1019,33 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
             const double2* X = (nt.traj == v.traj && nt.spin == v.spin)
                                    ? v.X
                                    : uni_ptr(stage_input(a.b.ds[nt.traj], a, ss, nt.spin));
-            tile_prefetch<NP, NW, TILING>(a, nt, X, pre);
+            tile_prefetch<NP, NW, TILING>(a, nt, X, input_compact(a, ss), pre);
         }
 
         // fused pass: this lane's R and S elements load while the MFMA loop runs
         double2 rpre[FUSED ? PW : 1][4], spre[FUSED ? PW : 1][4];
         if constexpr (FUSED) {
+            auto preload = [&](auto compact) {
 #pragma unroll
-            for (int u = 0; u < PW; ++u) {
-                const int t = wave + NW * u;
-                const int n = (t % NT) * 16 + (lane & 15);
+                for (int u = 0; u < PW; ++u) {
+                    const int t = wave + NW * u;
+                    const int n = (t % NT) * 16 + (lane & 15);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int m = (t / NT) * 16 + E::row_of(lane, j);
-                    rpre[u][j] = spre[u][j] = make_double2(0.0, 0.0);
-                    if (t < TT && m < N && n < N) {
-                        const int64_t off = base + m * sm + n * sn;
-                        rpre[u][j] = gload(v.R, off);
-                        spre[u][j] = gload(v.S, off);
+                    for (int j = 0; j < 4; ++j) {
+                        const int m = (t / NT) * 16 + E::row_of(lane, j);
+                        rpre[u][j] = spre[u][j] = make_double2(0.0, 0.0);
+                        if (t < TT && m < N && n < N) {
+                            const int64_t off = base + m * sm + n * sn;
+                            rpre[u][j] = cload(v.R, off, decltype(compact)::value);
+                            spre[u][j] = gload(v.S, off);
+                        }
                     }
                 }
-            }
+            };
+            if (a.compact)
+                preload(std::true_type{});
+            else
+                preload(std::false_type{});
         }
 
         d4 ar[PW], ai[PW];
@@ -1021,49 +1053,69 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
 
         if constexpr (!FUSED) {
             double2* out = v.R;
+            auto store = [&](auto compact) {
+                constexpr bool C = decltype(compact)::value;
 #pragma unroll
-            for (int u = 0; u < PW; ++u) {
-                const int t = wave + NW * u;
-                if (t >= TT) continue;
-                const int n = (t % NT) * 16 + (lane & 15);
+                for (int u = 0; u < PW; ++u) {
+                    const int t = wave + NW * u;
+                    if (t >= TT) continue;
+                    const int n = (t % NT) * 16 + (lane & 15);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int m = (t / NT) * 16 + E::row_of(lane, j);
-                    if (m < N && n < N) {
-                        const int64_t off = base + m * sm + n * sn;
-                        double2 val = make_double2(ar[u][j], ai[u][j]);
-                        if (a.accumulate) val = cadd(gload(out, off), val);
-                        gstore(out, off, val);
+                    for (int j = 0; j < 4; ++j) {
+                        const int m = (t / NT) * 16 + E::row_of(lane, j);
+                        if (m < N && n < N) {
+                            const int64_t off = base + m * sm + n * sn;
+                            double2 val = make_double2(ar[u][j], ai[u][j]);
+                            if (a.accumulate) val = cadd(cload(out, off, C), val);
+                            cstore(out, off, val, C);
+                        }
                     }
                 }
-            }
+            };
+            if (a.compact)
+                store(std::true_type{});
+            else
+                store(std::false_type{});
         } else {
             // fused update for tile (p1, q1) = (t0, t1), element (p2, q2) = (m, n):
             // R already holds the strided-pass terms plus the sources (k_sources),
             // K = -i (R + O); then A += w K and the next stage input
             // Y = S + c_next K, or at the last stage S += dt (A + w K) / 6.
+            auto update = [&](auto compact) {
+                constexpr bool C = decltype(compact)::value;
 #pragma unroll
-            for (int u = 0; u < PW; ++u) {
-                const int t = wave + NW * u;
-                if (t >= TT) continue;
-                const int n = (t % NT) * 16 + (lane & 15);
+                for (int u = 0; u < PW; ++u) {
+                    const int t = wave + NW * u;
+                    if (t >= TT) continue;
+                    const int n = (t % NT) * 16 + (lane & 15);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int m = (t / NT) * 16 + E::row_of(lane, j);
-                    if (m >= N || n >= N) continue;
-                    const int64_t off = base + m * sm + n * sn;
-                    const double2 K = cmul_mi(cadd(rpre[u][j], make_double2(ar[u][j], ai[u][j])));
-                    const double2 wK = cscale(K, ss.wk);
-                    const double2 s = spre[u][j];
-                    if (!ss.last) {
-                        gstore(v.A, off, ss.first ? wK : cadd(gload(v.A, off), wK));
-                        gstore(v.Y, off, cadd(s, cscale(K, ss.cn)));
-                    } else {
-                        const double2 A = cadd(gload(v.A, off), wK);
-                        gstore(v.S, off, cadd(s, cdiv6(cscale(A, ss.dt))));
+                    for (int j = 0; j < 4; ++j) {
+                        const int m = (t / NT) * 16 + E::row_of(lane, j);
+                        if (m >= N || n >= N) continue;
+                        const int64_t off = base + m * sm + n * sn;
+                        const double2 K = cmul_mi(cadd(rpre[u][j], make_double2(ar[u][j], ai[u][j])));
+                        const double2 wK = cscale(K, ss.wk);
+                        const double2 s = spre[u][j];
+                        double2 next;  // input of the next stage (the new state after the last)
+                        if (!ss.last) {
+                            gstore(v.A, off, ss.first ? wK : cadd(gload(v.A, off), wK));
+                            next = cadd(s, cscale(K, ss.cn));
+                            cstore(v.Y, off, next, C);
+                        } else {
+                            const double2 A = cadd(gload(v.A, off), wK);
+                            next = cadd(s, cdiv6(cscale(A, ss.dt)));
+                            gstore(v.S, off, next);
+                        }
+                        // partial trace of the next stage input, FP64:
+                        // D[X=p1][Y=q1][Z=p2] for p2 = q2
+                        if (m == n) gstore(v.D, ((int64_t)t0 * N + t1) * N + m, next);
                     }
                 }
-            }
+            };
+            if (a.compact)
+                update(std::true_type{});
+            else
+                update(std::false_type{});
         }
         gt = next;
         tr = nt;
@@ -1124,12 +1176,14 @@ __global__ __launch_bounds__(256) void k_sources(BTileArgs a) {
     const double2* Q1 = uni_ptr(d.Q[1]);
     const double2* T0 = uni_ptr(d.Ft[0]);
     const double2* T1 = uni_ptr(d.Ft[1]);
-    double2* R0 = uni_ptr(d.R2[0]) + (int64_t)p1 * N2 * N + (int64_t)q1 * N;
-    double2* R1 = uni_ptr(d.R2[1]) + (int64_t)p1 * N2 * N + (int64_t)q1 * N;
+    const bool cp = a.compact != 0;
+    const int64_t rbase = (int64_t)p1 * N2 * N + (int64_t)q1 * N;  // element (p1, 0, q1, 0)
+    void* R0 = cp ? (void*)((float2*)uni_ptr(d.R2[0]) + rbase) : (void*)(uni_ptr(d.R2[0]) + rbase);
+    void* R1 = cp ? (void*)((float2*)uni_ptr(d.R2[1]) + rbase) : (void*)(uni_ptr(d.R2[1]) + rbase);
     for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
         const int m = e / N, n = e % N;
         const int64_t off = (int64_t)m * N2 + n;
-        const double2 r0 = gload(R0, off), r1 = gload(R1, off);
+        const double2 r0 = cload(R0, off, cp), r1 = cload(R1, off, cp);
         const double2 g22 = gload(G1, e);
         const double2 gg = cmul(g11, g22);
         // W_xx differs from W_xy only on its diagonal (zero)
@@ -1137,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_sources(BTileArgs a) {
         const double wxx = (m == p1 ? 0.0 : wp[m]) - (n == q1 ? 0.0 : wq[n]);
         const double2* Qs[2] = {Q0, Q1};
         const double2* Ts[2] = {T0, T1};
-        double2* Rs[2] = {R0, R1};
+        void* Rs[2] = {R0, R1};
         const double2 rs[2] = {r0, r1};
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -1147,8 +1201,33 @@ __global__ __launch_bounds__(256) void k_sources(BTileArgs a) {
             const double2 pi = cmul(g22, csub(fv[s][m], fv[s][n]));
             const double2 pid =
                 cconj(cmul(gq1p1, csub(gload(Ts[s], (int64_t)q1 * N2 + e), gload(Ts[s], (int64_t)p1 * N2 + e))));
-            gstore(Rs[s], off, cadd(rs[s], csub(cadd(psi, pi), pid)));
+            cstore(Rs[s], off, cadd(rs[s], csub(cadd(psi, pi), pid)), cp);
         }
+    }
+}
+
+// Partial traces of the state at the start of a run (the stage-0 input):
+// D[X][Y][Z] = S[X][Z][Y][Z] for both spins; within a run each fused pass
+// writes those of the next stage input.  One block per slice (X, Y), blocks in
+// the bucket's tile order (spin 0 tiles).
+__global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = a.k.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.k.prefix[mid] / 2 <= b) lo = mid; else hi = mid - 1;
+    }
+    const int traj = a.k.list[lo];
+    const DevState& d = a.b.ds[traj];
+    const int N = a.b.tp[traj].N;
+    const int tile = (int)(b - a.k.prefix[lo] / 2);
+    if (tile >= N * N) return;
+    const int X = tile / N, Y = tile % N;
+    const int64_t N2 = (int64_t)N * N;
+    for (int Z = threadIdx.x; Z < N; Z += blockDim.x) {
+        const int64_t src = (int64_t)X * N2 * N + (int64_t)Z * N2 + (int64_t)Y * N + Z;
+        const int64_t dst = ((int64_t)X * N + Y) * N + Z;
+        for (int s = 0; s < 2; ++s) gstore(d.D[s], dst, gload(d.S2[s], src));
     }
 }
 
@@ -1417,12 +1496,21 @@ int tile_wave_options(int NP, int* opts) {
     return 3;
 }
 
-void launch_b_sources(const BatchDev& b, const Bucket& k, const StageRef& r, hipStream_t s) {
+void launch_b_d_from_state(const BatchDev& b, const Bucket& k, hipStream_t s) {
+    if (k.tiles <= 0) return;
+    BTileArgs a{};
+    a.b = b;
+    a.k = k;
+    hipLaunchKernelGGL(k_d_from_state, dim3((unsigned)(k.tiles / 2)), dim3(64), 0, s, a);
+}
+
+void launch_b_sources(const BatchDev& b, const Bucket& k, const StageRef& r, int precision, hipStream_t s) {
     if (k.tiles <= 0) return;
     BTileArgs a{};
     a.b = b;
     a.k = k;
     a.r = r;
+    a.compact = precision != G12_PREC_FP64;
     // one block per (p1, q1) slice: half the bucket's tiles (two spins each),
     // rounded up to whole XCD ranges
     const int64_t nsl = k.tiles / 2, per_xcd = (nsl + 7) / 8;
@@ -1441,7 +1529,7 @@ void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fu
     a.accumulate = 0;
     a.nspin = 2;
     a.use_y = 1;
-    a.extract_d = fused ? 0 : 1;
+    a.compact = precision != G12_PREC_FP64;
     a.h_override = nullptr;
     launch_tile(a, precision, s);
 }
@@ -1496,7 +1584,7 @@ void launch_comm2(const double2* T, const double2* hx, const double2* hy, double
     a.r.val.first = 1;
     a.nspin = 1;
     a.use_y = 0;
-    a.extract_d = 0;
+    a.compact = 0;
     a.fused = 0;
     int first = 1;
     for (int tiling = 0; tiling < 2; ++tiling)
