@@ -3,6 +3,7 @@
 // with hipGraph replay of whole RK4 steps, the tile-shape autotuner, and the
 // routine-level entry points (one per reference routine).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -391,7 +392,7 @@ void run_stage(Run& r, const StageRef& sr) {
     hipStream_t s = r.stream;
     timed(r, 2, [&] { g12::launch_b_stage_g1(r.b, sr, s); });
     for (const Bucket& k : r.buckets) timed(r, 0, [&] { g12::launch_b_tile(r.b, k, sr, 0, r.precision, s); });
-    if (r.any_corr) timed(r, 2, [&] { g12::launch_b_tables(r.b, sr, s); });
+    if (r.any_corr) timed(r, 2, [&] { g12::launch_b_tables(r.b, sr, r.precision, s); });
     timed(r, 2, [&] { g12::launch_b_g1_rhs(r.b, sr, s); });
     for (const Bucket& k : r.buckets) timed(r, 2, [&] { g12::launch_b_sources(r.b, k, sr, r.precision, s); });
     for (const Bucket& k : r.buckets) timed(r, 1, [&] { g12::launch_b_tile(r.b, k, sr, 1, r.precision, s); });
@@ -444,7 +445,8 @@ void drop_graphs(Run& r) {
 // (tile size, precision, tile count).
 int tune(Run& r) {
     if (r.tuned || !r.autotune) return 0;
-    static std::map<std::tuple<int, int, int64_t>, std::pair<int, int>> cache;
+    // (waves, workgroups-per-CU cap) of the strided and the fused pass
+    static std::map<std::tuple<int, int, int64_t>, std::array<int, 4>> cache;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -455,37 +457,46 @@ int tune(Run& r) {
         const auto key = std::make_tuple(k.NP, r.precision, k.tiles);
         auto it = cache.find(key);
         if (it != cache.end()) {
-            k.waves = it->second.first;
-            k.waves_fused = it->second.second;
+            k.waves = it->second[0];
+            k.wg_per_cu = it->second[1];
+            k.waves_fused = it->second[2];
+            k.wg_per_cu_fused = it->second[3];
             continue;
         }
         int opts[8];
         const int nopt = g12::tile_wave_options(k.NP, opts);
-        int best[2] = {k.waves, k.waves_fused};
+        // fewer resident workgroups give each more tiles to overlap loads with
+        static const int caps[3] = {0, 2, 1};
+        std::array<int, 4> best = {k.waves, k.wg_per_cu, k.waves_fused, k.wg_per_cu_fused};
         for (int fused = 0; fused < 2 && !rc; ++fused) {
             float best_ms = 1e30f;
-            for (int o = 0; o < nopt; ++o) {
-                Bucket c = k;
-                c.waves = c.waves_fused = opts[o];
-                g12::launch_b_tile(r.b, c, sr, fused, r.precision, r.stream);  // warm-up
-                hipEventRecord(e0, r.stream);
-                for (int rep = 0; rep < 2; ++rep) g12::launch_b_tile(r.b, c, sr, fused, r.precision, r.stream);
-                hipEventRecord(e1, r.stream);
-                if (hipEventSynchronize(e1) != hipSuccess) {
-                    rc = G12_E_HIP;
-                    break;
+            for (int o = 0; o < nopt && !rc; ++o)
+                for (int cap : caps) {
+                    Bucket c = k;
+                    c.waves = c.waves_fused = opts[o];
+                    c.wg_per_cu = c.wg_per_cu_fused = cap;
+                    g12::launch_b_tile(r.b, c, sr, fused, r.precision, r.stream);  // warm-up
+                    hipEventRecord(e0, r.stream);
+                    for (int rep = 0; rep < 2; ++rep) g12::launch_b_tile(r.b, c, sr, fused, r.precision, r.stream);
+                    hipEventRecord(e1, r.stream);
+                    if (hipEventSynchronize(e1) != hipSuccess) {
+                        rc = G12_E_HIP;
+                        break;
+                    }
+                    float ms = 0.f;
+                    hipEventElapsedTime(&ms, e0, e1);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best[2 * fused] = opts[o];
+                        best[2 * fused + 1] = cap;
+                    }
                 }
-                float ms = 0.f;
-                hipEventElapsedTime(&ms, e0, e1);
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best[fused] = opts[o];
-                }
-            }
         }
         k.waves = best[0];
-        k.waves_fused = best[1];
-        cache[key] = {best[0], best[1]};
+        k.wg_per_cu = best[1];
+        k.waves_fused = best[2];
+        k.wg_per_cu_fused = best[3];
+        cache[key] = best;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -716,7 +727,10 @@ int configure(Run& r, int use_graphs, int autotune) {
     if (autotune >= 0) {
         const bool want = autotune != 0;
         if (!want && r.tuned) {  // back to the compiled defaults
-            for (Bucket& k : r.buckets) k.waves = k.waves_fused = g12::default_waves(k.NP);
+            for (Bucket& k : r.buckets) {
+                k.waves = k.waves_fused = g12::default_waves(k.NP);
+                k.wg_per_cu = k.wg_per_cu_fused = 0;
+            }
             drop_graphs(r);
         }
         r.autotune = want;

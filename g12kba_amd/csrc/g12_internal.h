@@ -98,12 +98,13 @@ struct Bucket {
     int64_t tiles;
     int waves;               // waves per workgroup, strided pass (autotuned)
     int waves_fused;         // waves per workgroup, fused pass (autotuned)
-    int wg_per_cu;           // workgroups per CU cap (0: as many as LDS and waves allow)
+    int wg_per_cu;           // workgroups per CU cap, strided pass (0: as many as LDS and waves allow)
+    int wg_per_cu_fused;     // the same for the fused pass
 };
 
 // batched stage kernels (kernels.hip)
 void launch_b_stage_g1(const BatchDev& b, const StageRef& sr, hipStream_t s);
-void launch_b_tables(const BatchDev& b, const StageRef& sr, hipStream_t s);
+void launch_b_tables(const BatchDev& b, const StageRef& sr, int precision, hipStream_t s);
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_record(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int fused, int precision, hipStream_t s);

@@ -307,12 +307,24 @@ __device__ inline void q_tables_block(const DevState& d, int Z, int X) {
 // (Pi_xyxy with the swapped G2 pair, pi.h:25-95 / RK_solver.h:163-164), the
 // transposed copies Ft[Z][Y][X], and I[X][Y] = F^xx[X][Y][X]
 // (collision_int, collision_integral.h:27-69).  One block per (X, Y).
-__device__ inline void f_tables_block(const DevState& d, int X, int Y) {
+// With `herm` (mixed-precision runs) D enters through its Hermitian part
+// (D[X][Y][Z] + conj D[Y][X][Z]) / 2.  A Hermitian G2 (the physical states:
+// G2 evolved from zero stays Hermitian) has D[X][Y][Z] = conj D[Y][X][Z], and
+// that identity makes Re tr I, and with it the change of tr G1, vanish; FP32
+// or BF16 tile products break it slightly, the average restores particle-number
+// conservation to FP64 level.  FP64 runs use D as it is, as the reference does,
+// for any input.
+__device__ inline void f_tables_block(const DevState& d, int X, int Y, bool herm) {
     __shared__ double2 dxx[G12_MAX_NB * 64], dxy[G12_MAX_NB * 64];
     const int N = d.N;
     for (int w = threadIdx.x; w < N; w += blockDim.x) {
-        dxx[w] = d.D[0][((int64_t)X * N + Y) * N + w];
-        dxy[w] = d.D[1][((int64_t)X * N + Y) * N + w];
+        const int64_t xy = ((int64_t)X * N + Y) * N + w, yx = ((int64_t)Y * N + X) * N + w;
+        dxx[w] = d.D[0][xy];
+        dxy[w] = d.D[1][xy];
+        if (herm) {
+            dxx[w] = cscale(cadd(dxx[w], cconj(d.D[0][yx])), 0.5);
+            dxy[w] = cscale(cadd(dxy[w], cconj(d.D[1][yx])), 0.5);
+        }
     }
     __syncthreads();
     for (int Z = threadIdx.x; Z < N; Z += blockDim.x) {
@@ -346,13 +358,13 @@ __global__ __launch_bounds__(64) void k_q_tables(BatchDev b, StageRef r) {
     q_tables_block(d, blockIdx.x / N, blockIdx.x % N);
 }
 
-__global__ __launch_bounds__(64) void k_f_tables(BatchDev b, StageRef r) {
+__global__ __launch_bounds__(64) void k_f_tables(BatchDev b, StageRef r, int herm) {
     const int ti = blockIdx.y;
     const DevState& d = b.ds[ti];
     if (!b.tp[ti].corr || !active(d, b.tp[ti], stage_of(r))) return;
     const int N = d.N;
     if ((int)blockIdx.x >= N * N) return;
-    f_tables_block(d, blockIdx.x / N, blockIdx.x % N);
+    f_tables_block(d, blockIdx.x / N, blockIdx.x % N, herm != 0);
 }
 
 // G1 right-hand side: comm = [h, Y1], K = -i(comm + I + I^dag), RK
@@ -1466,10 +1478,10 @@ void launch_b_stage_g1(const BatchDev& b, const StageRef& r, hipStream_t s) {
     hipLaunchKernelGGL(k_stage_g1, dim3((b.maxN * b.maxN + 255) / 256, b.ntraj), dim3(256), 0, s, b, r);
 }
 
-void launch_b_tables(const BatchDev& b, const StageRef& r, hipStream_t s) {
+void launch_b_tables(const BatchDev& b, const StageRef& r, int precision, hipStream_t s) {
     const unsigned n2 = (unsigned)(b.maxN * b.maxN);
     hipLaunchKernelGGL(k_q_tables, dim3(n2, b.ntraj), dim3(64), 0, s, b, r);
-    hipLaunchKernelGGL(k_f_tables, dim3(n2, b.ntraj), dim3(64), 0, s, b, r);
+    hipLaunchKernelGGL(k_f_tables, dim3(n2, b.ntraj), dim3(64), 0, s, b, r, precision != G12_PREC_FP64 ? 1 : 0);
 }
 
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& r, hipStream_t s) {
@@ -1521,7 +1533,10 @@ void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fu
     BTileArgs a{};
     a.b = b;
     a.k = k;
-    if (fused) a.k.waves = k.waves_fused;
+    if (fused) {
+        a.k.waves = k.waves_fused;
+        a.k.wg_per_cu = k.wg_per_cu_fused;
+    }
     a.r = r;
     a.tiling = fused ? 1 : 0;
     a.fused = fused;

@@ -141,3 +141,32 @@ def test_config2_full_run():
     assert relerr(Gxxe.reshape(-1)[idx], z["G2xx_sample"]) < 1e-9
     assert relerr(Gxye.reshape(-1)[idx], z["G2xy_sample"]) < 1e-9
     assert abs(np.linalg.norm(Gxxe) / float(z["G2xx_norm"]) - 1) < 1e-10
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32_3m", 1e-5), ("bf16x3", 1e-5)])
+def test_n64_invariants(prec, tol):
+    # tr G1 stays fixed to round-off in every precision: mixed-precision runs
+    # take D through its Hermitian part in the collision integral
+    """Size-independent checks at the full N = 64 workload size (the oracle
+    cannot run it): from G2 = 0 the equations keep each G2 tensor Hermitian as
+    an N^2 x N^2 matrix, symmetric under exchanging both particle pairs
+    (T[p1][p2][q1][q2] = T[p2][p1][q2][q1]), and the trace of G1 fixed."""
+    from oracle import fast
+    Ns, Nb, N = 32, 2, 64
+    p = g.make_params(Ns=Ns, Nb=Nb, t1=1.0, t2=0.5, U=1.5, dt=0.01, time_steps=40, quench_strength=1.0, Nq=8,
+                      quench_rate=2.0, quench_on_midpoint=0.1, quench_gate_time=0.1, EHM=True, V_law="coulomb",
+                      precision=prec)
+    with g.Propagator(p) as prop:
+        prop.set_state(g.make_G0(p))
+        rc, traj = prop.rk4()
+        G1, Gxx, Gxy = prop.get_state()
+    assert rc == 0
+    tr = np.trace(traj, axis1=1, axis2=2)
+    assert np.abs(tr - tr[0]).max() < 1e-12
+    for G in (Gxx, Gxy):
+        scale = np.abs(G).max()
+        assert scale > 1e-6  # correlations have built up
+        assert np.abs(G - G.conj().T).max() <= tol * scale
+        T = fast.ref_to_nat(G, Ns, Nb).reshape(N, N, N, N)
+        assert np.abs(T - T.transpose(1, 0, 3, 2)).max() <= tol * scale
+        del T
