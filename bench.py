@@ -219,6 +219,33 @@ def extrapolator_leg(g, total=2000, prefix=400):
     return out
 
 
+def mixed_error_run(g, rk_steps):
+    """Mixed-precision error along a physical run of the N = 64 workload: from
+    G1(0) and G2 = 0, adiabatic switching then `rk_steps` production steps in
+    every precision; the G1 trajectories are compared with FP64's."""
+    import numpy as np
+    out, trajs = {}, {}
+    for prec in ("fp64", "fp32_3m", "bf16x3"):
+        p = workload_params(g, prec, time_steps=rk_steps)
+        with g.Propagator(p) as prop:
+            prop.set_state(g.make_G0(p))
+            t0 = time.perf_counter()
+            nas = prop.adiabatic_switching()
+            rc, traj = prop.rk4()
+            sec = time.perf_counter() - t0
+        if rc != 0:
+            raise SystemExit(f"density guard tripped in the error run ({prec})")
+        trajs[prec] = traj
+        out[prec] = {"as_steps": nas, "rk_steps": len(traj), "seconds": round(sec, 2)}
+    d64 = np.diagonal(trajs["fp64"], axis1=1, axis2=2).imag
+    p64 = g.dipole(trajs["fp64"])
+    for prec in ("fp32_3m", "bf16x3"):
+        d = np.diagonal(trajs[prec], axis1=1, axis2=2).imag
+        out[prec]["max_density_err_vs_fp64"] = float(np.abs(d - d64).max())
+        out[prec]["max_dipole_err_vs_fp64"] = float(np.abs(g.dipole(trajs[prec]) - p64).max())
+    return out
+
+
 def roofline(N, precision, k, steps):
     # dominant kernel: the tile pass with the larger summed time; one launch of
     # either pass applies two of the four one-particle terms of [H2,G2] to both
@@ -252,6 +279,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-trajectory ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=3)
     ap.add_argument("--no-extrapolator", action="store_true", help="skip the learned-extrapolator leg")
+    ap.add_argument("--mixed-steps", type=int, default=200,
+                    help="production steps of the mixed-precision error run (5000: the full workload)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {a.gpus} is not supported "
@@ -282,6 +311,8 @@ def main():
                                "max_density_err_vs_fp64": float(np.abs(np.diagonal(G1m).imag - d64).max()),
                                "err_horizon_steps": a.warmup + a.steps,
                                "roofline": roofline(N, prec, km, a.steps)}
+    if mixed and a.mixed_steps > 0:
+        mixed["error_run"] = mixed_error_run(g, a.mixed_steps)
     ensemble = None if a.no_ensemble else ensemble_leg(g, a.ensemble_steps, 1)
     extrap = None if a.no_extrapolator else extrapolator_leg(g)
     cpu = None
