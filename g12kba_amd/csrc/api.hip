@@ -235,6 +235,7 @@ int64_t layout(Run& r, char* base) {
         d.Y1 = cv.take<double2>(d.n2);
         d.h = cv.take<double2>(d.n2);
         d.I = cv.take<double2>(d.n2);
+        d.G1p = cv.take<double2>(2 * d.n2);
         d.Wxy = cv.take<double>(d.n2);
         d.Wxx = cv.take<double>(d.n2);
         // mixed-precision runs keep Y2 and R2 as complex64 (half the elements' bytes)
@@ -246,8 +247,6 @@ int64_t layout(Run& r, char* base) {
             d.R2[s] = cv.take<double2>(n4c);
             d.D[s] = cv.take<double2>(n3);
             d.F[s] = cv.take<double2>(n3);
-            d.Ft[s] = cv.take<double2>(n3);
-            d.Q[s] = cv.take<double2>(n3);
             d.Qp[s] = cv.take<double2>(n3);
         }
         d.traj_slots = r.window;
@@ -385,16 +384,18 @@ void timed(Run& r, int cls, F&& launch) {
 }
 
 // One RK4 stage of every trajectory (RK_solver.h:120-203 / AS.h:111-200):
-// G1 stage set-up, strided commutator pass (+ D), Q/F tables and collision
-// integral, G1 right-hand side and update, G2 sources (Psi, Pi, Pi^+) into the
-// strided-pass result, fused commutator + G2 update.
+// G1 stage set-up, Q/F tables and collision integral, strided commutator pass
+// with the sources whose (p1, q1) dependence is a vector (Psi W and Q terms,
+// Pi^+), G1 right-hand side and update, fused commutator pass with the
+// remaining sources (Psi Qp term, Pi) and the G2 update.
 void run_stage(Run& r, const StageRef& sr) {
     hipStream_t s = r.stream;
     timed(r, 2, [&] { g12::launch_b_stage_g1(r.b, sr, s); });
-    for (const Bucket& k : r.buckets) timed(r, 0, [&] { g12::launch_b_tile(r.b, k, sr, 0, r.precision, s); });
+    // the tables come from the stage G1 and the partial traces D of the stage
+    // input; both tile passes add their share of the sources from them
     if (r.any_corr) timed(r, 2, [&] { g12::launch_b_tables(r.b, sr, r.precision, s); });
+    for (const Bucket& k : r.buckets) timed(r, 0, [&] { g12::launch_b_tile(r.b, k, sr, 0, r.precision, s); });
     timed(r, 2, [&] { g12::launch_b_g1_rhs(r.b, sr, s); });
-    for (const Bucket& k : r.buckets) timed(r, 2, [&] { g12::launch_b_sources(r.b, k, sr, r.precision, s); });
     for (const Bucket& k : r.buckets) timed(r, 1, [&] { g12::launch_b_tile(r.b, k, sr, 1, r.precision, s); });
 }
 

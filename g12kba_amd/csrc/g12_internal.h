@@ -68,13 +68,14 @@ struct DevState {
     int Ns, Nb, N;
     int64_t n2, n3, n4;
     double2 *S1, *K1, *A1, *Y1, *h, *I;
+    double2* G1p;                      // pairs (Y1[X][Y], conj Y1[Y][X]) of the stage G1
     double *Wxy, *Wxx;                 // N x N interaction tables (xy, xx)
     // state, next stage input, RK accumulator, strided-pass result; in
     // mixed-precision runs Y2 and R2 hold complex64 elements
     double2 *S2[2], *Y2[2], *A2[2], *R2[2];
     double2 *D[2];                     // D[X][Y][Z] = T[X][Z][Y][Z] of the stage input
-    double2 *F[2], *Ft[2];             // F[X][Y][Z], Ft[Z][Y][X]
-    double2 *Q[2], *Qp[2];             // Q[Z][X][Y], Qp[X][Y][Z]
+    double2 *F[2];                     // F[X][Y][Z]
+    double2 *Qp[2];                    // Qp[X][Y][Z] = Q[Z][X][Y]
     int* fail;                         // 0 while running, step+1 after the guard trips
     double2* traj;                     // device trajectory ring (traj_slots x N^2) or null
     int traj_slots;
@@ -108,7 +109,6 @@ void launch_b_tables(const BatchDev& b, const StageRef& sr, int precision, hipSt
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_record(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int fused, int precision, hipStream_t s);
-void launch_b_sources(const BatchDev& b, const Bucket& k, const StageRef& sr, int precision, hipStream_t s);
 void launch_b_d_from_state(const BatchDev& b, const Bucket& k, hipStream_t s);
 void launch_b_advance_counter(int* counter, hipStream_t s);
 int device_cu_count();

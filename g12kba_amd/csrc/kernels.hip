@@ -229,7 +229,8 @@ __device__ inline bool active(const DevState& d, const TrajParams& t, const Stag
     return !(ss.as_phase && ss.step >= t.as_steps);
 }
 
-// Stage set-up for G1: Y1 = S1 + c K1, h = h_HF[Y1], W tables.  Blocks of
+// Stage set-up for G1: Y1 = S1 + c K1, the pairs (Y1[X][Y], conj Y1[Y][X])
+// read by the strided pass, h = h_HF[Y1], W tables.  Blocks of
 // 256 elements per trajectory; each block forms the diagonal of Y1 itself (for
 // the Hartree sums) and takes the Y1 elements of its Fock terms from S1 and
 // K1, so the blocks need no ordering among themselves.
@@ -262,7 +263,10 @@ __global__ __launch_bounds__(256) void k_stage_g1(BatchDev b, StageRef r) {
     const int e = e0 + threadIdx.x;
     if (e < n2) {
         const int X = e / N, Y = e % N;
-        gstore(d.Y1, e, yv(e));
+        const double2 y = yv(e);
+        gstore(d.Y1, e, y);
+        gstore(d.G1p, 2 * (int64_t)e, y);
+        gstore(d.G1p, 2 * (int64_t)e + 1, cconj(yv((int64_t)Y * N + X)));
         gstore(d.h, e, hf_element_with(yv, diag, X, Y, d.Ns, d.Nb, sp));
         const double w = w_xy(X, Y, d.Ns, sp);
         ((GLOBAL double*)d.Wxy)[e] = w;
@@ -271,8 +275,8 @@ __global__ __launch_bounds__(256) void k_stage_g1(BatchDev b, StageRef r) {
 }
 
 // ---------------------------------------------------------------------------
-// Q^s[Z][X][Y] = sum_w Y1[X][w] W^s[Z][w] Y1[w][Y] and Qp^s[X][Y][Z] = Q^s[Z][X][Y]
-// for s = xx (index 0, W_xx) and xy (index 1, W_xy).  One block per (Z, X),
+// Qp^s[X][Y][Z] = Q^s[Z][X][Y] = sum_w Y1[X][w] W^s[Z][w] Y1[w][Y] for
+// s = xx (index 0, W_xx) and xy (index 1, W_xy).  One block per (Z, X),
 // threads over Y.
 __device__ inline void q_tables_block(const DevState& d, int Z, int X) {
     __shared__ double2 u[G12_MAX_NB * 64];
@@ -293,10 +297,7 @@ __device__ inline void q_tables_block(const DevState& d, int Z, int X) {
             axy = cadd(axy, t);
             if (w != Z) axx = cadd(axx, t);
         }
-        const int64_t q = ((int64_t)Z * N + X) * N + Y;
         const int64_t qp = ((int64_t)X * N + Y) * N + Z;
-        d.Q[0][q] = axx;
-        d.Q[1][q] = axy;
         d.Qp[0][qp] = axx;
         d.Qp[1][qp] = axy;
     }
@@ -304,8 +305,8 @@ __device__ inline void q_tables_block(const DevState& d, int Z, int X) {
 
 // F^xx[X][Y][Z] = -i sum_w ( W_xy[Z][w] D^xy[X][Y][w] + W_xx[Z][w] D^xx[X][Y][w] )
 // F^xy[X][Y][Z] = -i sum_w ( W_xy[Z][w] D^xx[X][Y][w] + W_xx[Z][w] D^xy[X][Y][w] )
-// (Pi_xyxy with the swapped G2 pair, pi.h:25-95 / RK_solver.h:163-164), the
-// transposed copies Ft[Z][Y][X], and I[X][Y] = F^xx[X][Y][X]
+// (Pi_xyxy with the swapped G2 pair, pi.h:25-95 / RK_solver.h:163-164) and
+// I[X][Y] = F^xx[X][Y][X]
 // (collision_int, collision_integral.h:27-69).  One block per (X, Y).
 // With `herm` (mixed-precision runs) D enters through its Hermitian part
 // (D[X][Y][Z] + conj D[Y][X][Z]) / 2.  A Hermitian G2 (the physical states:
@@ -340,11 +341,8 @@ __device__ inline void f_tables_block(const DevState& d, int X, int Y, bool herm
         fxx = cmul_mi(fxx);
         fxy = cmul_mi(fxy);
         const int64_t f = ((int64_t)X * N + Y) * N + Z;
-        const int64_t ft = ((int64_t)Z * N + Y) * N + X;
         d.F[0][f] = fxx;
         d.F[1][f] = fxy;
-        d.Ft[0][ft] = fxx;
-        d.Ft[1][ft] = fxy;
         if (Z == X) d.I[(int64_t)X * N + Y] = fxx;
     }
 }
@@ -796,6 +794,7 @@ struct BTileArgs {
     int nspin;
     int use_y;        // stage input from the Y buffer after stage 0 (0: always S)
     int compact;      // R and Y held as complex64 (mixed-precision runs)
+    int sources;      // runs: each pass adds its share of the G2 sources (tile_sources)
     const double2* h_override;  // routine calls: h to use instead of the trajectory's h
 };
 
@@ -892,6 +891,9 @@ struct View {
     double2* R;         // unfused output / strided-pass result (compact in mixed runs)
     double2* D;         // fused: partial traces of the next stage input
     double2 *A, *Y;     // fused: RK accumulator, next stage input (compact in mixed runs)
+    const double2* G1p; // pairs (g[X][Y], conj g[Y][X]) of the stage G1 g
+    const double2 *Qp, *F;  // source tables of the spin
+    const double* W;        // W_xx (spin 0) or W_xy (spin 1)
 };
 
 __device__ inline const double2* stage_input(const DevState& d, const BTileArgs& a, const StageShared& ss, int spin) {
@@ -911,6 +913,10 @@ __device__ inline void load_view(View& v, const BTileArgs& a, const StageShared&
     v.xc = input_compact(a, ss);
     v.S = uni_ptr(d.S2[spin]);
     v.R = uni_ptr(d.R2[spin]);
+    v.G1p = uni_ptr(d.G1p);
+    v.Qp = uni_ptr(d.Qp[spin]);
+    v.F = uni_ptr(d.F[spin]);
+    v.W = uni_ptr(spin == 0 ? d.Wxx : d.Wxy);
     if constexpr (FUSED) {
         v.A = uni_ptr(d.A2[spin]);
         v.Y = uni_ptr(d.Y2[spin]);
@@ -965,6 +971,54 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
     }
 }
 
+// Sources of the G2 equation (psi.h:26-277, pi.h:25-95, G1_G2_t_step.h:55-86
+// through the partial-trace tables), for element (p1, p2, q1, q2) of spin s
+// with g the stage G1, g11 = g(p1,q1), g22 = g(p2,q2), W = W_xx (s = 0) or
+// W_xy (s = 1), Qp[X][Y][Z] = Q[Z][X][Y] and F the tables of the stage:
+//   Psi  = g11 g22 (W[p1][p2] - W[q1][q2]) + i g11 (Qp[p2][q2][p1] - Qp[p2][q2][q1])
+//          + i g22 (Qp[p1][q1][p2] - Qp[p1][q1][q2])
+//   Pi   = g22 (F[p1][q1][p2] - F[p1][q1][q2])
+//   Pi^+ = conj( g(q1,p1) (F[q2][p2][q1] - F[q2][p2][p1]) )
+// Each tile pass adds the terms that are differences of two vectors over its
+// own tile's rows and columns, so that no source needs a matrix per tile:
+//   strided tile, fixed (p2, q2), element (m, n) = (p1, q1):
+//     g11 (a[m] - b[n]) - conj(g(q1,p1)) (c[n] - c[m]),
+//     a[x] = g22 W[x][p2] + i Qp[p2][q2][x], b[x] = g22 W[x][q2] + i Qp[p2][q2][x],
+//     c[x] = conj F[q2][p2][x];
+//   fused tile, fixed (p1, q1), element (m, n) = (p2, q2):
+//     g22 (e[m] - e[n]),  e[x] = i Qp[p1][q1][x] + F[p1][q1][x].
+// g11 and conj g(q1,p1) (strided) and g22 (fused) are read per element from
+// the G1 pair table, which every tile shares; the vectors hold one element
+// per lane (N <= 64) and reach the lanes that need them through cross-lane
+// permutes.  The tables depend only on the stage G1 and on the partial traces
+// of the stage input, so the strided pass can run after them.
+struct SrcVec {
+    double2 q, f, g;  // Qp row, F row, g22 (strided only)
+    double w0, w1;    // W rows at p2 and q2 (strided only)
+};
+
+// Source vectors of tile t from the buffers of its (trajectory, spin) in v.
+template <bool FUSED>
+__device__ __forceinline__ void src_load(const View& v, const TileRef& t, int lane, SrcVec& sv) {
+    const int N = t.N;
+    const int t0 = t.tile / N, t1 = t.tile % N;
+    const int64_t N2 = (int64_t)N * N;
+    const int x = lane < N ? lane : 0;  // lanes past N read a valid element they never use
+    sv.q = gload(v.Qp, (int64_t)t0 * N2 + (int64_t)t1 * N + x);
+    if constexpr (FUSED) {
+        sv.f = gload(v.F, (int64_t)t0 * N2 + (int64_t)t1 * N + x);
+    } else {
+        sv.f = gload(v.F, (int64_t)t1 * N2 + (int64_t)t0 * N + x);
+        sv.w0 = gload1(v.W, (int64_t)t0 * N + x);  // W is symmetric
+        sv.w1 = gload1(v.W, (int64_t)t1 * N + x);
+        sv.g = gload(v.G1p, 2 * ((int64_t)t0 * N + t1));
+    }
+}
+
+__device__ __forceinline__ double2 lane_value(double2 v, int src) {
+    return make_double2(__shfl(v.x, src), __shfl(v.y, src));
+}
+
 template <int NP, int NW, int PREC, bool FUSED>
 __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     typedef Engine<NP, NW, PREC> E;
@@ -995,9 +1049,16 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     tile_prefetch<NP, NW, TILING>(a, tr, uni_ptr(stage_input(a.b.ds[tr.traj], a, ss, tr.spin)),
                                   input_compact(a, ss), pre);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // source vectors of the current tile: loaded after the previous tile's
+    // epilogue, used in this tile's
+    SrcVec sv{};
+    if (a.sources) {
+        View v0;
+        load_view<FUSED>(v0, a, ss, tr.traj, tr.spin);
+        src_load<FUSED>(v0, tr, lane, sv);
+    }
     while (gt >= 0) {
         const int N = tr.N;
-        const int64_t N2 = (int64_t)N * N;
         lds_sync();  // previous tile's readers of X (and h) are done
         if (tr.traj != v.traj || tr.spin != v.spin) {
             const bool new_h = tr.traj != v.traj;
@@ -1065,6 +1126,10 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
 
         if constexpr (!FUSED) {
             double2* out = v.R;
+            // strided-pass sources: g11 (va[m] - vb[n]) - conj g(q1,p1) (vc[n] - vc[m])
+            const double2 va = cadd(cscale(sv.g, sv.w0), cmul_i(sv.q));
+            const double2 vb = cadd(cscale(sv.g, sv.w1), cmul_i(sv.q));
+            const double2 vc = cconj(sv.f);
             auto store = [&](auto compact) {
                 constexpr bool C = decltype(compact)::value;
 #pragma unroll
@@ -1072,12 +1137,27 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                     const int t = wave + NW * u;
                     if (t >= TT) continue;
                     const int n = (t % NT) * 16 + (lane & 15);
+                    double2 bn{}, cn{};
+                    if (a.sources) {  // permutes in wave-uniform control flow
+                        bn = lane_value(vb, n);
+                        cn = lane_value(vc, n);
+                    }
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
+                        double2 am{}, cm{};
+                        if (a.sources) {
+                            am = lane_value(va, m);
+                            cm = lane_value(vc, m);
+                        }
                         if (m < N && n < N) {
                             const int64_t off = base + m * sm + n * sn;
                             double2 val = make_double2(ar[u][j], ai[u][j]);
+                            if (a.sources) {
+                                const int64_t gi = 2 * ((int64_t)m * N + n);
+                                const double2 g11 = gload(v.G1p, gi), gt = gload(v.G1p, gi + 1);
+                                val = cadd(val, csub(cmul(g11, csub(am, bn)), cmul(gt, csub(cn, cm))));
+                            }
                             if (a.accumulate) val = cadd(cload(out, off, C), val);
                             cstore(out, off, val, C);
                         }
@@ -1090,9 +1170,11 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                 store(std::false_type{});
         } else {
             // fused update for tile (p1, q1) = (t0, t1), element (p2, q2) = (m, n):
-            // R already holds the strided-pass terms plus the sources (k_sources),
-            // K = -i (R + O); then A += w K and the next stage input
-            // Y = S + c_next K, or at the last stage S += dt (A + w K) / 6.
+            // R holds the strided-pass terms and their sources; with this
+            // pass's sources g22 (ve[m] - ve[n]), K = -i (R + O + sources);
+            // then A += w K and the next stage input Y = S + c_next K, or at
+            // the last stage S += dt (A + w K) / 6.
+            const double2 ve = cadd(cmul_i(sv.q), sv.f);
             auto update = [&](auto compact) {
                 constexpr bool C = decltype(compact)::value;
 #pragma unroll
@@ -1100,12 +1182,16 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                     const int t = wave + NW * u;
                     if (t >= TT) continue;
                     const int n = (t % NT) * 16 + (lane & 15);
+                    const double2 en = a.sources ? lane_value(ve, n) : double2{};
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
+                        const double2 em = a.sources ? lane_value(ve, m) : double2{};
                         if (m >= N || n >= N) continue;
                         const int64_t off = base + m * sm + n * sn;
-                        const double2 K = cmul_mi(cadd(rpre[u][j], make_double2(ar[u][j], ai[u][j])));
+                        double2 rr = cadd(rpre[u][j], make_double2(ar[u][j], ai[u][j]));
+                        if (a.sources) rr = cadd(rr, cmul(gload(v.G1p, 2 * ((int64_t)m * N + n)), csub(em, en)));
+                        const double2 K = cmul_mi(rr);
                         const double2 wK = cscale(K, ss.wk);
                         const double2 s = spre[u][j];
                         double2 next;  // input of the next stage (the new state after the last)
@@ -1129,92 +1215,17 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
             else
                 update(std::false_type{});
         }
+        if (a.sources && next >= 0) {
+            if (nt.traj == v.traj && nt.spin == v.spin) {
+                src_load<FUSED>(v, nt, lane, sv);
+            } else {
+                View vn;
+                load_view<FUSED>(vn, a, ss, nt.traj, nt.spin);
+                src_load<FUSED>(vn, nt, lane, sv);
+            }
+        }
         gt = next;
         tr = nt;
-    }
-}
-
-// Sources of the G2 equation added to the strided-pass result, for the slice
-// (p1, q1) of both spins of one trajectory (element (p2, q2) = (m, n)):
-//   Psi  = g11 g22 (W[p1][p2] - W[q1][q2]) + i g11 (Q[p1][p2][q2] - Q[q1][p2][q2])
-//          + i g22 (Qp[p1][q1][p2] - Qp[p1][q1][q2])
-//   Pi   = g22 (F[p1][q1][p2] - F[p1][q1][q2])
-//   Pi^+ = conj( g(q1,p1) (Ft[q1][p2][q2] - Ft[p1][p2][q2]) )
-//   R   += Psi + Pi - Pi^+
-// with g = the stage G1, g11 = g(p1,q1), g22 = g(p2,q2), W = W_xx (spin 0)
-// or W_xy (spin 1) (psi.h:26-277, pi.h:25-95, G1_G2_t_step.h:55-86 through
-// the partial-trace tables).  A streaming kernel: its table and state loads
-// are hidden by occupancy instead of stalling the MFMA pass.  The slice's
-// vectors (W rows, Qp and F rows) are staged in LDS; G1 and W are shared by
-// the two spins.  One block per slice; block b is the bucket's tile b (spin 0
-// tiles only).
-__global__ __launch_bounds__(256) void k_sources(BTileArgs a) {
-    __shared__ double wp[G12_MAX_NB * 64], wq[G12_MAX_NB * 64];
-    __shared__ double2 qv[2][G12_MAX_NB * 64], fv[2][G12_MAX_NB * 64];
-    const StageShared ss = uni_stage(a.r);
-    // slice b of the bucket (listed trajectory j, tile t < N^2 of spin 0);
-    // each XCD takes a contiguous range of slices, so the slices that share
-    // p1 (and with it the Q and Ft rows of p1) stay in one L2
-    const int64_t nsl = a.k.tiles / 2, per_xcd = (nsl + 7) / 8;
-    const int64_t b = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (b >= nsl) return;
-    int lo = 0, hi = a.k.n - 1;
-    while (lo < hi) {  // last listed trajectory whose slices start at or before b
-        const int mid = (lo + hi + 1) >> 1;
-        if (a.k.prefix[mid] / 2 <= b) lo = mid; else hi = mid - 1;
-    }
-    const int traj = uni(a.k.list[lo]);
-    const DevState& d = a.b.ds[traj];
-    if (!active(d, a.b.tp[traj], ss)) return;
-    const int N = a.b.tp[traj].N;
-    const int tile = (int)(b - a.k.prefix[lo] / 2);
-    if (tile >= N * N) return;
-    const int64_t N2 = (int64_t)N * N;
-    const int p1 = tile / N, q1 = tile % N;
-    const double2* G1 = uni_ptr(d.Y1);
-    const double* Wxy = uni_ptr(d.Wxy);
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        wp[i] = gload1(Wxy, (int64_t)p1 * N + i);
-        wq[i] = gload1(Wxy, (int64_t)q1 * N + i);
-        for (int s = 0; s < 2; ++s) {
-            qv[s][i] = gload(d.Qp[s], ((int64_t)p1 * N + q1) * N + i);
-            fv[s][i] = gload(d.F[s], ((int64_t)p1 * N + q1) * N + i);
-        }
-    }
-    __syncthreads();
-    const double2 g11 = gload(G1, (int64_t)p1 * N + q1);
-    const double2 gq1p1 = gload(G1, (int64_t)q1 * N + p1);
-    const double2* Q0 = uni_ptr(d.Q[0]);
-    const double2* Q1 = uni_ptr(d.Q[1]);
-    const double2* T0 = uni_ptr(d.Ft[0]);
-    const double2* T1 = uni_ptr(d.Ft[1]);
-    const bool cp = a.compact != 0;
-    const int64_t rbase = (int64_t)p1 * N2 * N + (int64_t)q1 * N;  // element (p1, 0, q1, 0)
-    void* R0 = cp ? (void*)((float2*)uni_ptr(d.R2[0]) + rbase) : (void*)(uni_ptr(d.R2[0]) + rbase);
-    void* R1 = cp ? (void*)((float2*)uni_ptr(d.R2[1]) + rbase) : (void*)(uni_ptr(d.R2[1]) + rbase);
-    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
-        const int m = e / N, n = e % N;
-        const int64_t off = (int64_t)m * N2 + n;
-        const double2 r0 = cload(R0, off, cp), r1 = cload(R1, off, cp);
-        const double2 g22 = gload(G1, e);
-        const double2 gg = cmul(g11, g22);
-        // W_xx differs from W_xy only on its diagonal (zero)
-        const double wxy = wp[m] - wq[n];
-        const double wxx = (m == p1 ? 0.0 : wp[m]) - (n == q1 ? 0.0 : wq[n]);
-        const double2* Qs[2] = {Q0, Q1};
-        const double2* Ts[2] = {T0, T1};
-        void* Rs[2] = {R0, R1};
-        const double2 rs[2] = {r0, r1};
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            double2 psi = cscale(gg, s == 0 ? wxx : wxy);
-            psi = cadd(psi, cmul_i(cmul(g11, csub(gload(Qs[s], (int64_t)p1 * N2 + e), gload(Qs[s], (int64_t)q1 * N2 + e)))));
-            psi = cadd(psi, cmul_i(cmul(g22, csub(qv[s][m], qv[s][n]))));
-            const double2 pi = cmul(g22, csub(fv[s][m], fv[s][n]));
-            const double2 pid =
-                cconj(cmul(gq1p1, csub(gload(Ts[s], (int64_t)q1 * N2 + e), gload(Ts[s], (int64_t)p1 * N2 + e))));
-            cstore(Rs[s], off, cadd(rs[s], csub(cadd(psi, pi), pid)), cp);
-        }
     }
 }
 
@@ -1516,19 +1527,6 @@ void launch_b_d_from_state(const BatchDev& b, const Bucket& k, hipStream_t s) {
     hipLaunchKernelGGL(k_d_from_state, dim3((unsigned)(k.tiles / 2)), dim3(64), 0, s, a);
 }
 
-void launch_b_sources(const BatchDev& b, const Bucket& k, const StageRef& r, int precision, hipStream_t s) {
-    if (k.tiles <= 0) return;
-    BTileArgs a{};
-    a.b = b;
-    a.k = k;
-    a.r = r;
-    a.compact = precision != G12_PREC_FP64;
-    // one block per (p1, q1) slice: half the bucket's tiles (two spins each),
-    // rounded up to whole XCD ranges
-    const int64_t nsl = k.tiles / 2, per_xcd = (nsl + 7) / 8;
-    hipLaunchKernelGGL(k_sources, dim3((unsigned)(8 * per_xcd)), dim3(256), 0, s, a);
-}
-
 void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fused, int precision, hipStream_t s) {
     BTileArgs a{};
     a.b = b;
@@ -1545,6 +1543,7 @@ void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fu
     a.nspin = 2;
     a.use_y = 1;
     a.compact = precision != G12_PREC_FP64;
+    a.sources = 1;
     a.h_override = nullptr;
     launch_tile(a, precision, s);
 }

@@ -23,3 +23,19 @@ def oracle_cfg(p):
 
 # (Ns, Nb, nV) geometries for routine-level parity; nV long-range shells
 GEOMS = [(2, 1, 0), (3, 2, 2), (2, 3, 1), (4, 2, 3), (5, 1, 4), (3, 3, 2)]
+
+
+def physical_g2(rng, Ns, Nb, scale=1e-3):
+    """Random G2 in the reference layout with the symmetries of a G2 evolved
+    from zero: Hermitian as an N^2 x N^2 matrix and symmetric under exchanging
+    both particle pairs, T[p1][p2][q1][q2] = T[p2][p1][q2][q1] (reference pair
+    index of (p1, p2) with p = site + orb*Ns: i + j*Ns + (a + b*Nb)*Ns^2)."""
+    N = Ns * Nb
+    site, orb = np.arange(N) % Ns, np.arange(N) // Ns
+    pair = (site[:, None] + site[None, :] * Ns + (orb[:, None] + orb[None, :] * Nb) * Ns * Ns).ravel()
+    A = rand_c(rng, N * N, N * N, scale=scale)
+    T = (A + A.conj().T).reshape(N, N, N, N)
+    T = 0.5 * (T + T.transpose(1, 0, 3, 2))
+    M = np.empty((N * N, N * N), np.complex128)
+    M[np.ix_(pair, pair)] = T.reshape(N * N, N * N)
+    return M

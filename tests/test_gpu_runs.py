@@ -170,3 +170,48 @@ def test_n64_invariants(prec, tol):
         T = fast.ref_to_nat(G, Ns, Nb).reshape(N, N, N, N)
         assert np.abs(T - T.transpose(1, 0, 3, 2)).max() <= tol * scale
         del T
+
+
+def n64_params(prec):
+    return g.make_params(Ns=32, Nb=2, t1=1.0, t2=0.5, U=1.5, dt=0.01, time_steps=2, quench_strength=1.0, Nq=8,
+                         q_type="full", EHM=True, V_law="coulomb", quench_rate=2.0, quench_on_midpoint=30.0,
+                         AS_rate=0.5, AS_midpoint=10.0, precision=prec)
+
+
+@pytest.fixture(scope="module")
+def n64_oracle_run():
+    """The N = 64 workload (Ns = 32, Nb = 2, EHM with V_n = U/(1+n), full
+    quench ramp) from a random physical state, two RK4 iterations through the
+    factorised C++ oracle (about ten seconds on a multi-core host)."""
+    from oracle.oracle import Oracle
+    from tests.helpers import oracle_cfg, physical_g2
+    rng = np.random.default_rng(64)
+    p = n64_params("fp64")
+    state = (g.make_G0(p), physical_g2(rng, 32, 2), physical_g2(rng, 32, 2))
+    ref = Oracle().fast_run(oracle_cfg(p), *state, V_profile=list(g.v_profile(p)), run_as=False,
+                            traj_every=1, traj_cap=8)
+    assert ref["rc"] == 0
+    return p, state, ref
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_3m", "bf16x3"])
+def test_n64_workload_matches_oracle(n64_oracle_run, prec):
+    """Full-size parity: G1 and both G2 tensors after two RK4 iterations of
+    the N = 64 workload.  FP64 within 1e-10 relative of the oracle; the mixed
+    modes within 1e-6 on the G1 densities and 1e-4 relative on G2."""
+    _, state, ref = n64_oracle_run
+    p = n64_params(prec)
+    with g.Propagator(p) as prop:
+        prop.set_state(*state)
+        rc, traj = prop.rk4()
+        G1, Gxx, Gxy = prop.get_state()
+    assert rc == 0 and len(traj) == ref["rk_steps"] == len(ref["traj"])
+    if prec == "fp64":
+        assert relerr(G1, ref["G1"]) < 1e-10 and relerr(traj, ref["traj"]) < 1e-10
+        assert relerr(Gxx, ref["Gxx"]) < 1e-10 and relerr(Gxy, ref["Gxy"]) < 1e-10
+    else:
+        dens = np.abs(np.diagonal(G1 - ref["G1"]).imag).max()
+        assert dens < 1e-6
+        assert relerr(Gxx, ref["Gxx"]) < 1e-4 and relerr(Gxy, ref["Gxy"]) < 1e-4
+    tr = np.trace(traj, axis1=1, axis2=2)
+    assert np.abs(tr - tr[0]).max() < 1e-12
