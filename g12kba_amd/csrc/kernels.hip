@@ -76,6 +76,35 @@ __device__ __forceinline__ void cstore(void* p, int64_t i, double2 v, bool compa
         gstore((double2*)p, i, v);
 }
 
+// The same accesses with 32-bit element offsets from a base pointer held in
+// scalar registers.  The compiler emits them in the global saddr form (scalar
+// base + 32-bit byte offset per lane), one VGPR per address instead of two
+// and no 64-bit address arithmetic.  Valid for buffers below 4 GB: the tile
+// kernels index N^4 complex128 elements with N <= 64 (268 MB).
+__device__ __forceinline__ double2 gld(const double2* p, uint32_t i) {
+    const dv2 v = *(const GLOBAL dv2*)((const GLOBAL char*)p + (i << 4));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void gst(double2* p, uint32_t i, double2 v) {
+    *(GLOBAL dv2*)((GLOBAL char*)p + (i << 4)) = dv2{v.x, v.y};
+}
+__device__ __forceinline__ double gld1(const double* p, uint32_t i) {
+    return *(const GLOBAL double*)((const GLOBAL char*)p + (i << 3));
+}
+__device__ __forceinline__ double2 cld(const void* p, uint32_t i, bool compact) {
+    if (compact) {
+        const fv2 v = *(const GLOBAL fv2*)((const GLOBAL char*)p + (i << 3));
+        return make_double2(v.x, v.y);
+    }
+    return gld((const double2*)p, i);
+}
+__device__ __forceinline__ void cst(void* p, uint32_t i, double2 v, bool compact) {
+    if (compact)
+        *(GLOBAL fv2*)((GLOBAL char*)p + (i << 3)) = fv2{(float)v.x, (float)v.y};
+    else
+        gst((double2*)p, i, v);
+}
+
 // Values that are the same in every lane, moved to scalar registers: the
 // trajectory descriptors are read with vector loads (the kernel writes global
 // memory, so scalar loads of them are not allowed), and left in vector
@@ -924,17 +953,18 @@ __device__ inline void load_view(View& v, const BTileArgs& a, const StageShared&
     }
 }
 
-__device__ __forceinline__ void tile_geometry(int tiling, int N, int tile, int64_t& base, int64_t& sm, int64_t& sn,
-                                              int& t0, int& t1) {
-    const int64_t N2 = (int64_t)N * N, N3 = N2 * N;
+// Element offsets of a tile within one tensor (32-bit: N <= 64).
+__device__ __forceinline__ void tile_geometry(int tiling, int N, int tile, uint32_t& base, uint32_t& sm,
+                                              uint32_t& sn, int& t0, int& t1) {
+    const uint32_t n = (uint32_t)N, N2 = n * n, N3 = N2 * n;
     t0 = tile / N;
     t1 = tile % N;
     if (tiling == 0) {  // strided: fixed (p2, q2) = (t0, t1); m = p1, n = q1
-        base = (int64_t)t0 * N2 + t1;
+        base = (uint32_t)t0 * N2 + t1;
         sm = N3;
-        sn = N;
+        sn = n;
     } else {  // contiguous: fixed (p1, q1) = (t0, t1); m = p2, n = q2
-        base = (int64_t)t0 * N3 + (int64_t)t1 * N;
+        base = (uint32_t)t0 * N3 + (uint32_t)t1 * n;
         sm = N2;
         sn = 1;
     }
@@ -948,7 +978,7 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
                                               double2 (&pre)[TileCfg<NP, NW>::PF]) {
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
     const int N = t.N;
-    int64_t base, sm, sn;
+    uint32_t base, sm, sn;
     int t0, t1;
     tile_geometry(TILING < 0 ? a.tiling : TILING, N, t.tile, base, sm, sn, t0, t1);
     // one branch on the storage form, outside the unrolled loads
@@ -957,7 +987,7 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
             double2 y = make_double2(0.0, 0.0);
-            if (e < N * N) y = cload(X, base + (e / N) * sm + (e % N) * sn, true);
+            if (e < N * N) y = cld(X, base + (uint32_t)(e / N) * sm + (uint32_t)(e % N) * sn, true);
             pre[i] = y;
         }
     } else {
@@ -965,7 +995,7 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
             double2 y = make_double2(0.0, 0.0);
-            if (e < N * N) y = gload(X, base + (e / N) * sm + (e % N) * sn);
+            if (e < N * N) y = gld(X, base + (uint32_t)(e / N) * sm + (uint32_t)(e % N) * sn);
             pre[i] = y;
         }
     }
@@ -1002,16 +1032,16 @@ template <bool FUSED>
 __device__ __forceinline__ void src_load(const View& v, const TileRef& t, int lane, SrcVec& sv) {
     const int N = t.N;
     const int t0 = t.tile / N, t1 = t.tile % N;
-    const int64_t N2 = (int64_t)N * N;
-    const int x = lane < N ? lane : 0;  // lanes past N read a valid element they never use
-    sv.q = gload(v.Qp, (int64_t)t0 * N2 + (int64_t)t1 * N + x);
+    const uint32_t n = (uint32_t)N;
+    const uint32_t x = lane < N ? lane : 0;  // lanes past N read a valid element they never use
+    sv.q = gld(v.Qp, ((uint32_t)t0 * n + t1) * n + x);
     if constexpr (FUSED) {
-        sv.f = gload(v.F, (int64_t)t0 * N2 + (int64_t)t1 * N + x);
+        sv.f = gld(v.F, ((uint32_t)t0 * n + t1) * n + x);
     } else {
-        sv.f = gload(v.F, (int64_t)t1 * N2 + (int64_t)t0 * N + x);
-        sv.w0 = gload1(v.W, (int64_t)t0 * N + x);  // W is symmetric
-        sv.w1 = gload1(v.W, (int64_t)t1 * N + x);
-        sv.g = gload(v.G1p, 2 * ((int64_t)t0 * N + t1));
+        sv.f = gld(v.F, ((uint32_t)t1 * n + t0) * n + x);
+        sv.w0 = gld1(v.W, (uint32_t)t0 * n + x);  // W is symmetric
+        sv.w1 = gld1(v.W, (uint32_t)t1 * n + x);
+        sv.g = gld(v.G1p, 2 * ((uint32_t)t0 * n + t1));
     }
 }
 
@@ -1075,7 +1105,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                 lds_sync();
             }
         }
-        int64_t base, sm, sn;
+        uint32_t base, sm, sn;
         int t0, t1;
         tile_geometry(TILING < 0 ? a.tiling : TILING, N, tr.tile, base, sm, sn, t0, t1);
 #pragma unroll
@@ -1108,9 +1138,9 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
                         rpre[u][j] = spre[u][j] = make_double2(0.0, 0.0);
                         if (t < TT && m < N && n < N) {
-                            const int64_t off = base + m * sm + n * sn;
-                            rpre[u][j] = cload(v.R, off, decltype(compact)::value);
-                            spre[u][j] = gload(v.S, off);
+                            const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
+                            rpre[u][j] = cld(v.R, off, decltype(compact)::value);
+                            spre[u][j] = gld(v.S, off);
                         }
                     }
                 }
@@ -1151,15 +1181,15 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                             cm = lane_value(vc, m);
                         }
                         if (m < N && n < N) {
-                            const int64_t off = base + m * sm + n * sn;
+                            const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
                             double2 val = make_double2(ar[u][j], ai[u][j]);
                             if (a.sources) {
-                                const int64_t gi = 2 * ((int64_t)m * N + n);
-                                const double2 g11 = gload(v.G1p, gi), gt = gload(v.G1p, gi + 1);
+                                const uint32_t gi = 2 * (uint32_t)(m * N + n);
+                                const double2 g11 = gld(v.G1p, gi), gt = gld(v.G1p, gi + 1);
                                 val = cadd(val, csub(cmul(g11, csub(am, bn)), cmul(gt, csub(cn, cm))));
                             }
-                            if (a.accumulate) val = cadd(cload(out, off, C), val);
-                            cstore(out, off, val, C);
+                            if (a.accumulate) val = cadd(cld(out, off, C), val);
+                            cst(out, off, val, C);
                         }
                     }
                 }
@@ -1188,25 +1218,25 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
                         const double2 em = a.sources ? lane_value(ve, m) : double2{};
                         if (m >= N || n >= N) continue;
-                        const int64_t off = base + m * sm + n * sn;
+                        const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
                         double2 rr = cadd(rpre[u][j], make_double2(ar[u][j], ai[u][j]));
-                        if (a.sources) rr = cadd(rr, cmul(gload(v.G1p, 2 * ((int64_t)m * N + n)), csub(em, en)));
+                        if (a.sources) rr = cadd(rr, cmul(gld(v.G1p, 2 * (uint32_t)(m * N + n)), csub(em, en)));
                         const double2 K = cmul_mi(rr);
                         const double2 wK = cscale(K, ss.wk);
                         const double2 s = spre[u][j];
                         double2 next;  // input of the next stage (the new state after the last)
                         if (!ss.last) {
-                            gstore(v.A, off, ss.first ? wK : cadd(gload(v.A, off), wK));
+                            gst(v.A, off, ss.first ? wK : cadd(gld(v.A, off), wK));
                             next = cadd(s, cscale(K, ss.cn));
-                            cstore(v.Y, off, next, C);
+                            cst(v.Y, off, next, C);
                         } else {
-                            const double2 A = cadd(gload(v.A, off), wK);
+                            const double2 A = cadd(gld(v.A, off), wK);
                             next = cadd(s, cdiv6(cscale(A, ss.dt)));
-                            gstore(v.S, off, next);
+                            gst(v.S, off, next);
                         }
                         // partial trace of the next stage input, FP64:
                         // D[X=p1][Y=q1][Z=p2] for p2 = q2
-                        if (m == n) gstore(v.D, ((int64_t)t0 * N + t1) * N + m, next);
+                        if (m == n) gst(v.D, (uint32_t)((t0 * N + t1) * N + m), next);
                     }
                 }
             };
