@@ -92,6 +92,9 @@ def _load():
         "g12_g1_step": [p, p, p, p, d, i, i],
         "g12_g2_step": [p, p, p, p, p, d, i, i],
         "g12_dipole": [p, p, i64, i],
+        "g12_energy": [ctypes.POINTER(Params), d, d, p, p, p, p],
+        "g12_state_energy": [vp, p],
+        "g12_batch_state_energy": [vp, i, p],
         "g12_write_outputs": [ctypes.c_char_p, p, i64, i, p, d],
         "g12_save_state": [vp, ctypes.c_char_p],
         "g12_load_state": [vp, ctypes.c_char_p],
@@ -151,7 +154,7 @@ EXPORTED = [
     "g12_get_time", "g12_rk4_record", "g12_timing_enable", "g12_timing_get", "g12_memory_bytes", "g12_synchronize",
     "g12_hf_hamiltonian", "g12_h1_commutator", "g12_h2_commutator", "g12_collision",
     "g12_psi_xxxx", "g12_psi_xyxy", "g12_pi", "g12_g1_step", "g12_g2_step", "g12_fermi",
-    "g12_dipole", "g12_write_outputs", "g12_save_state", "g12_load_state", "g12_version",
+    "g12_dipole", "g12_energy", "g12_state_energy", "g12_batch_state_energy", "g12_write_outputs", "g12_save_state", "g12_load_state", "g12_version",
     "g12_configure", "g12_batch_create", "g12_batch_destroy", "g12_batch_size",
     "g12_batch_memory_bytes", "g12_batch_set_state", "g12_batch_get_state",
     "g12_batch_adiabatic_switching", "g12_batch_rk4", "g12_batch_rk4_advance", "g12_batch_rk4_record", "g12_batch_guard",
@@ -393,6 +396,23 @@ def dipole(traj):
     return p
 
 
+ENERGY_PARTS = ("one_body", "hartree", "exchange", "correlation")
+
+
+def energy(p, G1, G2xx=None, G2xy=None, lam=1.0, w=0.0):
+    """Energy of a state in the reference layouts, both spins: array of
+    (one-body, Hartree, exchange, correlation); the total is their sum.  lam
+    scales U and every V_n; w is the quench shift on sites < Nq.  Without G2
+    the correlation part is 0."""
+    N = p.Ns * p.Nb
+    G1 = _c(G1, (N, N))
+    Gxx = None if G2xx is None else _c(G2xx, (N ** 2, N ** 2))
+    Gxy = None if G2xy is None else _c(G2xy, (N ** 2, N ** 2))
+    E = np.zeros(4)
+    _check(lib.g12_energy(ctypes.byref(p), lam, w, _p(G1), _p(Gxx), _p(Gxy), _p(E)), "g12_energy")
+    return E
+
+
 def write_outputs(directory, traj, dip, progress=-1.0):
     traj = _c(traj)
     dip = np.ascontiguousarray(dip, dtype=np.float64)
@@ -482,6 +502,13 @@ class Propagator:
         n = ctypes.c_int64(0)
         lib.g12_get_time(self.h, ctypes.byref(t), ctypes.byref(n))
         return t.value, n.value
+
+    def energy(self):
+        """(one-body, Hartree, exchange, correlation) of the current state at full
+        U and the quench shift of the current time."""
+        E = np.zeros(4)
+        _check(lib.g12_state_energy(self.h, _p(E)), "g12_state_energy")
+        return E
 
     def timing(self, on=True):
         lib.g12_timing_enable(self.h, int(on))
@@ -623,6 +650,12 @@ class Batch:
 
     def tune(self):
         _check(lib.g12_batch_tune(self.h), "g12_batch_tune")
+
+    def energy(self, i):
+        """Propagator.energy of trajectory i."""
+        E = np.zeros(4)
+        _check(lib.g12_batch_state_energy(self.h, i, _p(E)), "g12_batch_state_energy")
+        return E
 
     def buckets(self):
         """[(NP, trajectories, waves_strided, waves_fused)] of the tile passes."""

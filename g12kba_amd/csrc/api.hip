@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -567,8 +568,18 @@ int run_as(Run& r, int* steps) {
     return 0;
 }
 
-// Stage blocks of production step n at time t (RK_solver.h:120-151): quench
-// shift phi * strength on sites < Nq once the stage time reaches the gate.
+// Quench ramp at stage time ts (RK_solver.h:141-151): 0 before the gate,
+// else the switch-on logistic (times the switch-off one for a pulse); the
+// shift on sites < Nq is phi * quench_strength.
+double quench_phi(const g12_params& p, double ts, int* gate) {
+    *gate = 0;
+    if (ts < p.quench_gate_time) return 0.0;
+    *gate = 1;
+    const double on = g12_fermi(ts, p.quench_on_midpoint, p.quench_rate);
+    return p.q_type == G12_QUENCH_PULSE ? on * (1 - g12_fermi(ts, p.quench_off_midpoint, p.quench_rate)) : on;
+}
+
+// Stage blocks of production step n at time t (RK_solver.h:120-151).
 void production_blocks(const g12_params& p, double t, int64_t n, int slot, StageShared* blk) {
     for (int k = 0; k < 4; ++k) {
         const double dtk = (k == 1 || k == 2) ? p.dt / 2 : p.dt;
@@ -576,12 +587,7 @@ void production_blocks(const g12_params& p, double t, int64_t n, int slot, Stage
         blk[k] = stage_block(k, p.dt);
         blk[k].step = (int)n;
         blk[k].slot = slot;
-        if (ts >= p.quench_gate_time) {
-            const double on = g12_fermi(ts, p.quench_on_midpoint, p.quench_rate);
-            blk[k].gate = 1;
-            blk[k].phi = p.q_type == G12_QUENCH_PULSE ? on * (1 - g12_fermi(ts, p.quench_off_midpoint, p.quench_rate))
-                                                      : on;
-        }
+        blk[k].phi = quench_phi(p, ts, &blk[k].gate);
     }
 }
 
@@ -980,6 +986,83 @@ int check_routine_geometry(int Ns, int Nb, int nV) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// energy.  No reference counterpart (README.md:61-63 lists energies as future
+// work).  With rho = -i G1 per spin (the reference's Hartree term -iU G1_XX
+// is +U n_X) and the interaction tables W_xx, W_xy scaled by lam:
+//   one-body     2 Re sum_XY h0[X][Y] rho[Y][X]   (hopping, eps, quench shift w
+//                on sites < Nq: the non-interacting part of h_HFx_quench, HF.h:57-81)
+//   Hartree      Re sum_XY (W_xx + W_xy)(X,Y) rho[X][X] rho[Y][Y]
+//   exchange     - sum_XY W_xx(X,Y) |rho[X][Y]|^2
+//   correlation  - Re sum_XY ( W_xx(X,Y) M^xx[(X,Y);(X,Y)] + W_xy(X,Y) M^xy[(X,Y);(X,Y)] )
+// The weight -1 of the correlation term is the one for which the equations of
+// motion (G1_G2_t_step.h:24-86) conserve the total: for a time-independent h0
+// the total is constant along the exact dynamics when the Fock term is the
+// true exchange.  The reference's onsite
+// inter-orbital Fock term takes the transposed G1 element (HF.h:83), so for
+// Nb >= 2 the total drifts once those elements become complex; for Nb = 1 it
+// is conserved to the integrator's error.
+template <class Diag>
+void energy_parts(const g12_params& p, double lam, double w, const double* G1_, Diag diag, bool corr, double* E) {
+    typedef std::complex<double> cplx;
+    const cplx* G1 = reinterpret_cast<const cplx*>(G1_);
+    const int Ns = p.Ns, Nb = p.Nb, N = Ns * Nb;
+    double prof[G12_MAX_SHELLS];
+    const int nV = v_profile(p, prof);
+    auto wxy = [&](int X, int Y) {
+        int d = X % Ns - Y % Ns;
+        d = d < 0 ? -d : d;
+        if (d == 0) return lam * p.U;
+        return d <= nV ? lam * p.U * prof[d - 1] : 0.0;
+    };
+    auto rho = [&](int X, int Y) { return cplx(0, -1) * G1[(int64_t)X * N + Y]; };
+    double e1 = 0, eh = 0, ef = 0, ec = 0;
+    for (int X = 0; X < N; ++X) {
+        const int sx = X % Ns, ox = X / Ns;
+        // h0 row X: eps + quench shift, -t1 to the neighbouring sites, -t2 to the adjacent orbitals
+        double diag0 = p.eps[ox] + (sx < p.Nq ? w : 0.0);
+        e1 += 2 * (diag0 * rho(X, X)).real();
+        if (sx > 0) e1 += 2 * (-p.t1 * rho(X - 1, X)).real();
+        if (sx + 1 < Ns) e1 += 2 * (-p.t1 * rho(X + 1, X)).real();
+        if (ox > 0) e1 += 2 * (-p.t2 * rho(X - Ns, X)).real();
+        if (ox + 1 < Nb) e1 += 2 * (-p.t2 * rho(X + Ns, X)).real();
+        for (int Y = 0; Y < N; ++Y) {
+            const double a = wxy(X, Y), b = X == Y ? 0.0 : a;
+            if (a == 0.0) continue;
+            eh += ((a + b) * rho(X, X) * rho(Y, Y)).real();
+            ef -= b * std::norm(rho(X, Y));
+            if (corr) ec -= (b * diag(0, X, Y) + a * diag(1, X, Y)).real();
+        }
+    }
+    E[0] = e1;
+    E[1] = eh;
+    E[2] = ef;
+    E[3] = ec;
+}
+
+int state_energy(Run& r, int i, double* E) {
+    if (i < 0 || i >= r.ntraj || !E) return G12_E_ARG;
+    const DevState& d = r.ds[i];
+    const g12_params& p = r.p[i];
+    const bool corr = r.tp[i].corr != 0;
+    std::vector<std::complex<double>> g1(d.n2), pd(corr ? 2 * d.n2 : 0);
+    double2* tmp = nullptr;
+    if (corr) {
+        HIPCHK(hipMalloc(&tmp, sizeof(double2) * 2 * d.n2));
+        for (int s = 0; s < 2; ++s) g12::launch_pair_diag(d.S2[s], tmp + s * d.n2, d.N, r.stream);
+        HIPCHK(hipMemcpyAsync(pd.data(), tmp, sizeof(double2) * 2 * d.n2, hipMemcpyDeviceToHost, r.stream));
+    }
+    HIPCHK(hipMemcpyAsync(g1.data(), d.S1, sizeof(double2) * d.n2, hipMemcpyDeviceToHost, r.stream));
+    const hipError_t e = hipStreamSynchronize(r.stream);
+    if (tmp) hipFree(tmp);
+    if (e != hipSuccess) return G12_E_HIP;
+    int gate = 0;
+    const double w = quench_phi(p, r.t, &gate) * p.quench_strength;
+    energy_parts(p, 1.0, w, reinterpret_cast<const double*>(g1.data()),
+                 [&](int s, int X, int Y) { return pd[(int64_t)s * d.n2 + (int64_t)X * d.N + Y]; }, corr, E);
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1171,6 +1254,34 @@ int g12_g2_step(const double* comm, const double* psi, const double* pi, double*
     g12::launch_nat_to_ref(dR, raw, Ns, Nb, api_stream());
     return down(RK_result, raw, sizeof(double2) * n4);
 }
+
+// ---------------------------------------------------------------------------
+// energy (see energy_parts)
+
+int g12_energy(const g12_params* p, double lam, double w, const double* G1, const double* G2xx, const double* G2xy,
+               double* E) {
+    if (!p || !G1 || !E || !valid_geometry(p->Ns, p->Nb)) return G12_E_ARG;
+    typedef std::complex<double> cplx;
+    const int Ns = p->Ns, Nb = p->Nb, N = Ns * Nb;
+    const int64_t N2 = (int64_t)N * N;
+    const cplx* M[2] = {reinterpret_cast<const cplx*>(G2xx), reinterpret_cast<const cplx*>(G2xy)};
+    const bool corr = G2xx && G2xy;
+    // reference pair index of (X, Y) = ((i, a), (j, b)): i + j Ns + (a + b Nb) Ns^2
+    auto pair = [&](int X, int Y) {
+        return (int64_t)(X % Ns) + (int64_t)(Y % Ns) * Ns + ((int64_t)(X / Ns) + (int64_t)(Y / Ns) * Nb) * Ns * Ns;
+    };
+    energy_parts(*p, lam, w, G1,
+                 [&](int s, int X, int Y) {
+                     const int64_t A = pair(X, Y);
+                     return M[s][A * N2 + A];
+                 },
+                 corr, E);
+    return 0;
+}
+
+int g12_state_energy(g12_ctx* c, double* E) { return c ? state_energy(c->r, 0, E) : G12_E_ARG; }
+
+int g12_batch_state_energy(g12_batch* b, int i, double* E) { return b ? state_energy(b->r, i, E) : G12_E_ARG; }
 
 // ---------------------------------------------------------------------------
 // checkpoints: magic, N, Ns, Nb, t, steps, G1, G2xx, G2xy (reference layouts)

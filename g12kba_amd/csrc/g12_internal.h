@@ -124,6 +124,7 @@ void launch_comm2(const double2* T, const double2* hx, const double2* hy, double
 void launch_hf(double2* h, const double2* G1, int Ns, int Nb, const StageParams& sp, hipStream_t s);
 void launch_comm1(const double2* G, const double2* h, double2* out, int N, hipStream_t s);
 void launch_d_gather(const double2* T, double2* D, int N, hipStream_t s);
+void launch_pair_diag(const double2* T, double2* out, int N, hipStream_t s);
 void launch_w_tables(double* Wxy, double* Wxx, int Ns, int Nb, const StageParams& sp, hipStream_t s);
 void launch_f_tables(const double2* DP, const double2* DQ, const double* Wxy, const double* Wxx,
                      double2* F, double2* Ft, int N, hipStream_t s);

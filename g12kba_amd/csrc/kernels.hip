@@ -1409,6 +1409,16 @@ __global__ void k_d_gather(const double2* T, double2* D, int N) {
     D[e] = T[(((int64_t)X * N + Z) * N + Y) * N + Z];
 }
 
+// Pair-diagonal elements M[(X,Y);(X,Y)] = T[X][Y][X][Y] of a G2 tensor in the
+// natural layout (the correlation energy's only input from G2).
+__global__ void k_pair_diag(const double2* T, double2* out, int N) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N2 = (int64_t)N * N;
+    if (e >= N2) return;
+    const int64_t X = e / N, Y = e % N;
+    out[e] = T[((X * N + Y) * N + X) * N + Y];
+}
+
 __global__ void k_w_tables(double* Wxy, double* Wxx, int Ns, int Nb, StageParams sp) {
     const int N = Ns * Nb;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1659,6 +1669,10 @@ void launch_comm1(const double2* G, const double2* h, double2* out, int N, hipSt
 void launch_d_gather(const double2* T, double2* D, int N, hipStream_t s) {
     const int64_t n3 = (int64_t)N * N * N;
     hipLaunchKernelGGL(k_d_gather, dim3(blocks_for(n3, 256)), dim3(256), 0, s, T, D, N);
+}
+
+void launch_pair_diag(const double2* T, double2* out, int N, hipStream_t s) {
+    hipLaunchKernelGGL(k_pair_diag, dim3(blocks_for((int64_t)N * N, 256)), dim3(256), 0, s, T, out, N);
 }
 
 void launch_w_tables(double* Wxy, double* Wxx, int Ns, int Nb, const StageParams& sp, hipStream_t s) {

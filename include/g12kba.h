@@ -138,6 +138,12 @@ int g12_rk4_record(g12_ctx* ctx, int64_t n, double* traj, int64_t traj_cap, int6
 /* Current production time and step count of the context. */
 int g12_get_time(g12_ctx* ctx, double* t, int64_t* steps);
 
+/* Energy of the context's current state, as g12_energy with full U and the
+ * quench shift at the context's current time.  Only G1 and the N^2
+ * pair-diagonal elements of each G2 tensor leave the device.  No reference
+ * counterpart (README.md:61-63 lists energies as future work). */
+int g12_state_energy(g12_ctx* ctx, double* E);
+
 /* Per-kernel timing with HIP events on the context's stream.
  * g12_timing_enable(ctx, 1) starts recording; g12_timing_get returns, for the
  * kernel class `which` (0: two-particle commutator pass over strided tiles,
@@ -196,6 +202,8 @@ int g12_batch_timing_get(g12_batch* b, int which, int64_t* launches, double* tot
 int g12_batch_configure(g12_batch* b, int use_graphs, int autotune);
 int g12_batch_tune(g12_batch* b);
 int g12_batch_buckets(g12_batch* b, int cap, int* NP, int* count, int* waves_strided, int* waves_fused);
+/* g12_state_energy of trajectory i. */
+int g12_batch_state_energy(g12_batch* b, int i, double* E);
 
 /* ---- learned G1 extrapolator ------------------------------------------------
  * No reference counterpart (the reference's README names extrapolation of the
@@ -288,6 +296,17 @@ double g12_fermi(double t, double t0, double tau);
 
 /* Dipole trajectory (dipole, dipole.h:25-33); traj holds T N x N matrices. */
 int g12_dipole(double* p, const double* traj, int64_t T, int N);
+
+/* Energy of a state in the reference layouts, summed over both spins:
+ * E[0] one-body (hopping, eps, quench shift w on sites < Nq), E[1] Hartree,
+ * E[2] exchange, E[3] correlation (from the pair-diagonal elements of G2;
+ * 0 when G2xx or G2xy is NULL).  lam scales U and every V_n (1 after
+ * adiabatic switching).  The correlation weight is the one the equations of
+ * motion conserve: for Nb = 1 and a time-independent h0 the total is constant
+ * along a run.  No reference counterpart (README.md:61-63 lists energies as
+ * future work); host code, no GPU needed. */
+int g12_energy(const g12_params* p, double lam, double w, const double* G1, const double* G2xx, const double* G2xy,
+               double* E);
 
 /* Byte-compatible text outputs (write_to_file.h:26-69): <dir>/real_G1.txt,
  * <dir>/imag_G1.txt (one row per matrix element, one column per step, a space

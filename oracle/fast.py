@@ -167,6 +167,28 @@ def derivative(G1, Txx, Txy, Ns, Nb, U, t1, t2, w, eps, V, quench_extent, corr=T
     return K1, out[0], out[1], h
 
 
+def energy(G1, Txx, Txy, Ns, Nb, U, t1, t2, w, eps, V, quench_extent):
+    """(one-body, Hartree, exchange, correlation) of a state, both spins, with
+    rho = -i G1 and V the absolute V_n.  No reference counterpart (README.md:61-63
+    lists energies as future work).  Pinned in tests/test_cpu_energy.py: the
+    one-body part by the filled-band closed form, Hartree + exchange by
+    <Phi|H|Phi> of Slater determinants in Fock space, and the correlation
+    weight -1 by conservation of the total along the derivative above."""
+    N = Ns * Nb
+    h0 = hf(np.zeros((N, N), np.complex128), Ns, Nb, 0.0, t1, t2, w, eps, [0.0] * len(V), quench_extent)
+    rho = -1j * np.asarray(G1)
+    Wxy, Wxx = w_tables(Ns, Nb, U, V)
+    d = np.diag(rho)
+    e1 = 2 * np.trace(h0 @ rho).real
+    eh = (d @ (Wxx + Wxy) @ d).real
+    ef = -np.sum(Wxx * np.abs(rho) ** 2)
+    ec = 0.0
+    if Txx is not None:
+        pd = lambda T: np.einsum("xyxy->xy", np.asarray(T).reshape(N, N, N, N))
+        ec = -np.sum(Wxx * pd(Txx) + Wxy * pd(Txy)).real
+    return np.array([e1, eh, ef, ec])
+
+
 def fermi(t, t0, tau):
     return 1.0 / (1.0 + math.exp((t0 - t) / tau))
 
