@@ -196,6 +196,8 @@ struct Run {
     std::vector<Bucket> buckets;
     std::vector<std::vector<int>> lists;
     std::vector<std::vector<int64_t>> prefixes;
+    std::vector<int> tab_list;         // correlated trajectories
+    std::vector<int64_t> tab_prefix;   // prefix of their N^2 table blocks
     BatchDev b{};
     hipStream_t stream = nullptr;
     Timing timing;
@@ -263,6 +265,8 @@ int64_t layout(Run& r, char* base) {
         r.buckets[k].list = cv.take<int>(r.buckets[k].n);
         r.buckets[k].prefix = cv.take<int64_t>(r.buckets[k].n + 1);
     }
+    r.b.tlist = cv.take<int>((int64_t)r.tab_list.size());
+    r.b.tprefix = cv.take<int64_t>((int64_t)r.tab_prefix.size());
     return cv.off;
 }
 
@@ -336,6 +340,12 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
         r.lists.push_back(l);
         r.prefixes.push_back(pre);
     }
+    r.tab_prefix.assign(1, 0);
+    for (int i = 0; i < ntraj; ++i)
+        if (r.tp[i].corr) {
+            r.tab_list.push_back(i);
+            r.tab_prefix.push_back(r.tab_prefix.back() + r.ds[i].n2);
+        }
     const int64_t total = layout(r, nullptr);
     if (hipMalloc(&r.arena, total) != hipSuccess) {
         r.arena = nullptr;
@@ -358,11 +368,19 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
         HIPCHK(hipMemcpyAsync(const_cast<int64_t*>(r.buckets[k].prefix), r.prefixes[k].data(),
                               sizeof(int64_t) * r.prefixes[k].size(), hipMemcpyHostToDevice, r.stream));
     }
+    if (!r.tab_list.empty()) {
+        HIPCHK(hipMemcpyAsync(const_cast<int*>(r.b.tlist), r.tab_list.data(), sizeof(int) * r.tab_list.size(),
+                              hipMemcpyHostToDevice, r.stream));
+        HIPCHK(hipMemcpyAsync(const_cast<int64_t*>(r.b.tprefix), r.tab_prefix.data(),
+                              sizeof(int64_t) * r.tab_prefix.size(), hipMemcpyHostToDevice, r.stream));
+    }
     HIPCHK(hipStreamSynchronize(r.stream));
     r.b.ds = r.d_ds;
     r.b.tp = r.d_tp;
     r.b.ntraj = ntraj;
     r.b.maxN = r.maxN;
+    r.b.ncorr = (int)r.tab_list.size();
+    r.b.tblocks = r.tab_prefix.back();
     const char* e = std::getenv("G12KBA_GRAPHS");
     if (e && e[0] == '0') r.graphs = false;
     e = std::getenv("G12KBA_AUTOTUNE");

@@ -89,6 +89,12 @@ struct BatchDev {
     const TrajParams* tp;    // device array [ntraj]
     int ntraj;
     int maxN;
+    // correlated trajectories and the prefix of their N^2 table blocks, so
+    // that a table launch has exactly one block per (trajectory, X, Y)
+    const int* tlist;        // device array [ncorr]
+    const int64_t* tprefix;  // device array [ncorr + 1]
+    int ncorr;
+    int64_t tblocks;
 };
 
 struct Bucket {

@@ -376,22 +376,33 @@ __device__ inline void f_tables_block(const DevState& d, int X, int Y, bool herm
     }
 }
 
+// Table launches enumerate blocks over the correlated trajectories only:
+// block blk is entry (blk - tprefix[j]) of the N x N index space of listed
+// trajectory j.
+__device__ inline int table_block(const BatchDev& b, int64_t blk, int& e) {
+    int lo = 0, hi = b.ncorr - 1;
+    while (lo < hi) {  // last correlated trajectory whose blocks start at or before blk
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.tprefix[mid] <= blk) lo = mid; else hi = mid - 1;
+    }
+    e = (int)(blk - b.tprefix[lo]);
+    return b.tlist[lo];
+}
+
 __global__ __launch_bounds__(64) void k_q_tables(BatchDev b, StageRef r) {
-    const int ti = blockIdx.y;
+    int e = 0;
+    const int ti = table_block(b, blockIdx.x, e);
     const DevState& d = b.ds[ti];
-    if (!b.tp[ti].corr || !active(d, b.tp[ti], stage_of(r))) return;
-    const int N = d.N;
-    if ((int)blockIdx.x >= N * N) return;
-    q_tables_block(d, blockIdx.x / N, blockIdx.x % N);
+    if (!active(d, b.tp[ti], stage_of(r))) return;
+    q_tables_block(d, e / d.N, e % d.N);
 }
 
 __global__ __launch_bounds__(64) void k_f_tables(BatchDev b, StageRef r, int herm) {
-    const int ti = blockIdx.y;
+    int e = 0;
+    const int ti = table_block(b, blockIdx.x, e);
     const DevState& d = b.ds[ti];
-    if (!b.tp[ti].corr || !active(d, b.tp[ti], stage_of(r))) return;
-    const int N = d.N;
-    if ((int)blockIdx.x >= N * N) return;
-    f_tables_block(d, blockIdx.x / N, blockIdx.x % N, herm != 0);
+    if (!active(d, b.tp[ti], stage_of(r))) return;
+    f_tables_block(d, e / d.N, e % d.N, herm != 0);
 }
 
 // G1 right-hand side: comm = [h, Y1], K = -i(comm + I + I^dag), RK
@@ -1530,9 +1541,9 @@ void launch_b_stage_g1(const BatchDev& b, const StageRef& r, hipStream_t s) {
 }
 
 void launch_b_tables(const BatchDev& b, const StageRef& r, int precision, hipStream_t s) {
-    const unsigned n2 = (unsigned)(b.maxN * b.maxN);
-    hipLaunchKernelGGL(k_q_tables, dim3(n2, b.ntraj), dim3(64), 0, s, b, r);
-    hipLaunchKernelGGL(k_f_tables, dim3(n2, b.ntraj), dim3(64), 0, s, b, r, precision != G12_PREC_FP64 ? 1 : 0);
+    if (b.tblocks <= 0) return;
+    hipLaunchKernelGGL(k_q_tables, dim3((unsigned)b.tblocks), dim3(64), 0, s, b, r);
+    hipLaunchKernelGGL(k_f_tables, dim3((unsigned)b.tblocks), dim3(64), 0, s, b, r, precision != G12_PREC_FP64 ? 1 : 0);
 }
 
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& r, hipStream_t s) {

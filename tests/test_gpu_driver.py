@@ -41,6 +41,28 @@ def test_rk4_matches_oracle(Ns, Nb, ehm, law, qt):
         assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(b).max())
 
 
+@pytest.mark.parametrize("Ns,Nb", [(1, 1), (1, 2), (1, 3), (2, 1)])
+def test_tiny_geometries_match_oracle(Ns, Nb):
+    """Edge sizes N = 1..3 (one site: no hopping, no long-range shells; one
+    orbital: no t2), padded to 16-wide tiles on the device."""
+    p = g.make_params(Ns=Ns, Nb=Nb, t1=1.0, t2=0.4, U=1.4, dt=0.02, time_steps=12, quench_strength=1.3,
+                      Nq=max(1, Ns // 2), q_type="pulse", quench_rate=2.0, quench_on_midpoint=0.05,
+                      quench_off_midpoint=0.15, quench_gate_time=0.06, EHM=True, V_law="coulomb")
+    N = Ns * Nb
+    rng = np.random.default_rng(N)
+    _, Gxx, Gxy = small_state(rng, p)
+    G1 = g.make_G0(p) + 0.1j * np.eye(N)
+    with g.Propagator(p) as prop:
+        prop.set_state(G1, Gxx, Gxy)
+        rc, traj = prop.rk4()
+        G1g, Gxxg, Gxyg = prop.get_state()
+    rc2, G1o, Gxxo, Gxyo, trajo = O.rk4(oracle_cfg(p), G1, Gxx, Gxy, V_profile=g.v_profile(p))
+    assert rc == rc2 == 0 and traj.shape == trajo.shape
+    assert np.abs(traj - trajo).max() < 1e-12
+    for a, b in ((G1g, G1o), (Gxxg, Gxxo), (Gxyg, Gxyo)):
+        assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(b).max())
+
+
 def test_adiabatic_switching_matches_oracle():
     p = g.make_params(Ns=3, Nb=2, t1=1.0, t2=0.5, U=1.0, AS_rate=0.05, AS_midpoint=0.3, AS_tail=4,
                       EHM=True, V_law="coulomb")
