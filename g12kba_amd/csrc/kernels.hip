@@ -583,9 +583,9 @@ __device__ __forceinline__ void tile_product(const double* ar_, const double* ai
     }
 }
 
-template <int NP, int NW>
+template <int NP, int NW, class Mid>
 __device__ __forceinline__ void tile_mfma_fp64(const TileLds<NP>& L, int hmode, d4 (&ar)[TileCfg<NP, NW>::PW],
-                                               d4 (&ai)[TileCfg<NP, NW>::PW], int wave, int lane) {
+                                               d4 (&ai)[TileCfg<NP, NW>::PW], int wave, int lane, Mid&& mid) {
     constexpr int PW = TileCfg<NP, NW>::PW;
 #pragma unroll
     for (int u = 0; u < PW; ++u) {
@@ -593,6 +593,10 @@ __device__ __forceinline__ void tile_mfma_fp64(const TileLds<NP>& L, int hmode, 
         ai[u] = d4{0.0, 0.0, 0.0, 0.0};
     }
     if (hmode & 1) tile_product<NP, NW, false>(L.hr, L.hi, L.xr, L.xi, ar, ai, wave, lane);  // + h.X
+    mid([&](int u, int j, double2 v) {  // adds v to result element (u, j)
+        ar[u][j] += v.x;
+        ai[u][j] += v.y;
+    });
     if (hmode & 2) tile_product<NP, NW, true>(L.xr, L.xi, L.hr, L.hi, ar, ai, wave, lane);   // - X.h
 }
 
@@ -621,9 +625,12 @@ struct Engine<NP, NW, G12_PREC_FP64> {
     }
     __device__ static void zero(Lds&, int, int) {}
     __device__ static int row_of(int lane, int j) { return 4 * j + (lane >> 4); }
+    // mid(add) runs between the h.X and X.h products; add(u, j, v) adds v to
+    // result element (u, j) through the engine's accumulators
+    template <class Mid>
     __device__ static void compute(const Lds& L, int hmode, d4 (&ar)[TileCfg<NP, NW>::PW],
-                                   d4 (&ai)[TileCfg<NP, NW>::PW], int wave, int lane) {
-        tile_mfma_fp64<NP, NW>(L, hmode, ar, ai, wave, lane);
+                                   d4 (&ai)[TileCfg<NP, NW>::PW], int wave, int lane, Mid&& mid) {
+        tile_mfma_fp64<NP, NW>(L, hmode, ar, ai, wave, lane, mid);
     }
 };
 
@@ -684,7 +691,9 @@ struct Engine<NP, NW, G12_PREC_FP32_3M> {
             for (int u = 0; u < PW; ++u) p3[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(NEG ? -a3[u] : a3[u], b3[u], p3[u], 0, 0, 0);
         }
     }
-    __device__ static void compute(const Lds& L, int hmode, d4 (&ar)[PW], d4 (&ai)[PW], int wave, int lane) {
+    template <class Mid>
+    __device__ static void compute(const Lds& L, int hmode, d4 (&ar)[PW], d4 (&ai)[PW], int wave, int lane,
+                                   Mid&& mid) {
         f4 p1[PW], p2[PW], p3[PW];
 #pragma unroll
         for (int u = 0; u < PW; ++u) {
@@ -693,6 +702,11 @@ struct Engine<NP, NW, G12_PREC_FP32_3M> {
             p3[u] = p1[u];
         }
         if (hmode & 1) product<false>(L.hr, L.hi, L.hs, L.xr, L.xi, L.xs, p1, p2, p3, wave, lane);
+        // v enters Re = P1 - P2 through P1 and Im = P3 - P1 - P2 through P3
+        mid([&](int u, int j, double2 v) {
+            p1[u][j] += (float)v.x;
+            p3[u][j] += (float)(v.x + v.y);
+        });
         if (hmode & 2) product<true>(L.xr, L.xi, L.xs, L.hr, L.hi, L.hs, p1, p2, p3, wave, lane);
 #pragma unroll
         for (int u = 0; u < PW; ++u)
@@ -802,7 +816,9 @@ struct Engine<NP, NW, G12_PREC_BF16X3> {
             }
         }
     }
-    __device__ static void compute(const Lds& L, int hmode, d4 (&ar)[PW], d4 (&ai)[PW], int wave, int lane) {
+    template <class Mid>
+    __device__ static void compute(const Lds& L, int hmode, d4 (&ar)[PW], d4 (&ai)[PW], int wave, int lane,
+                                   Mid&& mid) {
         f4 cr[PW], ci[PW];
 #pragma unroll
         for (int u = 0; u < PW; ++u) {
@@ -810,6 +826,10 @@ struct Engine<NP, NW, G12_PREC_BF16X3> {
             ci[u] = cr[u];
         }
         if (hmode & 1) product<false>(L, 0, 12, cr, ci, wave, lane);  // + h.X : A = h, B = X^T planes
+        mid([&](int u, int j, double2 v) {
+            cr[u][j] += (float)v.x;
+            ci[u][j] += (float)v.y;
+        });
         if (hmode & 2) product<true>(L, 8, 4, cr, ci, wave, lane);    // - X.h : A = X, B = h^T planes
 #pragma unroll
         for (int u = 0; u < PW; ++u)
@@ -1136,8 +1156,11 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
             tile_prefetch<NP, NW, TILING>(a, nt, X, input_compact(a, ss), pre);
         }
 
-        // fused pass: this lane's R and S elements load while the MFMA loop runs
-        double2 rpre[FUSED ? PW : 1][4], spre[FUSED ? PW : 1][4];
+        // fused pass: this lane's R and S elements load while the MFMA loop
+        // runs.  Between the two products R is added to the accumulators and
+        // the RK accumulator A starts loading into the registers R held, so
+        // A's latency is hidden without more registers.
+        double2 rpre[FUSED ? PW : 1][4], spre[FUSED ? PW : 1][4], apre[FUSED ? PW : 1][4];
         if constexpr (FUSED) {
             auto preload = [&](auto compact) {
 #pragma unroll
@@ -1163,7 +1186,23 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         }
 
         d4 ar[PW], ai[PW];
-        E::compute(L, a.hmode, ar, ai, wave, lane);
+        E::compute(L, a.hmode, ar, ai, wave, lane, [&](auto&& add) {
+            if constexpr (FUSED) {
+#pragma unroll
+                for (int u = 0; u < PW; ++u) {
+                    const int t = wave + NW * u;
+                    const int n = (t % NT) * 16 + (lane & 15);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int m = (t / NT) * 16 + E::row_of(lane, j);
+                        add(u, j, rpre[u][j]);
+                        apre[u][j] = make_double2(0.0, 0.0);
+                        if (!ss.first && t < TT && m < N && n < N)
+                            apre[u][j] = gld(v.A, base + (uint32_t)m * sm + (uint32_t)n * sn);
+                    }
+                }
+            }
+        });
 
         if constexpr (!FUSED) {
             double2* out = v.R;
@@ -1211,10 +1250,10 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                 store(std::false_type{});
         } else {
             // fused update for tile (p1, q1) = (t0, t1), element (p2, q2) = (m, n):
-            // R holds the strided-pass terms and their sources; with this
-            // pass's sources g22 (ve[m] - ve[n]), K = -i (R + O + sources);
-            // then A += w K and the next stage input Y = S + c_next K, or at
-            // the last stage S += dt (A + w K) / 6.
+            // the accumulators hold R + O (R: the strided-pass terms and their
+            // sources); with this pass's sources g22 (ve[m] - ve[n]),
+            // K = -i (R + O + sources); then A += w K and the next stage input
+            // Y = S + c_next K, or at the last stage S += dt (A + w K) / 6.
             const double2 ve = cadd(cmul_i(sv.q), sv.f);
             auto update = [&](auto compact) {
                 constexpr bool C = decltype(compact)::value;
@@ -1230,18 +1269,21 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                         const double2 em = a.sources ? lane_value(ve, m) : double2{};
                         if (m >= N || n >= N) continue;
                         const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
-                        double2 rr = cadd(rpre[u][j], make_double2(ar[u][j], ai[u][j]));
+                        double2 rr = make_double2(ar[u][j], ai[u][j]);  // R + O
                         if (a.sources) rr = cadd(rr, cmul(gld(v.G1p, 2 * (uint32_t)(m * N + n)), csub(em, en)));
                         const double2 K = cmul_mi(rr);
                         const double2 wK = cscale(K, ss.wk);
                         const double2 s = spre[u][j];
                         double2 next;  // input of the next stage (the new state after the last)
                         if (!ss.last) {
-                            gst(v.A, off, ss.first ? wK : cadd(gld(v.A, off), wK));
+                            if (ss.first)
+                                gst(v.A, off, wK);
+                            else
+                                gst(v.A, off, cadd(apre[u][j], wK));
                             next = cadd(s, cscale(K, ss.cn));
                             cst(v.Y, off, next, C);
                         } else {
-                            const double2 A = cadd(gld(v.A, off), wK);
+                            const double2 A = cadd(apre[u][j], wK);
                             next = cadd(s, cdiv6(cscale(A, ss.dt)));
                             gst(v.S, off, next);
                         }
