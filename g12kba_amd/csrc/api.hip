@@ -197,7 +197,7 @@ struct Run {
     std::vector<std::vector<int>> lists;
     std::vector<std::vector<int64_t>> prefixes;
     std::vector<int> tab_list;         // correlated trajectories
-    std::vector<int64_t> tab_prefix;   // prefix of their N^2 table blocks
+    std::vector<int64_t> tab_prefix;   // prefix of their table blocks
     BatchDev b{};
     hipStream_t stream = nullptr;
     Timing timing;
@@ -340,11 +340,13 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
         r.lists.push_back(l);
         r.prefixes.push_back(pre);
     }
+    // table launches: N x ceil(N / G12_TABLE_GROUP) blocks per correlated trajectory
     r.tab_prefix.assign(1, 0);
     for (int i = 0; i < ntraj; ++i)
         if (r.tp[i].corr) {
+            const int64_t N = r.ds[i].N;
             r.tab_list.push_back(i);
-            r.tab_prefix.push_back(r.tab_prefix.back() + r.ds[i].n2);
+            r.tab_prefix.push_back(r.tab_prefix.back() + N * ((N + G12_TABLE_GROUP - 1) / G12_TABLE_GROUP));
         }
     const int64_t total = layout(r, nullptr);
     if (hipMalloc(&r.arena, total) != hipSuccess) {

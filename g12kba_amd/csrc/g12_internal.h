@@ -84,13 +84,18 @@ struct DevState {
 // A set of trajectories advanced together: device arrays of states and
 // constants, and (for the tile passes) buckets of trajectories with the same
 // padded tile size.
+// Table launches: one block per (X, group of G12_TABLE_GROUP values of the
+// other index) of each correlated trajectory.
+#define G12_TABLE_GROUP 8
+
 struct BatchDev {
     const DevState* ds;      // device array [ntraj]
     const TrajParams* tp;    // device array [ntraj]
     int ntraj;
     int maxN;
-    // correlated trajectories and the prefix of their N^2 table blocks, so
-    // that a table launch has exactly one block per (trajectory, X, Y)
+    // correlated trajectories and the prefix of their table blocks
+    // (N x ceil(N / G12_TABLE_GROUP) each), so that a table launch has no
+    // idle blocks
     const int* tlist;        // device array [ncorr]
     const int64_t* tprefix;  // device array [ncorr + 1]
     int ncorr;

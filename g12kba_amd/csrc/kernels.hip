@@ -304,25 +304,37 @@ __global__ __launch_bounds__(256) void k_stage_g1(BatchDev b, StageRef r) {
 }
 
 // ---------------------------------------------------------------------------
+// Stage tables.  A block handles TG values of one index for one X, so that
+// the operand every output needs (the stage G1 for Qp, W_xy for F) is read
+// from L2 once per block into LDS instead of once per output row.  N <= 64.
+constexpr int TG = G12_TABLE_GROUP;
+constexpr int TMAX = 64;
+
+__device__ inline int table_groups(int N) { return (N + TG - 1) / TG; }
+
 // Qp^s[X][Y][Z] = Q^s[Z][X][Y] = sum_w Y1[X][w] W^s[Z][w] Y1[w][Y] for
-// s = xx (index 0, W_xx) and xy (index 1, W_xy).  One block per (Z, X),
-// threads over Y.
-__device__ inline void q_tables_block(const DevState& d, int Z, int X) {
-    __shared__ double2 u[G12_MAX_NB * 64];
-    __shared__ double wz[G12_MAX_NB * 64];
+// s = xx (index 0, W_xx) and xy (index 1, W_xy), Z in [Z0, Z0 + TG).
+__device__ inline void q_tables_block(const DevState& d, int X, int Z0) {
+    __shared__ double2 Gs[TMAX * TMAX];  // Y1
+    __shared__ double wz[TG * TMAX];     // W rows of the block's Z
     const int N = d.N;
     const double2* G = d.Y1;
-    for (int w = threadIdx.x; w < N; w += blockDim.x) {
-        wz[w] = d.Wxy[(int64_t)w * N + Z];  // W_xy is symmetric
-        u[w] = G[(int64_t)X * N + w];
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) Gs[e] = G[e];
+    for (int e = threadIdx.x; e < TG * N; e += blockDim.x) {
+        const int Z = Z0 + e / N, w = e % N;
+        wz[e] = Z < N ? d.Wxy[(int64_t)w * N + Z] : 0.0;  // W_xy is symmetric
     }
     __syncthreads();
-    for (int Y = threadIdx.x; Y < N; Y += blockDim.x) {
+    const double2* u = Gs + X * N;  // Y1[X][w]
+    for (int e = threadIdx.x; e < TG * N; e += blockDim.x) {
+        const int zi = e / N, Y = e % N, Z = Z0 + zi;
+        if (Z >= N) continue;
+        const double* wr = wz + zi * N;
         double2 axy = make_double2(0.0, 0.0), axx = make_double2(0.0, 0.0);
         for (int w = 0; w < N; ++w) {
-            const double wv = wz[w];
+            const double wv = wr[w];
             if (wv == 0.0) continue;
-            const double2 t = cscale(cmul(u[w], G[(int64_t)w * N + Y]), wv);
+            const double2 t = cscale(cmul(u[w], Gs[w * N + Y]), wv);
             axy = cadd(axy, t);
             if (w != Z) axx = cadd(axx, t);
         }
@@ -335,8 +347,8 @@ __device__ inline void q_tables_block(const DevState& d, int Z, int X) {
 // F^xx[X][Y][Z] = -i sum_w ( W_xy[Z][w] D^xy[X][Y][w] + W_xx[Z][w] D^xx[X][Y][w] )
 // F^xy[X][Y][Z] = -i sum_w ( W_xy[Z][w] D^xx[X][Y][w] + W_xx[Z][w] D^xy[X][Y][w] )
 // (Pi_xyxy with the swapped G2 pair, pi.h:25-95 / RK_solver.h:163-164) and
-// I[X][Y] = F^xx[X][Y][X]
-// (collision_int, collision_integral.h:27-69).  One block per (X, Y).
+// I[X][Y] = F^xx[X][Y][X] (collision_int, collision_integral.h:27-69), for Y
+// in [Y0, Y0 + TG).
 // With `herm` (mixed-precision runs) D enters through its Hermitian part
 // (D[X][Y][Z] + conj D[Y][X][Z]) / 2.  A Hermitian G2 (the physical states:
 // G2 evolved from zero stays Hermitian) has D[X][Y][Z] = conj D[Y][X][Z], and
@@ -344,26 +356,38 @@ __device__ inline void q_tables_block(const DevState& d, int Z, int X) {
 // or BF16 tile products break it slightly, the average restores particle-number
 // conservation to FP64 level.  FP64 runs use D as it is, as the reference does,
 // for any input.
-__device__ inline void f_tables_block(const DevState& d, int X, int Y, bool herm) {
-    __shared__ double2 dxx[G12_MAX_NB * 64], dxy[G12_MAX_NB * 64];
+__device__ inline void f_tables_block(const DevState& d, int X, int Y0, bool herm) {
+    __shared__ double Ws[TMAX * TMAX];                   // W_xy
+    __shared__ double2 dxx[TG * TMAX], dxy[TG * TMAX];  // D rows of the block's Y
     const int N = d.N;
-    for (int w = threadIdx.x; w < N; w += blockDim.x) {
-        const int64_t xy = ((int64_t)X * N + Y) * N + w, yx = ((int64_t)Y * N + X) * N + w;
-        dxx[w] = d.D[0][xy];
-        dxy[w] = d.D[1][xy];
-        if (herm) {
-            dxx[w] = cscale(cadd(dxx[w], cconj(d.D[0][yx])), 0.5);
-            dxy[w] = cscale(cadd(dxy[w], cconj(d.D[1][yx])), 0.5);
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) Ws[e] = d.Wxy[e];
+    for (int e = threadIdx.x; e < TG * N; e += blockDim.x) {
+        const int Y = Y0 + e / N, w = e % N;
+        double2 a = make_double2(0.0, 0.0), b = a;
+        if (Y < N) {
+            const int64_t xy = ((int64_t)X * N + Y) * N + w, yx = ((int64_t)Y * N + X) * N + w;
+            a = d.D[0][xy];
+            b = d.D[1][xy];
+            if (herm) {
+                a = cscale(cadd(a, cconj(d.D[0][yx])), 0.5);
+                b = cscale(cadd(b, cconj(d.D[1][yx])), 0.5);
+            }
         }
+        dxx[e] = a;
+        dxy[e] = b;
     }
     __syncthreads();
-    for (int Z = threadIdx.x; Z < N; Z += blockDim.x) {
+    for (int e = threadIdx.x; e < TG * N; e += blockDim.x) {
+        const int yi = e / N, Z = e % N, Y = Y0 + yi;
+        if (Y >= N) continue;
+        const double2* rxx = dxx + yi * N;
+        const double2* rxy = dxy + yi * N;
         double2 fxx = make_double2(0.0, 0.0), fxy = make_double2(0.0, 0.0);
         for (int w = 0; w < N; ++w) {
-            const double a = d.Wxy[(int64_t)w * N + Z];  // W_xy is symmetric
+            const double a = Ws[w * N + Z];  // W_xy is symmetric
             if (a == 0.0) continue;
             const double b = (w == Z) ? 0.0 : a;
-            const double2 vxx = dxx[w], vxy = dxy[w];
+            const double2 vxx = rxx[w], vxy = rxy[w];
             fxx = cadd(fxx, cadd(cscale(vxy, a), cscale(vxx, b)));
             fxy = cadd(fxy, cadd(cscale(vxx, a), cscale(vxy, b)));
         }
@@ -377,8 +401,8 @@ __device__ inline void f_tables_block(const DevState& d, int X, int Y, bool herm
 }
 
 // Table launches enumerate blocks over the correlated trajectories only:
-// block blk is entry (blk - tprefix[j]) of the N x N index space of listed
-// trajectory j.
+// block blk is entry (blk - tprefix[j]) of the N x ceil(N / TG) block space
+// of listed trajectory j.
 __device__ inline int table_block(const BatchDev& b, int64_t blk, int& e) {
     int lo = 0, hi = b.ncorr - 1;
     while (lo < hi) {  // last correlated trajectory whose blocks start at or before blk
@@ -389,20 +413,22 @@ __device__ inline int table_block(const BatchDev& b, int64_t blk, int& e) {
     return b.tlist[lo];
 }
 
-__global__ __launch_bounds__(64) void k_q_tables(BatchDev b, StageRef r) {
+__global__ __launch_bounds__(256) void k_q_tables(BatchDev b, StageRef r) {
     int e = 0;
     const int ti = table_block(b, blockIdx.x, e);
     const DevState& d = b.ds[ti];
     if (!active(d, b.tp[ti], stage_of(r))) return;
-    q_tables_block(d, e / d.N, e % d.N);
+    const int ng = table_groups(d.N);
+    q_tables_block(d, e / ng, (e % ng) * TG);
 }
 
-__global__ __launch_bounds__(64) void k_f_tables(BatchDev b, StageRef r, int herm) {
+__global__ __launch_bounds__(256) void k_f_tables(BatchDev b, StageRef r, int herm) {
     int e = 0;
     const int ti = table_block(b, blockIdx.x, e);
     const DevState& d = b.ds[ti];
     if (!active(d, b.tp[ti], stage_of(r))) return;
-    f_tables_block(d, e / d.N, e % d.N, herm != 0);
+    const int ng = table_groups(d.N);
+    f_tables_block(d, e / ng, (e % ng) * TG, herm != 0);
 }
 
 // G1 right-hand side: comm = [h, Y1], K = -i(comm + I + I^dag), RK
@@ -1584,8 +1610,8 @@ void launch_b_stage_g1(const BatchDev& b, const StageRef& r, hipStream_t s) {
 
 void launch_b_tables(const BatchDev& b, const StageRef& r, int precision, hipStream_t s) {
     if (b.tblocks <= 0) return;
-    hipLaunchKernelGGL(k_q_tables, dim3((unsigned)b.tblocks), dim3(64), 0, s, b, r);
-    hipLaunchKernelGGL(k_f_tables, dim3((unsigned)b.tblocks), dim3(64), 0, s, b, r, precision != G12_PREC_FP64 ? 1 : 0);
+    hipLaunchKernelGGL(k_q_tables, dim3((unsigned)b.tblocks), dim3(256), 0, s, b, r);
+    hipLaunchKernelGGL(k_f_tables, dim3((unsigned)b.tblocks), dim3(256), 0, s, b, r, precision != G12_PREC_FP64 ? 1 : 0);
 }
 
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& r, hipStream_t s) {
