@@ -69,6 +69,7 @@ def _load():
         "g12_params_default": [ctypes.POINTER(Params)],
         "g12_parse_input": [ctypes.c_char_p, ctypes.POINTER(Params)],
         "g12_make_g0": [ctypes.POINTER(Params), p, i],
+        "g12_init_g_test": [ctypes.POINTER(Params), p],
         "g12_create": [ctypes.POINTER(Params), ctypes.POINTER(vp)],
         "g12_destroy": [vp],
         "g12_set_state": [vp, p, p, p],
@@ -149,7 +150,7 @@ lib = _load()
 
 # symbols declared in include/g12kba.h (checked by the CPU test-suite)
 EXPORTED = [
-    "g12_params_default", "g12_parse_input", "g12_make_g0", "g12_create", "g12_destroy",
+    "g12_params_default", "g12_parse_input", "g12_make_g0", "g12_init_g_test", "g12_create", "g12_destroy",
     "g12_set_state", "g12_get_state", "g12_adiabatic_switching", "g12_rk4", "g12_rk4_advance",
     "g12_get_time", "g12_rk4_record", "g12_timing_enable", "g12_timing_get", "g12_memory_bytes", "g12_synchronize",
     "g12_hf_hamiltonian", "g12_h1_commutator", "g12_h2_commutator", "g12_collision",
@@ -292,6 +293,15 @@ def make_G0(p, fp32_emulation=False):
     N = p.Ns * p.Nb
     G1 = np.zeros((N, N), np.complex128)
     _check(lib.g12_make_g0(ctypes.byref(p), _p(G1), int(fp32_emulation)), "g12_make_g0")
+    return G1
+
+
+def init_G_test(p):
+    """Domain-wall initial G1 (init_G_test, init_G.h:78-86): every orbital of
+    the first int(Ns/2) sites occupied."""
+    N = p.Ns * p.Nb
+    G1 = np.zeros((N, N), np.complex128)
+    _check(lib.g12_init_g_test(ctypes.byref(p), _p(G1)), "g12_init_g_test")
     return G1
 
 

@@ -2,10 +2,12 @@
 // parse ./input, build G1(0), adiabatic switching, RK4 with the G1 trajectory,
 // dipole, and write real_G1.txt / imag_G1.txt / dipole.txt / progress.txt.
 //
-//   g12kba [input-file] [--g0-fp32] [--save-state FILE] [--load-state FILE] [--g0 FILE]
+//   g12kba [input-file] [--g0-fp32] [--g0-test] [--save-state FILE] [--load-state FILE] [--g0 FILE]
 //
 // --g0 FILE reads G1(0) (N*N complex128, raw binary) instead of the
-// eigen-solver; --load-state skips G0 and adiabatic switching.
+// eigen-solver; --g0-test starts from the domain-wall state of init_G_test
+// (first half of the chain filled); --load-state skips G0 and adiabatic
+// switching.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -18,11 +20,13 @@
 
 int main(int argc, char** argv) {
     std::string input = "input", save, load, g0file;
-    int fp32 = 0;
+    int fp32 = 0, g0test = 0;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "--g0-fp32")
             fp32 = 1;
+        else if (a == "--g0-test")
+            g0test = 1;
         else if (a == "--save-state" && i + 1 < argc)
             save = argv[++i];
         else if (a == "--load-state" && i + 1 < argc)
@@ -60,6 +64,8 @@ int main(int argc, char** argv) {
                 return 4;
             }
             std::fclose(f);
+        } else if (g0test) {
+            g12_init_g_test(&p, G1.data());
         } else {
             g12_make_g0(&p, G1.data(), fp32);
         }

@@ -280,6 +280,19 @@ int g12_parse_input(const char* path, g12_params* p) {
     return 0;
 }
 
+// init_G_test, init_G.h:78-86 (whose stray backslashes do not compile,
+// reference defect B-1): every orbital of the first int(Ns/2) sites filled,
+// G1[X][X] = i for site(X) < int(Ns/2), all other elements 0.
+int g12_init_g_test(const g12_params* p, double* G1) {
+    if (!p || !G1 || p->Ns < 1 || p->Nb < 1 || p->Nb > G12_MAX_NB) return G12_E_ARG;
+    const int Ns = p->Ns, Nb = p->Nb, N = Ns * Nb;
+    cplx* g = reinterpret_cast<cplx*>(G1);
+    for (int64_t e = 0; e < (int64_t)N * N; ++e) g[e] = 0.0;
+    for (int i = 0; i < Ns / 2; ++i)
+        for (int a = 0; a < Nb; ++a) g[(int64_t)(i + a * Ns) * N + i + a * Ns] = cplx(0.0, 1.0);
+    return 0;
+}
+
 // make_G0, init_G.h:57-75: h(U=0) = eps - t1 (neighbouring sites) - t2
 // (adjacent orbitals); G1 = i sum_{k < p} phi_k phi_k^dagger.
 int g12_make_g0(const g12_params* p, double* G1, int fp32_emulation) {

@@ -105,6 +105,10 @@ int g12_parse_input(const char* path, g12_params* p);
  * fp32_emulation = 1 rounds h and the eigenvectors through float). */
 int g12_make_g0(const g12_params* p, double* G1, int fp32_emulation);
 
+/* Domain-wall initial state (init_G_test, init_G.h:78-86): every orbital of
+ * the first int(Ns/2) sites occupied, G1[X][X] = i there and 0 elsewhere. */
+int g12_init_g_test(const g12_params* p, double* G1);
+
 /* ---- context, state, drivers --------------------------------------------- */
 
 int g12_create(const g12_params* p, g12_ctx** out);

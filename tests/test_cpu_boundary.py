@@ -142,3 +142,17 @@ def test_g0_fp32_emulation(Ns, Nb, t2):
     assert np.abs(P @ P - P).max() < 1e-5
     d = np.abs(G32 - G64).max()
     assert 0 < d < 1e-5  # rounded through float, and only that far
+
+
+@pytest.mark.parametrize("Ns,Nb", [(7, 1), (8, 2), (5, 3)])
+def test_init_g_test_domain_wall(Ns, Nb):
+    """init_G_test (init_G.h:78-86): G1[X][X] = i for every orbital of the
+    first int(Ns/2) sites, zero elsewhere; a projector with trace Nb*int(Ns/2)."""
+    p = g.make_params(Ns=Ns, Nb=Nb)
+    G = g.init_G_test(p)
+    N = Ns * Nb
+    site = np.arange(N) % Ns
+    expect = np.diag(np.where(site < Ns // 2, 1j, 0.0))
+    assert np.array_equal(G, expect)
+    P = -1j * G
+    assert np.allclose(P @ P, P) and abs(np.trace(P) - Nb * (Ns // 2)) == 0

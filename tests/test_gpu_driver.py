@@ -148,3 +148,24 @@ def test_checkpoint_resume_is_bit_exact(tmp_path):
     assert np.array_equal(np.concatenate([first, rest]), full)
     for a, b in zip(end, end2):
         assert np.array_equal(a, b)
+
+
+def test_domain_wall_run_matches_oracle():
+    """From init_G_test's domain wall (init_G.h:78-86): switching and 60 RK4
+    steps against the factorised oracle; the wall melts (the dipole moves)."""
+    p = g.make_params(Ns=6, Nb=2, t1=1.0, t2=0.4, U=1.2, dt=0.02, time_steps=60, quench_strength=0.0, Nq=0,
+                      EHM=True, V_law="coulomb")
+    N = p.Ns * p.Nb
+    G1 = g.init_G_test(p)
+    Z = np.zeros((N * N, N * N), np.complex128)
+    with g.Propagator(p) as prop:
+        prop.set_state(G1)
+        nas = prop.adiabatic_switching()
+        rc, traj = prop.rk4()
+        G1g = prop.get_state()[0]
+    ref = O.fast_run(oracle_cfg(p), G1, Z, Z, V_profile=list(g.v_profile(p)), traj_every=1, traj_cap=80)
+    assert rc == ref["rc"] == 0 and nas == ref["as_steps"] and len(traj) == ref["rk_steps"]
+    assert np.abs(traj - ref["traj"]).max() < 1e-11
+    assert np.abs(G1g - ref["G1"]).max() < 1e-11
+    d = g.dipole(traj)
+    assert np.abs(d - d[0]).max() > 1e-3
