@@ -461,10 +461,10 @@ void drop_graphs(Run& r) {
 // Wave counts per bucket and pass, chosen by timing the compiled candidates on
 // the run's own tile lists.  Both passes are run as stage 0 of a step (the
 // strided pass writes only R2, the fused pass only Y2, A2 and D; the next
-// stage 0 overwrites R2, Y2 and A2 before reading them and every run starts by
-// rebuilding D from S), so tuning at a step boundary leaves the state
-// unchanged.  Results are cached per process by
-// (tile size, precision, tile count).
+// stage 0 overwrites R2 and A2 before reading them, and every run starts, after
+// tuning, by rebuilding D and the compact stage-0 input Y2 from S), so tuning
+// at a step boundary leaves the state unchanged.  Results are cached per
+// process by (tile size, precision, tile count).
 int tune(Run& r) {
     if (r.tuned || !r.autotune) return 0;
     // (waves, workgroups-per-CU cap) of the strided and the fused pass
@@ -538,9 +538,10 @@ int run_windows(Run& r, int phase, int64_t nsteps, Fill&& fill, After&& after) {
     if (rc) return rc;
     const bool graph = r.graphs && !r.timing.on;
     if (graph && (rc = capture(r, phase))) return rc;
-    // partial traces of the first stage's input (S); every fused pass then
-    // writes those of the next stage input
-    for (const Bucket& k : r.buckets) g12::launch_b_d_from_state(r.b, k, r.stream);
+    // partial traces of the first stage's input (S), and its complex64 copy
+    // in mixed-precision runs; every fused pass then writes both for the
+    // next stage input
+    for (const Bucket& k : r.buckets) g12::launch_b_d_from_state(r.b, k, r.precision, r.stream);
     for (int64_t w0 = 0; w0 < nsteps; w0 += r.window) {
         const int n = (int)std::min<int64_t>(r.window, nsteps - w0);
         for (int i = 0; i < n; ++i) fill(w0 + i, i, r.h_table + 4 * i);

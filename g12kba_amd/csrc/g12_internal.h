@@ -120,7 +120,7 @@ void launch_b_tables(const BatchDev& b, const StageRef& sr, int precision, hipSt
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_record(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int fused, int precision, hipStream_t s);
-void launch_b_d_from_state(const BatchDev& b, const Bucket& k, hipStream_t s);
+void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hipStream_t s);
 void launch_b_advance_counter(int* counter, hipStream_t s);
 int device_cu_count();
 int default_waves(int NP);

@@ -878,7 +878,7 @@ struct BTileArgs {
     int hmode;        // 1: +h.X, 2: -X.h, 3: both
     int accumulate;   // unfused: R += O instead of R = O
     int nspin;
-    int use_y;        // stage input from the Y buffer after stage 0 (0: always S)
+    int use_y;        // stage input from the Y buffer (after stage 0, or always when compact; 0: always S)
     int compact;      // R and Y held as complex64 (mixed-precision runs)
     int sources;      // runs: each pass adds its share of the G2 sources (tile_sources)
     const double2* h_override;  // routine calls: h to use instead of the trajectory's h
@@ -982,12 +982,15 @@ struct View {
     const double* W;        // W_xx (spin 0) or W_xy (spin 1)
 };
 
+// Mixed-precision runs keep the stage input compact at every stage: Y holds
+// the complex64 copy of S at stage 0 (written by the previous step's last
+// fused pass, or at the start of a run).
 __device__ inline const double2* stage_input(const DevState& d, const BTileArgs& a, const StageShared& ss, int spin) {
-    return (a.use_y && !ss.first) ? d.Y2[spin] : d.S2[spin];
+    return (a.use_y && (a.compact || !ss.first)) ? d.Y2[spin] : d.S2[spin];
 }
 // the stage input is compact when it is the Y buffer of a mixed-precision run
-__device__ inline bool input_compact(const BTileArgs& a, const StageShared& ss) {
-    return a.compact && a.use_y && !ss.first;
+__device__ inline bool input_compact(const BTileArgs& a, const StageShared&) {
+    return a.compact && a.use_y;
 }
 
 template <bool FUSED>
@@ -1057,6 +1060,38 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
         }
     }
 }
+
+// The same into raw complex64 registers (mixed-precision engines, whose tile
+// products take FP32 or BF16 operands): a compact stage input is loaded as it
+// is stored, with nothing waiting for it; a complex128 input (routine-level
+// calls only) is rounded on arrival.
+template <int NP, int NW, int TILING>
+__device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef& t, const double2* X, bool xc,
+                                              fv2 (&pre)[TileCfg<NP, NW>::PF]) {
+    constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
+    const int N = t.N;
+    uint32_t base, sm, sn;
+    int t0, t1;
+    tile_geometry(TILING < 0 ? a.tiling : TILING, N, t.tile, base, sm, sn, t0, t1);
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+        const int e = threadIdx.x + NTHR * i;
+        fv2 y = fv2{0.f, 0.f};
+        if (e < N * N) {
+            const uint32_t off = base + (uint32_t)(e / N) * sm + (uint32_t)(e % N) * sn;
+            if (xc) {
+                y = *(const GLOBAL fv2*)((const GLOBAL char*)X + (off << 3));
+            } else {
+                const double2 d = gld(X, off);
+                y = fv2{(float)d.x, (float)d.y};
+            }
+        }
+        pre[i] = y;
+    }
+}
+
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(fv2 v) { return make_double2(v.x, v.y); }
 
 // Sources of the G2 equation (psi.h:26-277, pi.h:25-95, G1_G2_t_step.h:55-86
 // through the partial-trace tables), for element (p1, p2, q1, q2) of spin s
@@ -1132,9 +1167,31 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     int64_t gt = next_tile(it);
     if (gt < 0) return;
     TileRef tr = ref_of(c, gt);
-    double2 pre[PF];
-    tile_prefetch<NP, NW, TILING>(a, tr, uni_ptr(stage_input(a.b.ds[tr.traj], a, ss, tr.spin)),
-                                  input_compact(a, ss), pre);
+    // Stage-input prefetch: a tile's X loads into registers while earlier
+    // tiles are multiplied.  The mixed-precision engines hold raw complex64
+    // elements (their stage input is compact), so two tiles fit in the
+    // registers of one complex128 tile and the tile after next is in flight
+    // too: their shorter MFMA loops do not cover one tile's load latency.
+    constexpr bool DEEP = PREC != G12_PREC_FP64;
+    typedef typename std::conditional<DEEP, fv2, double2>::type PreT;
+    PreT P0[PF], P1[PF];
+    auto prefetch = [&](const TileRef& t, PreT (&P)[PF]) {
+        const double2* X = (t.traj == v.traj && t.spin == v.spin)
+                               ? v.X
+                               : uni_ptr(stage_input(a.b.ds[t.traj], a, ss, t.spin));
+        tile_prefetch<NP, NW, TILING>(a, t, X, input_compact(a, ss), P);
+    };
+    prefetch(tr, P0);
+    int64_t g1 = -1;  // DEEP: the tile after tr, in flight in the other buffer
+    TileRef t1r = tr;
+    if constexpr (DEEP) {
+        ++it;
+        g1 = next_tile(it);
+        if (g1 >= 0) {
+            t1r = ref_of(c, g1);
+            prefetch(t1r, P1);
+        }
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // source vectors of the current tile: loaded after the previous tile's
     // epilogue, used in this tile's
@@ -1144,7 +1201,10 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         load_view<FUSED>(v0, a, ss, tr.traj, tr.spin);
         src_load<FUSED>(v0, tr, lane, sv);
     }
-    while (gt >= 0) {
+    // one tile; Pc holds its stage input and is refilled with the tile after
+    // the ones already in flight
+    auto step = [&](PreT (&Pc)[PF]) -> bool {
+        if (gt < 0) return false;
         const int N = tr.N;
         lds_sync();  // previous tile's readers of X (and h) are done
         if (tr.traj != v.traj || tr.spin != v.spin) {
@@ -1168,18 +1228,23 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
-            if (e < N * N) E::put_x(L, e / N, e % N, pre[i]);
+            if (e < N * N) E::put_x(L, e / N, e % N, to_d2(Pc[i]));
         }
         lds_sync();
         ++it;
-        const int64_t next = next_tile(it);
-        TileRef nt = tr;
-        if (next >= 0) {  // the next tile's loads overlap the MFMA loop
-            nt = ref_of(c, next);
-            const double2* X = (nt.traj == v.traj && nt.spin == v.spin)
-                                   ? v.X
-                                   : uni_ptr(stage_input(a.b.ds[nt.traj], a, ss, nt.spin));
-            tile_prefetch<NP, NW, TILING>(a, nt, X, input_compact(a, ss), pre);
+        const int64_t gnew = next_tile(it);
+        TileRef rnew = tr;
+        if (gnew >= 0) {  // its loads overlap the MFMA loop(s) before its turn
+            rnew = ref_of(c, gnew);
+            prefetch(rnew, Pc);
+        }
+        int64_t gnext = gnew;  // the tile after tr
+        TileRef nt = rnew;
+        if constexpr (DEEP) {
+            gnext = g1;
+            nt = t1r;
+            g1 = gnew;
+            t1r = rnew;
         }
 
         // fused pass: this lane's R and S elements load while the MFMA loop
@@ -1312,6 +1377,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                             const double2 A = cadd(apre[u][j], wK);
                             next = cadd(s, cdiv6(cscale(A, ss.dt)));
                             gst(v.S, off, next);
+                            if constexpr (C) cst(v.Y, off, next, true);  // next step's stage-0 input
                         }
                         // partial trace of the next stage input, FP64:
                         // D[X=p1][Y=q1][Z=p2] for p2 = q2
@@ -1324,7 +1390,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
             else
                 update(std::false_type{});
         }
-        if (a.sources && next >= 0) {
+        if (a.sources && gnext >= 0) {
             if (nt.traj == v.traj && nt.spin == v.spin) {
                 src_load<FUSED>(v, nt, lane, sv);
             } else {
@@ -1333,15 +1399,25 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                 src_load<FUSED>(vn, nt, lane, sv);
             }
         }
-        gt = next;
+        gt = gnext;
         tr = nt;
+        return true;
+    };
+    if constexpr (DEEP) {
+        while (step(P0) && step(P1)) {
+        }
+    } else {
+        while (step(P0)) {
+        }
     }
 }
 
-// Partial traces of the state at the start of a run (the stage-0 input):
-// D[X][Y][Z] = S[X][Z][Y][Z] for both spins; within a run each fused pass
-// writes those of the next stage input.  One block per slice (X, Y), blocks in
-// the bucket's tile order (spin 0 tiles).
+// Start of a run: partial traces of the state (the stage-0 input)
+// D[X][Y][Z] = S[X][Z][Y][Z] for both spins and, in mixed-precision runs, the
+// complex64 copy of S in Y that the tile passes read as the stage-0 input;
+// within a run each fused pass writes both for the next stage input.  One
+// block per slice (p1, q1) = (X, Y), blocks in the bucket's tile order (spin 0
+// tiles).
 __global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
     const int64_t b = blockIdx.x;
     int lo = 0, hi = a.k.n - 1;
@@ -1361,6 +1437,13 @@ __global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
         const int64_t dst = ((int64_t)X * N + Y) * N + Z;
         for (int s = 0; s < 2; ++s) gstore(d.D[s], dst, gload(d.S2[s], src));
     }
+    if (a.compact)  // elements (X, p2, Y, q2): N rows of N contiguous q2
+        for (int s = 0; s < 2; ++s)
+            for (int p2 = 0; p2 < N; ++p2)
+                for (int q2 = threadIdx.x; q2 < N; q2 += blockDim.x) {
+                    const int64_t e = (int64_t)X * N2 * N + (int64_t)p2 * N2 + (int64_t)Y * N + q2;
+                    cstore(d.Y2[s], e, gload(d.S2[s], e), true);
+                }
 }
 
 int device_cus() {
@@ -1638,11 +1721,12 @@ int tile_wave_options(int NP, int* opts) {
     return 3;
 }
 
-void launch_b_d_from_state(const BatchDev& b, const Bucket& k, hipStream_t s) {
+void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hipStream_t s) {
     if (k.tiles <= 0) return;
     BTileArgs a{};
     a.b = b;
     a.k = k;
+    a.compact = precision != G12_PREC_FP64;
     hipLaunchKernelGGL(k_d_from_state, dim3((unsigned)(k.tiles / 2)), dim3(64), 0, s, a);
 }
 
