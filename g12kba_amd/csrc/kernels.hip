@@ -880,7 +880,7 @@ struct BTileArgs {
     int nspin;
     int use_y;        // stage input from the Y buffer (after stage 0, or always when compact; 0: always S)
     int compact;      // R and Y held as complex64 (mixed-precision runs)
-    int sources;      // runs: each pass adds its share of the G2 sources (tile_sources)
+    int sources;      // runs: each pass adds its share of the G2 sources (see SrcVec)
     const double2* h_override;  // routine calls: h to use instead of the trajectory's h
 };
 

@@ -104,6 +104,38 @@ inline void G2_step(std::vector<std::complex<double>>& h2G2_comm, std::vector<st
 // fermi_func.h:21
 inline double fermi(double t, double t0, double tau) { return g12_fermi(t, t0, tau); }
 
+// init_G.h:57.  The reference builds h from G1u through h_HFx_quench, whose
+// interaction terms vanish for the zero G1u that Main.cpp:49-50 passes; the
+// eigenproblem runs at the reference's FP32 precision (LAPACKE_cheev,
+// init_G.h:31-38).  U and t3 do not enter.
+inline void make_G0(std::vector<std::complex<double>>& G1u, int Ns, int Nb, int p_number, double U, double t1,
+                    double t2, double t3, std::vector<double>& epsilon) {
+    (void)U;
+    (void)t3;
+    g12_params p;
+    g12_params_default(&p);
+    p.Ns = Ns;
+    p.Nb = Nb;
+    p.p_number = p_number;
+    p.t1 = t1;
+    p.t2 = t2;
+    for (int a = 0; a < Nb && a < G12_MAX_NB; ++a) p.eps[a] = a < (int)epsilon.size() ? epsilon[a] : 0.0;
+    G1u.assign((size_t)Ns * Nb * Ns * Nb, 0.0);
+    g12kba_compat::check(g12_make_g0(&p, g12kba_compat::d(G1u), 1), "make_G0");
+}
+
+// init_G.h:78 (the domain wall; G1u is sized by the caller, as in the reference)
+inline void init_G_test(std::vector<std::complex<double>>& G1u, int Ns, int Nb) {
+    g12_params p;
+    g12_params_default(&p);
+    p.Ns = Ns;
+    p.Nb = Nb;
+    std::vector<std::complex<double>> g((size_t)Ns * Nb * Ns * Nb);
+    g12kba_compat::check(g12_init_g_test(&p, g12kba_compat::d(g)), "init_G_test");
+    for (size_t e = 0; e < g.size() && e < G1u.size(); ++e)
+        if (g[e] != 0.0) G1u[e] = g[e];
+}
+
 // dipole.h:25
 inline void dipole(std::vector<double>& p, std::vector<std::vector<std::complex<double>>>& G) {
     if (G.empty()) return;

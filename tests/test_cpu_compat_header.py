@@ -1,5 +1,5 @@
 """The C++ compatibility header compiles against the C ABI and links the library
-(host-only calls: fermi and dipole; no GPU work)."""
+(host-only calls: fermi, dipole, make_G0 and init_G_test; no GPU work)."""
 import os
 import subprocess
 import tempfile
@@ -15,6 +15,13 @@ int main() {
     std::vector<double> p(3, 0.0);
     dipole(p, G);
     std::printf("%.6f %.6f\n", fermi(10.0, 10.0, 0.5), p[1]);
+    std::vector<double> eps(2, 0.0);
+    std::vector<std::complex<double>> g0, gt(36);
+    make_G0(g0, 3, 2, 2, 1.0, 1.0, 0.5, 0.0, eps);   // 2 of 6 levels filled
+    init_G_test(gt, 3, 2);                           // site 0 of both orbitals
+    double tr0 = 0, trt = 0;
+    for (int i = 0; i < 6; ++i) { tr0 += g0[i * 7].imag(); trt += gt[i * 7].imag(); }
+    std::printf("%.6f %.6f %.1f %.1f\n", tr0, trt, gt[0].imag(), gt[3 * 7].imag());
     return 0;
 }
 '''
@@ -30,6 +37,9 @@ def test_compat_header_compiles_and_links():
                            f"-Wl,-rpath,{lib}", "-o", exe])
     out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stderr
-    f, p1 = map(float, out.stdout.split())
+    lines = out.stdout.split("\n")
+    f, p1 = map(float, lines[0].split())
     assert f == 0.5
     assert abs(p1 - 0.5) < 1e-12  # N = 2: 2*(1 - 0 - 0.5)*(1 - 0)/2
+    tr0, trt, g00, g33 = map(float, lines[1].split())
+    assert abs(tr0 - 2) < 1e-6 and trt == 2 and g00 == 1 and g33 == 1
