@@ -9,10 +9,11 @@ symmetric perturbation.
 A "step" is one RK4 step (4 stages) of the production protocol; inputs are
 resident in HBM before the timed region.
 
-Prints one JSON line (bench contract).  `roofline` covers the dominant kernel
-(the two-particle commutator tile passes), timed with HIP events on the
-library's stream over the timed region; `cpu_baseline` times the project's
-faithful-complexity C++ restatement (OpenMP) on a bounded sample.
+Prints one JSON line (bench contract).  `value` times the steps as a run
+executes them (one captured hipGraph per step); `roofline` covers the dominant
+kernel (the two-particle commutator tile passes), timed with HIP events on the
+library's stream over a second run of as many steps; `cpu_baseline` times the
+project's faithful-complexity C++ restatement (OpenMP) on a bounded sample.
 """
 import argparse
 import json
@@ -80,13 +81,15 @@ PEAK = {"fp64": (78.6, "FP64 dense"), "fp32_3m": (157.3, "FP32 dense"), "bf16x3"
 
 
 def run_leg(g, precision, G1, Gxx, Gxy, steps, warmup):
-    """Times `steps` RK4 steps of one precision mode; returns (ms/step, G1 after, kernel timings)."""
+    """Times `steps` RK4 steps of one precision mode as a run executes them by
+    default (each step one captured hipGraph), then runs as many steps again
+    with HIP events around every launch for the kernel breakdown; returns
+    (ms/step of the timed steps, G1 after them, kernel timings)."""
     p = workload_params(g, precision)
     prop = g.Propagator(p)
     prop.set_state(G1, Gxx, Gxy)
     if warmup:
         prop.advance(warmup)
-    prop.timing(True)
     g.synchronize()
     t0 = time.perf_counter()
     rc = prop.advance(steps)
@@ -94,10 +97,14 @@ def run_leg(g, precision, G1, Gxx, Gxy, steps, warmup):
     t1 = time.perf_counter()
     if rc != 0:
         raise SystemExit(f"density guard tripped (rc={rc}, {precision})")
+    G1_end = prop.get_G1()
+    prop.timing(True)
+    rc = prop.advance(steps)
     k = [prop.timing_get(i) for i in range(3)]
     prop.timing(False)
-    G1_end = prop.get_G1()
     prop.close()
+    if rc != 0:
+        raise SystemExit(f"density guard tripped (rc={rc}, {precision})")
     return 1e3 * (t1 - t0) / steps, G1_end, k
 
 

@@ -405,16 +405,15 @@ void timed(Run& r, int cls, F&& launch) {
 }
 
 // One RK4 stage of every trajectory (RK_solver.h:120-203 / AS.h:111-200):
-// G1 stage set-up, Q/F tables and collision integral, strided commutator pass
-// with the sources whose (p1, q1) dependence is a vector (Psi W and Q terms,
-// Pi^+), G1 right-hand side and update, fused commutator pass with the
-// remaining sources (Psi Qp term, Pi) and the G2 update.
+// G1 stage set-up together with the Q/F tables and collision integral,
+// strided commutator pass with the sources that are vectors over p1 and q1
+// (Psi W and Q terms, Pi^+), G1 right-hand side and update, fused commutator
+// pass with the remaining sources (Psi Qp term, Pi) and the G2 update.
 void run_stage(Run& r, const StageRef& sr) {
     hipStream_t s = r.stream;
-    timed(r, 2, [&] { g12::launch_b_stage_g1(r.b, sr, s); });
-    // the tables come from the stage G1 and the partial traces D of the stage
-    // input; both tile passes add their share of the sources from them
-    if (r.any_corr) timed(r, 2, [&] { g12::launch_b_tables(r.b, sr, r.precision, s); });
+    // G1 set-up and the tables (from the stage G1 and the partial traces D of
+    // the stage input; both tile passes add their share of the sources from them)
+    timed(r, 2, [&] { g12::launch_b_stage_setup(r.b, sr, r.precision, s); });
     for (const Bucket& k : r.buckets) timed(r, 0, [&] { g12::launch_b_tile(r.b, k, sr, 0, r.precision, s); });
     timed(r, 2, [&] { g12::launch_b_g1_rhs(r.b, sr, s); });
     for (const Bucket& k : r.buckets) timed(r, 1, [&] { g12::launch_b_tile(r.b, k, sr, 1, r.precision, s); });

@@ -115,8 +115,7 @@ struct Bucket {
 };
 
 // batched stage kernels (kernels.hip)
-void launch_b_stage_g1(const BatchDev& b, const StageRef& sr, hipStream_t s);
-void launch_b_tables(const BatchDev& b, const StageRef& sr, int precision, hipStream_t s);
+void launch_b_stage_setup(const BatchDev& b, const StageRef& sr, int precision, hipStream_t s);
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_record(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int fused, int precision, hipStream_t s);

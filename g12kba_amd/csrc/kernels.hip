@@ -259,25 +259,15 @@ __device__ inline bool active(const DevState& d, const TrajParams& t, const Stag
 }
 
 // Stage set-up for G1: Y1 = S1 + c K1, the pairs (Y1[X][Y], conj Y1[Y][X])
-// read by the strided pass, h = h_HF[Y1], W tables.  Blocks of
-// 256 elements per trajectory; each block forms the diagonal of Y1 itself (for
-// the Hartree sums) and takes the Y1 elements of its Fock terms from S1 and
-// K1, so the blocks need no ordering among themselves.
-__global__ __launch_bounds__(256) void k_stage_g1(BatchDev b, StageRef r) {
-    const int ti = blockIdx.y;
-    const DevState& d = b.ds[ti];
-    const TrajParams& tp = b.tp[ti];
-    const StageShared ss = stage_of(r);
-    if (!active(d, tp, ss)) return;
+// read by the strided pass, h = h_HF[Y1], W tables.  Blocks of 256 elements
+// (`chunk`) per trajectory; each block forms the diagonal of Y1 itself (for
+// the Hartree sums, in `diag`) and takes the Y1 elements of its Fock terms
+// from S1 and K1, so the blocks need no ordering among themselves.
+__device__ inline void stage_g1_block(const DevState& d, const StageShared& ss, const StageParams& sp, int chunk,
+                                      double2* diag) {
     const int N = d.N;
     const int n2 = N * N;
-    const int e0 = blockIdx.x * blockDim.x;
-    if (e0 >= n2) return;
-    // built per block in LDS: the V table is indexed at run time and would
-    // otherwise live in per-lane scratch
-    __shared__ StageParams sp;
-    __shared__ double2 diag[G12_MAX_NB * 64];
-    fill_sp(sp, tp, ss);
+    const int e0 = chunk * blockDim.x;
     const double2* S1 = uni_ptr(d.S1);
     const double2* K1 = uni_ptr(d.K1);
     const bool first = ss.first != 0;
@@ -313,16 +303,22 @@ constexpr int TMAX = 64;
 __device__ inline int table_groups(int N) { return (N + TG - 1) / TG; }
 
 // Qp^s[X][Y][Z] = Q^s[Z][X][Y] = sum_w Y1[X][w] W^s[Z][w] Y1[w][Y] for
-// s = xx (index 0, W_xx) and xy (index 1, W_xy), Z in [Z0, Z0 + TG).
-__device__ inline void q_tables_block(const DevState& d, int X, int Z0) {
-    __shared__ double2 Gs[TMAX * TMAX];  // Y1
-    __shared__ double wz[TG * TMAX];     // W rows of the block's Z
+// s = xx (index 0, W_xx) and xy (index 1, W_xy), Z in [Z0, Z0 + TG).  The
+// stage G1 Y1 = S1 + c K1 and W are formed here as the G1 part forms them.
+// Gs: TMAX x TMAX complex (Y1), wz: TG x TMAX (the W rows of the block's Z).
+__device__ inline void q_tables_block(const DevState& d, const StageShared& ss, const StageParams& sp, int X, int Z0,
+                                      double2* Gs, double* wz) {
     const int N = d.N;
-    const double2* G = d.Y1;
-    for (int e = threadIdx.x; e < N * N; e += blockDim.x) Gs[e] = G[e];
+    const double2* S1 = uni_ptr(d.S1);
+    const double2* K1 = uni_ptr(d.K1);
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        double2 y = gload(S1, e);
+        if (!ss.first) y = cadd(y, cscale(gload(K1, e), ss.c));
+        Gs[e] = y;
+    }
     for (int e = threadIdx.x; e < TG * N; e += blockDim.x) {
         const int Z = Z0 + e / N, w = e % N;
-        wz[e] = Z < N ? d.Wxy[(int64_t)w * N + Z] : 0.0;  // W_xy is symmetric
+        wz[e] = Z < N ? w_xy(w, Z, d.Ns, sp) : 0.0;
     }
     __syncthreads();
     const double2* u = Gs + X * N;  // Y1[X][w]
@@ -348,7 +344,8 @@ __device__ inline void q_tables_block(const DevState& d, int X, int Z0) {
 // F^xy[X][Y][Z] = -i sum_w ( W_xy[Z][w] D^xx[X][Y][w] + W_xx[Z][w] D^xy[X][Y][w] )
 // (Pi_xyxy with the swapped G2 pair, pi.h:25-95 / RK_solver.h:163-164) and
 // I[X][Y] = F^xx[X][Y][X] (collision_int, collision_integral.h:27-69), for Y
-// in [Y0, Y0 + TG).
+// in [Y0, Y0 + TG), with W_xy formed from the stage parameters.  Ws: TMAX x
+// TMAX doubles, dxx and dxy: TG x TMAX complex (the D rows of the block's Y).
 // With `herm` (mixed-precision runs) D enters through its Hermitian part
 // (D[X][Y][Z] + conj D[Y][X][Z]) / 2.  A Hermitian G2 (the physical states:
 // G2 evolved from zero stays Hermitian) has D[X][Y][Z] = conj D[Y][X][Z], and
@@ -356,11 +353,10 @@ __device__ inline void q_tables_block(const DevState& d, int X, int Z0) {
 // or BF16 tile products break it slightly, the average restores particle-number
 // conservation to FP64 level.  FP64 runs use D as it is, as the reference does,
 // for any input.
-__device__ inline void f_tables_block(const DevState& d, int X, int Y0, bool herm) {
-    __shared__ double Ws[TMAX * TMAX];                   // W_xy
-    __shared__ double2 dxx[TG * TMAX], dxy[TG * TMAX];  // D rows of the block's Y
+__device__ inline void f_tables_block(const DevState& d, const StageParams& sp, int X, int Y0, bool herm, double* Ws,
+                                      double2* dxx, double2* dxy) {
     const int N = d.N;
-    for (int e = threadIdx.x; e < N * N; e += blockDim.x) Ws[e] = d.Wxy[e];
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) Ws[e] = w_xy(e / N, e % N, d.Ns, sp);
     for (int e = threadIdx.x; e < TG * N; e += blockDim.x) {
         const int Y = Y0 + e / N, w = e % N;
         double2 a = make_double2(0.0, 0.0), b = a;
@@ -413,22 +409,48 @@ __device__ inline int table_block(const BatchDev& b, int64_t blk, int& e) {
     return b.tlist[lo];
 }
 
-__global__ __launch_bounds__(256) void k_q_tables(BatchDev b, StageRef r) {
-    int e = 0;
-    const int ti = table_block(b, blockIdx.x, e);
-    const DevState& d = b.ds[ti];
-    if (!active(d, b.tp[ti], stage_of(r))) return;
-    const int ng = table_groups(d.N);
-    q_tables_block(d, e / ng, (e % ng) * TG);
-}
+// Stage set-up of every trajectory in one launch: the G1 part (Y1 = S1 + c K1,
+// the G1 pairs, h = h_HF[Y1], W tables) and the two table families, which
+// form Y1 and W themselves, so that none of the three waits for another.
+// Blocks [0, gA * ntraj): G1 part, gA blocks of 256 elements per trajectory;
+// then tblocks Qp blocks and tblocks F blocks.
+constexpr int SETUP_LDS = TMAX * TMAX * (int)sizeof(double2) + TG * TMAX * (int)sizeof(double);
+static_assert(TMAX * TMAX * sizeof(double) + 2 * TG * TMAX * sizeof(double2) <= SETUP_LDS, "F part LDS");
 
-__global__ __launch_bounds__(256) void k_f_tables(BatchDev b, StageRef r, int herm) {
-    int e = 0;
-    const int ti = table_block(b, blockIdx.x, e);
+__global__ __launch_bounds__(256) void k_stage_setup(BatchDev b, StageRef r, int herm, int gA) {
+    __shared__ StageParams sp;
+    __shared__ __align__(16) char lds[SETUP_LDS];
+    const StageShared ss = stage_of(r);
+    const int64_t nA = (int64_t)gA * b.ntraj, blk = blockIdx.x;
+    int ti, e, role;
+    if (blk < nA) {
+        role = 0;
+        ti = (int)(blk / gA);
+        e = (int)(blk % gA);
+    } else {
+        role = blk < nA + b.tblocks ? 1 : 2;
+        ti = table_block(b, role == 1 ? blk - nA : blk - nA - b.tblocks, e);
+    }
     const DevState& d = b.ds[ti];
-    if (!active(d, b.tp[ti], stage_of(r))) return;
-    const int ng = table_groups(d.N);
-    f_tables_block(d, e / ng, (e % ng) * TG, herm != 0);
+    const TrajParams& tp = b.tp[ti];
+    if (!active(d, tp, ss)) return;
+    if (role == 0 && e * (int)blockDim.x >= d.N * d.N) return;
+    // built per block in LDS: the V table is indexed at run time and would
+    // otherwise live in per-lane scratch
+    fill_sp(sp, tp, ss);
+    __syncthreads();
+    if (role == 0) {
+        stage_g1_block(d, ss, sp, e, reinterpret_cast<double2*>(lds));
+    } else {
+        const int ng = table_groups(d.N);
+        if (role == 1)
+            q_tables_block(d, ss, sp, e / ng, (e % ng) * TG, reinterpret_cast<double2*>(lds),
+                           reinterpret_cast<double*>(lds + TMAX * TMAX * sizeof(double2)));
+        else
+            f_tables_block(d, sp, e / ng, (e % ng) * TG, herm != 0, reinterpret_cast<double*>(lds),
+                           reinterpret_cast<double2*>(lds + TMAX * TMAX * sizeof(double)),
+                           reinterpret_cast<double2*>(lds + TMAX * TMAX * sizeof(double) + TG * TMAX * sizeof(double2)));
+    }
 }
 
 // G1 right-hand side: comm = [h, Y1], K = -i(comm + I + I^dag), RK
@@ -1687,14 +1709,11 @@ inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b
 // ---------------------------------------------------------------------------
 // launchers
 
-void launch_b_stage_g1(const BatchDev& b, const StageRef& r, hipStream_t s) {
-    hipLaunchKernelGGL(k_stage_g1, dim3((b.maxN * b.maxN + 255) / 256, b.ntraj), dim3(256), 0, s, b, r);
-}
-
-void launch_b_tables(const BatchDev& b, const StageRef& r, int precision, hipStream_t s) {
-    if (b.tblocks <= 0) return;
-    hipLaunchKernelGGL(k_q_tables, dim3((unsigned)b.tblocks), dim3(256), 0, s, b, r);
-    hipLaunchKernelGGL(k_f_tables, dim3((unsigned)b.tblocks), dim3(256), 0, s, b, r, precision != G12_PREC_FP64 ? 1 : 0);
+void launch_b_stage_setup(const BatchDev& b, const StageRef& r, int precision, hipStream_t s) {
+    const int gA = (b.maxN * b.maxN + 255) / 256;
+    const int64_t blocks = (int64_t)gA * b.ntraj + 2 * b.tblocks;
+    hipLaunchKernelGGL(k_stage_setup, dim3((unsigned)blocks), dim3(256), 0, s, b, r,
+                       precision != G12_PREC_FP64 ? 1 : 0, gA);
 }
 
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& r, hipStream_t s) {
