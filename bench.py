@@ -28,7 +28,7 @@ METRIC = "G2 RK4 steps/s at N=64 (FP64 vs mixed @≤1e-6 G1 err); batched traj/s
 # HBM bytes per launch of the dominant kernel (read + written), measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload and calibrated with a
 # plain copy kernel (bytes read = 2 x FETCH_SIZE KiB, written = WRITE_SIZE KiB).
-TRAFFIC_BYTES = {("fp64", "tile_fused"): 3.04e9, ("fp64", "tile_strided"): 1.10e9}
+TRAFFIC_BYTES = {("fp64", "tile_fused"): 3.01e9, ("fp64", "tile_strided"): 1.10e9}
 
 
 def workload_params(g, precision="fp64", time_steps=5000):
