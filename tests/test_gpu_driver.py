@@ -41,10 +41,11 @@ def test_rk4_matches_oracle(Ns, Nb, ehm, law, qt):
         assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(b).max())
 
 
-@pytest.mark.parametrize("Ns,Nb", [(1, 1), (1, 2), (1, 3), (2, 1)])
+@pytest.mark.parametrize("Ns,Nb", [(1, 1), (1, 2), (1, 3), (2, 1), (1, 8), (2, 8)])
 def test_tiny_geometries_match_oracle(Ns, Nb):
     """Edge sizes N = 1..3 (one site: no hopping, no long-range shells; one
-    orbital: no t2), padded to 16-wide tiles on the device."""
+    orbital: no t2), padded to 16-wide tiles on the device, and the largest
+    orbital count (Nb = 8)."""
     p = g.make_params(Ns=Ns, Nb=Nb, t1=1.0, t2=0.4, U=1.4, dt=0.02, time_steps=12, quench_strength=1.3,
                       Nq=max(1, Ns // 2), q_type="pulse", quench_rate=2.0, quench_on_midpoint=0.05,
                       quench_off_midpoint=0.15, quench_gate_time=0.06, EHM=True, V_law="coulomb")
