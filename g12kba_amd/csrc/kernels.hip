@@ -682,13 +682,14 @@ struct Engine<NP, NW, G12_PREC_FP64> {
     }
 };
 
-// 3M complex products on FP32 MFMA: with S = re + im planes,
+// 3M complex products on FP32 MFMA: with S = re + im (formed in registers
+// from the re and im operands, so LDS holds two planes per matrix),
 //   P1 = Ar Br, P2 = Ai Bi, P3 = As Bs;  Re = P1 - P2, Im = P3 - P1 - P2.
 template <int NP>
 struct TileLds32 {
     static constexpr int P = NP + 1;
-    float xr[NP * P], xi[NP * P], xs[NP * P];
-    float hr[NP * P], hi[NP * P], hs[NP * P];
+    float xr[NP * P], xi[NP * P];
+    float hr[NP * P], hi[NP * P];
 };
 
 template <int NP, int NW>
@@ -699,19 +700,16 @@ struct Engine<NP, NW, G12_PREC_FP32_3M> {
     __device__ static void put_h(Lds& L, int m, int n, double2 v) {
         L.hr[m * P + n] = (float)v.x;
         L.hi[m * P + n] = (float)v.y;
-        L.hs[m * P + n] = (float)(v.x + v.y);
     }
     __device__ static void put_x(Lds& L, int m, int n, double2 v) {
         L.xr[m * P + n] = (float)v.x;
         L.xi[m * P + n] = (float)v.y;
-        L.xs[m * P + n] = (float)(v.x + v.y);
     }
     __device__ static void zero(Lds&, int, int) {}
     __device__ static int row_of(int lane, int j) { return 4 * (lane >> 4) + j; }
     template <bool NEG>
-    __device__ static void product(const float* ar_, const float* ai_, const float* as_, const float* br_,
-                                   const float* bi_, const float* bs_, f4 (&p1)[PW], f4 (&p2)[PW], f4 (&p3)[PW],
-                                   int wave, int lane) {
+    __device__ static void product(const float* ar_, const float* ai_, const float* br_, const float* bi_,
+                                   f4 (&p1)[PW], f4 (&p2)[PW], f4 (&p3)[PW], int wave, int lane) {
         constexpr bool FULL = (TT % NW) == 0;
         constexpr int KS = NP / 4;
         const int r = lane & 15, kb = (lane >> 4) * KS;
@@ -726,10 +724,10 @@ struct Engine<NP, NW, G12_PREC_FP32_3M> {
                 const int mm = (t / NT) * 16 + r, nn = (t % NT) * 16 + r;
                 a1[u] = on ? ar_[mm * P + kk] : 0.f;
                 a2[u] = on ? ai_[mm * P + kk] : 0.f;
-                a3[u] = on ? as_[mm * P + kk] : 0.f;
                 b1[u] = on ? br_[kk * P + nn] : 0.f;
                 b2[u] = on ? bi_[kk * P + nn] : 0.f;
-                b3[u] = on ? bs_[kk * P + nn] : 0.f;
+                a3[u] = a1[u] + a2[u];
+                b3[u] = b1[u] + b2[u];
             }
 #pragma unroll
             for (int u = 0; u < PW; ++u) p1[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(NEG ? -a1[u] : a1[u], b1[u], p1[u], 0, 0, 0);
@@ -749,13 +747,13 @@ struct Engine<NP, NW, G12_PREC_FP32_3M> {
             p2[u] = p1[u];
             p3[u] = p1[u];
         }
-        if (hmode & 1) product<false>(L.hr, L.hi, L.hs, L.xr, L.xi, L.xs, p1, p2, p3, wave, lane);
+        if (hmode & 1) product<false>(L.hr, L.hi, L.xr, L.xi, p1, p2, p3, wave, lane);
         // v enters Re = P1 - P2 through P1 and Im = P3 - P1 - P2 through P3
         mid([&](int u, int j, double2 v) {
             p1[u][j] += (float)v.x;
             p3[u][j] += (float)(v.x + v.y);
         });
-        if (hmode & 2) product<true>(L.xr, L.xi, L.xs, L.hr, L.hi, L.hs, p1, p2, p3, wave, lane);
+        if (hmode & 2) product<true>(L.xr, L.xi, L.hr, L.hi, p1, p2, p3, wave, lane);
 #pragma unroll
         for (int u = 0; u < PW; ++u)
 #pragma unroll
