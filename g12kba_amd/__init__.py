@@ -99,6 +99,8 @@ def _load():
         "g12_write_outputs": [ctypes.c_char_p, p, i64, i, p, d],
         "g12_save_state": [vp, ctypes.c_char_p],
         "g12_load_state": [vp, ctypes.c_char_p],
+        "g12_batch_save_state": [vp, i, ctypes.c_char_p],
+        "g12_batch_load_state": [vp, i, ctypes.c_char_p],
         "g12_configure": [vp, i, i],
         "g12_batch_create": [ctypes.POINTER(Params), i, ctypes.POINTER(vp)],
         "g12_batch_destroy": [vp],
@@ -155,7 +157,7 @@ EXPORTED = [
     "g12_get_time", "g12_rk4_record", "g12_timing_enable", "g12_timing_get", "g12_memory_bytes", "g12_synchronize",
     "g12_hf_hamiltonian", "g12_h1_commutator", "g12_h2_commutator", "g12_collision",
     "g12_psi_xxxx", "g12_psi_xyxy", "g12_pi", "g12_g1_step", "g12_g2_step", "g12_fermi",
-    "g12_dipole", "g12_energy", "g12_state_energy", "g12_batch_state_energy", "g12_write_outputs", "g12_save_state", "g12_load_state", "g12_version",
+    "g12_dipole", "g12_energy", "g12_state_energy", "g12_batch_state_energy", "g12_write_outputs", "g12_save_state", "g12_load_state", "g12_batch_save_state", "g12_batch_load_state", "g12_version",
     "g12_configure", "g12_batch_create", "g12_batch_destroy", "g12_batch_size",
     "g12_batch_memory_bytes", "g12_batch_set_state", "g12_batch_get_state",
     "g12_batch_adiabatic_switching", "g12_batch_rk4", "g12_batch_rk4_advance", "g12_batch_rk4_record", "g12_batch_guard",
@@ -660,6 +662,14 @@ class Batch:
 
     def tune(self):
         _check(lib.g12_batch_tune(self.h), "g12_batch_tune")
+
+    def save(self, i, path):
+        """Binary checkpoint of trajectory i (with the batch's time and step)."""
+        _check(lib.g12_batch_save_state(self.h, i, path.encode()), "g12_batch_save_state")
+
+    def load(self, i, path):
+        """Loads trajectory i; the batch's time and step become the file's."""
+        _check(lib.g12_batch_load_state(self.h, i, path.encode()), "g12_batch_load_state")
 
     def energy(self, i):
         """Propagator.energy of trajectory i."""

@@ -1083,6 +1083,30 @@ int state_energy(Run& r, int i, double* E) {
     return 0;
 }
 
+int save_traj(Run& r, int i, const char* path) {
+    if (!path || i < 0 || i >= r.ntraj) return G12_E_ARG;
+    const DevState& d = r.ds[i];
+    std::vector<double> g1(2 * d.n2), gxx(2 * d.n4), gxy(2 * d.n4);
+    int rc = get_state(r, i, g1.data(), gxx.data(), gxy.data());
+    if (rc) return rc;
+    return g12io::save_checkpoint(path, d.Ns, d.Nb, r.t, r.steps, g1.data(), gxx.data(), gxy.data());
+}
+
+int load_traj(Run& r, int i, const char* path) {
+    if (!path || i < 0 || i >= r.ntraj) return G12_E_ARG;
+    const DevState& d = r.ds[i];
+    std::vector<double> g1(2 * d.n2), gxx(2 * d.n4), gxy(2 * d.n4);
+    double t = 0;
+    int64_t steps = 0;
+    int rc = g12io::load_checkpoint(path, d.Ns, d.Nb, &t, &steps, g1.data(), gxx.data(), gxy.data());
+    if (rc) return rc;
+    rc = set_state(r, i, g1.data(), gxx.data(), gxy.data());
+    if (rc) return rc;
+    r.t = t;
+    r.steps = steps;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1306,28 +1330,12 @@ int g12_batch_state_energy(g12_batch* b, int i, double* E) { return b ? state_en
 // ---------------------------------------------------------------------------
 // checkpoints: magic, N, Ns, Nb, t, steps, G1, G2xx, G2xy (reference layouts)
 
-int g12_save_state(g12_ctx* c, const char* path) {
-    if (!c || !path) return G12_E_ARG;
-    const DevState& d = c->r.ds[0];
-    std::vector<double> g1(2 * d.n2), gxx(2 * d.n4), gxy(2 * d.n4);
-    int rc = g12_get_state(c, g1.data(), gxx.data(), gxy.data());
-    if (rc) return rc;
-    return g12io::save_checkpoint(path, d.Ns, d.Nb, c->r.t, c->r.steps, g1.data(), gxx.data(), gxy.data());
-}
+int g12_save_state(g12_ctx* c, const char* path) { return c ? save_traj(c->r, 0, path) : G12_E_ARG; }
 
-int g12_load_state(g12_ctx* c, const char* path) {
-    if (!c || !path) return G12_E_ARG;
-    const DevState& d = c->r.ds[0];
-    std::vector<double> g1(2 * d.n2), gxx(2 * d.n4), gxy(2 * d.n4);
-    double t = 0;
-    int64_t steps = 0;
-    int rc = g12io::load_checkpoint(path, d.Ns, d.Nb, &t, &steps, g1.data(), gxx.data(), gxy.data());
-    if (rc) return rc;
-    rc = g12_set_state(c, g1.data(), gxx.data(), gxy.data());
-    if (rc) return rc;
-    c->r.t = t;
-    c->r.steps = steps;
-    return 0;
-}
+int g12_load_state(g12_ctx* c, const char* path) { return c ? load_traj(c->r, 0, path) : G12_E_ARG; }
+
+int g12_batch_save_state(g12_batch* b, int i, const char* path) { return b ? save_traj(b->r, i, path) : G12_E_ARG; }
+
+int g12_batch_load_state(g12_batch* b, int i, const char* path) { return b ? load_traj(b->r, i, path) : G12_E_ARG; }
 
 }  // extern "C"

@@ -323,6 +323,11 @@ int g12_write_outputs(const char* dir, const double* traj, int64_t T, int N, con
  * reference layouts. */
 int g12_save_state(g12_ctx* ctx, const char* path);
 int g12_load_state(g12_ctx* ctx, const char* path);
+/* The same for trajectory i of a batch.  The trajectories of a batch share
+ * one clock: loading sets the batch's time and step count to the file's.  No
+ * reference counterpart (the reference runs one trajectory per process). */
+int g12_batch_save_state(g12_batch* b, int i, const char* path);
+int g12_batch_load_state(g12_batch* b, int i, const char* path);
 
 /* Library version string. */
 const char* g12_version(void);

@@ -165,3 +165,38 @@ def test_mixed_precision_batch_error(prec):
     for x, y in zip(tr1, tr2):
         dens = np.abs(np.diagonal(x - y, axis1=1, axis2=2).imag).max()
         assert 0 < dens < 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_3m"])
+def test_batch_checkpoint_resume_is_bit_exact(tmp_path, prec):
+    """Every trajectory of a batch saved mid-run and loaded into a new batch:
+    continuing gives the uninterrupted batch's trajectories bit for bit."""
+    sizes = [(6, 2), (9, 1), (4, 3)]
+    ps = ensemble(6, 3, 40, sizes=sizes, precision=prec)
+    states = [start(p) for p in ps]
+
+    def new_batch():
+        b = g.Batch(ps)
+        for i, s in enumerate(states):
+            b.set_state(i, *s)
+        return b
+
+    with new_batch() as b:
+        b.adiabatic_switching()
+        rc, full = b.record(25)
+        end = [b.get_state(i) for i in range(len(ps))]
+    with new_batch() as b:
+        b.adiabatic_switching()
+        rc1, first = b.record(10)
+        for i in range(len(ps)):
+            b.save(i, str(tmp_path / f"t{i}.bin"))
+    with g.Batch(ps) as b:
+        for i in range(len(ps)):
+            b.load(i, str(tmp_path / f"t{i}.bin"))
+        rc2, rest = b.record(15)
+        end2 = [b.get_state(i) for i in range(len(ps))]
+    assert rc == rc1 == rc2 == 0
+    for i in range(len(ps)):
+        assert np.array_equal(np.concatenate([first[i], rest[i]]), full[i]), f"trajectory {i}"
+        for x, y in zip(end[i], end2[i]):
+            assert np.array_equal(x, y), f"trajectory {i}"
