@@ -335,6 +335,7 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
         k.NP = NP;
         k.n = (int)l.size();
         k.tiles = pre.back();
+        k.full = r.tp[l.back()].N == NP ? 1 : 0;  // l is sorted by N, largest first
         k.waves = k.waves_fused = g12::default_waves(NP);
         r.buckets.push_back(k);
         r.lists.push_back(l);

@@ -112,6 +112,7 @@ struct Bucket {
     int waves_fused;         // waves per workgroup, fused pass (autotuned)
     int wg_per_cu;           // workgroups per CU cap, strided pass (0: as many as LDS and waves allow)
     int wg_per_cu_fused;     // the same for the fused pass
+    int full;                // every listed trajectory has N = NP (no padded tile elements)
 };
 
 // batched stage kernels (kernels.hip)

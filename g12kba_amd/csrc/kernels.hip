@@ -1053,11 +1053,13 @@ __device__ __forceinline__ void tile_geometry(int tiling, int N, int tile, uint3
 // Stage input of one tile into registers (TILING: 0 strided, 1 contiguous,
 // -1 given at run time).  Plain loads: nothing waits for them before the
 // MFMA loop.
-template <int NP, int NW, int TILING>
+template <int NP, int NW, int TILING, bool FULL>
 __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef& t, const double2* X, bool xc,
                                               double2 (&pre)[TileCfg<NP, NW>::PF]) {
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
-    const int N = t.N;
+    // every slot holds a tile element only when the threads divide NP^2
+    constexpr bool EXACT = FULL && (NP * NP) % NTHR == 0;
+    const int N = FULL ? NP : t.N;
     uint32_t base, sm, sn;
     int t0, t1;
     tile_geometry(TILING < 0 ? a.tiling : TILING, N, t.tile, base, sm, sn, t0, t1);
@@ -1067,7 +1069,7 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
             double2 y = make_double2(0.0, 0.0);
-            if (e < N * N) y = cld(X, base + (uint32_t)(e / N) * sm + (uint32_t)(e % N) * sn, true);
+            if (EXACT || e < N * N) y = cld(X, base + (uint32_t)(e / N) * sm + (uint32_t)(e % N) * sn, true);
             pre[i] = y;
         }
     } else {
@@ -1075,7 +1077,7 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
             double2 y = make_double2(0.0, 0.0);
-            if (e < N * N) y = gld(X, base + (uint32_t)(e / N) * sm + (uint32_t)(e % N) * sn);
+            if (EXACT || e < N * N) y = gld(X, base + (uint32_t)(e / N) * sm + (uint32_t)(e % N) * sn);
             pre[i] = y;
         }
     }
@@ -1085,11 +1087,13 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
 // products take FP32 or BF16 operands): a compact stage input is loaded as it
 // is stored, with nothing waiting for it; a complex128 input (routine-level
 // calls only) is rounded on arrival.
-template <int NP, int NW, int TILING>
+template <int NP, int NW, int TILING, bool FULL>
 __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef& t, const double2* X, bool xc,
                                               fv2 (&pre)[TileCfg<NP, NW>::PF]) {
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
-    const int N = t.N;
+    // every slot holds a tile element only when the threads divide NP^2
+    constexpr bool EXACT = FULL && (NP * NP) % NTHR == 0;
+    const int N = FULL ? NP : t.N;
     uint32_t base, sm, sn;
     int t0, t1;
     tile_geometry(TILING < 0 ? a.tiling : TILING, N, t.tile, base, sm, sn, t0, t1);
@@ -1097,7 +1101,7 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
     for (int i = 0; i < PF; ++i) {
         const int e = threadIdx.x + NTHR * i;
         fv2 y = fv2{0.f, 0.f};
-        if (e < N * N) {
+        if (EXACT || e < N * N) {
             const uint32_t off = base + (uint32_t)(e / N) * sm + (uint32_t)(e % N) * sn;
             if (xc) {
                 y = *(const GLOBAL fv2*)((const GLOBAL char*)X + (off << 3));
@@ -1161,13 +1165,17 @@ __device__ __forceinline__ double2 lane_value(double2 v, int src) {
     return make_double2(__shfl(v.x, src), __shfl(v.y, src));
 }
 
-template <int NP, int NW, int PREC, bool FUSED>
+// FULL: every tile of the launch has N = NP, so the element bounds checks and
+// the divisions by N drop out at compile time.
+template <int NP, int NW, int PREC, bool FUSED, bool FULL>
 __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     typedef Engine<NP, NW, PREC> E;
     __shared__ typename E::Lds L;
     constexpr int NT = TileCfg<NP, NW>::NT, TT = TileCfg<NP, NW>::TT, PW = TileCfg<NP, NW>::PW;
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
     constexpr int TILING = FUSED ? 1 : -1;
+    constexpr bool ALLB = (TT % NW) == 0;                 // every wave owns PW output blocks
+    constexpr bool EXACT = FULL && (NP * NP) % NTHR == 0;  // every prefetch slot holds a tile element
     const StageShared ss = uni_stage(a.r);
     const TileSched sched(a.k.tiles, gridDim.x, blockIdx.x);
     E::zero(L, threadIdx.x, NTHR);
@@ -1199,7 +1207,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         const double2* X = (t.traj == v.traj && t.spin == v.spin)
                                ? v.X
                                : uni_ptr(stage_input(a.b.ds[t.traj], a, ss, t.spin));
-        tile_prefetch<NP, NW, TILING>(a, t, X, input_compact(a, ss), P);
+        tile_prefetch<NP, NW, TILING, FULL>(a, t, X, input_compact(a, ss), P);
     };
     prefetch(tr, P0);
     int64_t g1 = -1;  // DEEP: the tile after tr, in flight in the other buffer
@@ -1225,7 +1233,11 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     // the ones already in flight
     auto step = [&](PreT (&Pc)[PF]) -> bool {
         if (gt < 0) return false;
-        const int N = tr.N;
+        const int N = FULL ? NP : tr.N;
+        // element (m, n) of output block t lies inside the tile
+        auto inside = [&](int t, int m, int n) {
+            return (ALLB || t < TT) && (FULL || (m < N && n < N));
+        };
         lds_sync();  // previous tile's readers of X (and h) are done
         if (tr.traj != v.traj || tr.spin != v.spin) {
             const bool new_h = tr.traj != v.traj;
@@ -1248,7 +1260,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
-            if (e < N * N) E::put_x(L, e / N, e % N, to_d2(Pc[i]));
+            if (EXACT || e < N * N) E::put_x(L, e / N, e % N, to_d2(Pc[i]));
         }
         lds_sync();
         ++it;
@@ -1282,7 +1294,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                     for (int j = 0; j < 4; ++j) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
                         rpre[u][j] = spre[u][j] = make_double2(0.0, 0.0);
-                        if (t < TT && m < N && n < N) {
+                        if (inside(t, m, n)) {
                             const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
                             rpre[u][j] = cld(v.R, off, decltype(compact)::value);
                             spre[u][j] = gld(v.S, off);
@@ -1308,7 +1320,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
                         add(u, j, rpre[u][j]);
                         apre[u][j] = make_double2(0.0, 0.0);
-                        if (!ss.first && t < TT && m < N && n < N)
+                        if (!ss.first && inside(t, m, n))
                             apre[u][j] = gld(v.A, base + (uint32_t)m * sm + (uint32_t)n * sn);
                     }
                 }
@@ -1326,7 +1338,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
 #pragma unroll
                 for (int u = 0; u < PW; ++u) {
                     const int t = wave + NW * u;
-                    if (t >= TT) continue;
+                    if (!ALLB && t >= TT) continue;
                     const int n = (t % NT) * 16 + (lane & 15);
                     double2 bn{}, cn{};
                     if (a.sources) {  // permutes in wave-uniform control flow
@@ -1341,7 +1353,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                             am = lane_value(va, m);
                             cm = lane_value(vc, m);
                         }
-                        if (m < N && n < N) {
+                        if (inside(t, m, n)) {
                             const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
                             double2 val = make_double2(ar[u][j], ai[u][j]);
                             if (a.sources) {
@@ -1371,14 +1383,14 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
 #pragma unroll
                 for (int u = 0; u < PW; ++u) {
                     const int t = wave + NW * u;
-                    if (t >= TT) continue;
+                    if (!ALLB && t >= TT) continue;
                     const int n = (t % NT) * 16 + (lane & 15);
                     const double2 en = a.sources ? lane_value(ve, n) : double2{};
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
                         const double2 em = a.sources ? lane_value(ve, m) : double2{};
-                        if (m >= N || n >= N) continue;
+                        if (!inside(t, m, n)) continue;
                         const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
                         double2 rr = make_double2(ar[u][j], ai[u][j]);  // R + O
                         if (a.sources) rr = cadd(rr, cmul(gld(v.G1p, 2 * (uint32_t)(m * N + n)), csub(em, en)));
@@ -1477,7 +1489,7 @@ int device_cus() {
     return cus;
 }
 
-template <int NP, int NW, int PREC>
+template <int NP, int NW, int PREC, bool FULL>
 void launch_tile_cfg(const BTileArgs& a, hipStream_t s) {
     constexpr int lds = (int)sizeof(typename Engine<NP, NW, PREC>::Lds);
     int per_cu = (160 * 1024) / lds;
@@ -1490,9 +1502,11 @@ void launch_tile_cfg(const BTileArgs& a, hipStream_t s) {
     if (grid >= 8) grid &= ~7;
     if (grid < 1) return;
     if (a.fused)
-        hipLaunchKernelGGL((k_tile<NP, NW, PREC, true>), dim3((unsigned)grid), dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
+        hipLaunchKernelGGL((k_tile<NP, NW, PREC, true, FULL>), dim3((unsigned)grid), dim3(TileCfg<NP, NW>::NTHR), 0,
+                           s, a);
     else
-        hipLaunchKernelGGL((k_tile<NP, NW, PREC, false>), dim3((unsigned)grid), dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
+        hipLaunchKernelGGL((k_tile<NP, NW, PREC, false, FULL>), dim3((unsigned)grid), dim3(TileCfg<NP, NW>::NTHR), 0,
+                           s, a);
 }
 
 // Wave counts compiled for a padded tile size NP: W0, 2 W0 and 4 W0 with
@@ -1503,12 +1517,21 @@ constexpr int base_waves(int NP) { return NP == 16 ? 1 : NP == 32 ? 2 : 4; }
 template <int NP, int PREC>
 void launch_tile_np(const BTileArgs& a, hipStream_t s) {
     constexpr int W0 = base_waves(NP);
-    if (a.k.waves >= 4 * W0)
-        launch_tile_cfg<NP, 4 * W0, PREC>(a, s);
-    else if (a.k.waves >= 2 * W0)
-        launch_tile_cfg<NP, 2 * W0, PREC>(a, s);
-    else
-        launch_tile_cfg<NP, W0, PREC>(a, s);
+    if (a.k.waves >= 4 * W0) {
+        // buckets whose trajectories all have N = NP take the full-tile
+        // kernels, except BF16x3, whose fused pass measured 20% slower in them
+        if constexpr (PREC != G12_PREC_BF16X3) {
+            if (a.k.full) {
+                launch_tile_cfg<NP, 4 * W0, PREC, true>(a, s);
+                return;
+            }
+        }
+        launch_tile_cfg<NP, 4 * W0, PREC, false>(a, s);
+    } else if (a.k.waves >= 2 * W0) {
+        launch_tile_cfg<NP, 2 * W0, PREC, false>(a, s);
+    } else {
+        launch_tile_cfg<NP, W0, PREC, false>(a, s);
+    }
 }
 
 template <int PREC>
