@@ -322,21 +322,41 @@ __device__ inline void q_tables_block(const DevState& d, const StageShared& ss, 
     }
     __syncthreads();
     const double2* u = Gs + X * N;  // Y1[X][w]
-    for (int e = threadIdx.x; e < TG * N; e += blockDim.x) {
-        const int zi = e / N, Y = e % N, Z = Z0 + zi;
-        if (Z >= N) continue;
-        const double* wr = wz + zi * N;
-        double2 axy = make_double2(0.0, 0.0), axx = make_double2(0.0, 0.0);
+    // a thread owns column Y for the Z values zi and zi + TG/2 of the block:
+    // the product Y1[X][w] Y1[w][Y] is formed once per w for both, and each
+    // output sums its terms in the same order as one output alone would
+    static_assert(TG % 2 == 0, "two Z values per thread");
+    for (int e = threadIdx.x; e < (TG / 2) * N; e += blockDim.x) {
+        const int zi = e / N, Y = e % N;
+        const int Za = Z0 + zi, Zb = Za + TG / 2;
+        const double* wa = wz + zi * N;
+        const double* wb = wz + (zi + TG / 2) * N;  // zero rows past N
+        double2 axy = make_double2(0.0, 0.0), axx = axy, bxy = axy, bxx = axy;
         for (int w = 0; w < N; ++w) {
-            const double wv = wr[w];
-            if (wv == 0.0) continue;
-            const double2 t = cscale(cmul(u[w], Gs[w * N + Y]), wv);
-            axy = cadd(axy, t);
-            if (w != Z) axx = cadd(axx, t);
+            const double va = wa[w], vb = wb[w];
+            if (va == 0.0 && vb == 0.0) continue;
+            const double2 p = cmul(u[w], Gs[w * N + Y]);
+            if (va != 0.0) {
+                const double2 t = cscale(p, va);
+                axy = cadd(axy, t);
+                if (w != Za) axx = cadd(axx, t);
+            }
+            if (vb != 0.0) {
+                const double2 t = cscale(p, vb);
+                bxy = cadd(bxy, t);
+                if (w != Zb) bxx = cadd(bxx, t);
+            }
         }
-        const int64_t qp = ((int64_t)X * N + Y) * N + Z;
-        d.Qp[0][qp] = axx;
-        d.Qp[1][qp] = axy;
+        if (Za < N) {
+            const int64_t qp = ((int64_t)X * N + Y) * N + Za;
+            d.Qp[0][qp] = axx;
+            d.Qp[1][qp] = axy;
+        }
+        if (Zb < N) {
+            const int64_t qp = ((int64_t)X * N + Y) * N + Zb;
+            d.Qp[0][qp] = bxx;
+            d.Qp[1][qp] = bxy;
+        }
     }
 }
 
@@ -373,26 +393,40 @@ __device__ inline void f_tables_block(const DevState& d, const StageParams& sp, 
         dxy[e] = b;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < TG * N; e += blockDim.x) {
-        const int yi = e / N, Z = e % N, Y = Y0 + yi;
-        if (Y >= N) continue;
-        const double2* rxx = dxx + yi * N;
-        const double2* rxy = dxy + yi * N;
-        double2 fxx = make_double2(0.0, 0.0), fxy = make_double2(0.0, 0.0);
+    // a thread owns column Z for the Y values yi and yi + TG/2 of the block:
+    // the W entries are read once per w for both, and each output sums its
+    // terms in the same order as one output alone would
+    for (int e = threadIdx.x; e < (TG / 2) * N; e += blockDim.x) {
+        const int yi = e / N, Z = e % N;
+        const int Ya = Y0 + yi, Yb = Ya + TG / 2;
+        const double2* axxr = dxx + yi * N;
+        const double2* axyr = dxy + yi * N;
+        const double2* bxxr = dxx + (yi + TG / 2) * N;  // zero rows past N
+        const double2* bxyr = dxy + (yi + TG / 2) * N;
+        double2 fxx = make_double2(0.0, 0.0), fxy = fxx, gxx = fxx, gxy = fxx;
         for (int w = 0; w < N; ++w) {
             const double a = Ws[w * N + Z];  // W_xy is symmetric
             if (a == 0.0) continue;
             const double b = (w == Z) ? 0.0 : a;
-            const double2 vxx = rxx[w], vxy = rxy[w];
+            const double2 vxx = axxr[w], vxy = axyr[w];
             fxx = cadd(fxx, cadd(cscale(vxy, a), cscale(vxx, b)));
             fxy = cadd(fxy, cadd(cscale(vxx, a), cscale(vxy, b)));
+            const double2 uxx = bxxr[w], uxy = bxyr[w];
+            gxx = cadd(gxx, cadd(cscale(uxy, a), cscale(uxx, b)));
+            gxy = cadd(gxy, cadd(cscale(uxx, a), cscale(uxy, b)));
         }
-        fxx = cmul_mi(fxx);
-        fxy = cmul_mi(fxy);
-        const int64_t f = ((int64_t)X * N + Y) * N + Z;
-        d.F[0][f] = fxx;
-        d.F[1][f] = fxy;
-        if (Z == X) d.I[(int64_t)X * N + Y] = fxx;
+        if (Ya < N) {
+            const int64_t f = ((int64_t)X * N + Ya) * N + Z;
+            d.F[0][f] = cmul_mi(fxx);
+            d.F[1][f] = cmul_mi(fxy);
+            if (Z == X) d.I[(int64_t)X * N + Ya] = cmul_mi(fxx);
+        }
+        if (Yb < N) {
+            const int64_t f = ((int64_t)X * N + Yb) * N + Z;
+            d.F[0][f] = cmul_mi(gxx);
+            d.F[1][f] = cmul_mi(gxy);
+            if (Z == X) d.I[(int64_t)X * N + Yb] = cmul_mi(gxx);
+        }
     }
 }
 
