@@ -697,19 +697,23 @@ int set_state(Run& r, int i, const double* G1, const double* G2xx, const double*
     HIPCHK(hipMemcpyAsync(d.S1, G1, sizeof(double2) * d.n2, hipMemcpyHostToDevice, r.stream));
     const double* g2[2] = {G2xx, G2xy};
     double2* tmp = nullptr;
-    if (G2xx || G2xy) HIPCHK(hipMalloc(&tmp, sizeof(double2) * d.n4));
-    for (int s = 0; s < 2; ++s) {
+    if ((G2xx || G2xy) && hipMalloc(&tmp, sizeof(double2) * d.n4) != hipSuccess) {
+        (void)hipStreamSynchronize(r.stream);
+        return G12_E_NOMEM;
+    }
+    hipError_t e = hipSuccess;  // first failure; the stream is drained before tmp is freed
+    for (int s = 0; s < 2 && e == hipSuccess; ++s) {
         if (!g2[s]) {
-            hipMemsetAsync(d.S2[s], 0, sizeof(double2) * d.n4, r.stream);
+            e = hipMemsetAsync(d.S2[s], 0, sizeof(double2) * d.n4, r.stream);
             continue;
         }
-        hipMemcpyAsync(tmp, g2[s], sizeof(double2) * d.n4, hipMemcpyHostToDevice, r.stream);
-        g12::launch_ref_to_nat(tmp, d.S2[s], d.Ns, d.Nb, r.stream);
+        e = hipMemcpyAsync(tmp, g2[s], sizeof(double2) * d.n4, hipMemcpyHostToDevice, r.stream);
+        if (e == hipSuccess) g12::launch_ref_to_nat(tmp, d.S2[s], d.Ns, d.Nb, r.stream);
     }
-    hipMemsetAsync(d.fail, 0, sizeof(int), r.stream);
-    const hipError_t e = hipStreamSynchronize(r.stream);
-    if (tmp) hipFree(tmp);
-    if (e != hipSuccess) return G12_E_HIP;
+    if (e == hipSuccess) e = hipMemsetAsync(d.fail, 0, sizeof(int), r.stream);
+    const hipError_t es = hipStreamSynchronize(r.stream);
+    if (tmp) (void)hipFree(tmp);
+    if (e != hipSuccess || es != hipSuccess) return G12_E_HIP;
     r.t = 0.0;
     r.steps = 0;
     return 0;
@@ -721,15 +725,19 @@ int get_state(Run& r, int i, double* G1, double* G2xx, double* G2xy) {
     if (G1) HIPCHK(hipMemcpyAsync(G1, d.S1, sizeof(double2) * d.n2, hipMemcpyDeviceToHost, r.stream));
     double* g2[2] = {G2xx, G2xy};
     double2* tmp = nullptr;
-    if (G2xx || G2xy) HIPCHK(hipMalloc(&tmp, sizeof(double2) * d.n4));
-    for (int s = 0; s < 2; ++s) {
+    if ((G2xx || G2xy) && hipMalloc(&tmp, sizeof(double2) * d.n4) != hipSuccess) {
+        (void)hipStreamSynchronize(r.stream);
+        return G12_E_NOMEM;
+    }
+    hipError_t e = hipSuccess;  // first failure; the stream is drained before tmp is freed
+    for (int s = 0; s < 2 && e == hipSuccess; ++s) {
         if (!g2[s]) continue;
         g12::launch_nat_to_ref(d.S2[s], tmp, d.Ns, d.Nb, r.stream);
-        hipMemcpyAsync(g2[s], tmp, sizeof(double2) * d.n4, hipMemcpyDeviceToHost, r.stream);
+        e = hipMemcpyAsync(g2[s], tmp, sizeof(double2) * d.n4, hipMemcpyDeviceToHost, r.stream);
     }
-    const hipError_t e = hipStreamSynchronize(r.stream);
-    if (tmp) hipFree(tmp);
-    return e == hipSuccess ? 0 : G12_E_HIP;
+    const hipError_t es = hipStreamSynchronize(r.stream);
+    if (tmp) (void)hipFree(tmp);
+    return (e == hipSuccess && es == hipSuccess) ? 0 : G12_E_HIP;
 }
 
 int timing_enable(Run& r, int on) {
@@ -1068,15 +1076,16 @@ int state_energy(Run& r, int i, double* E) {
     const bool corr = r.tp[i].corr != 0;
     std::vector<std::complex<double>> g1(d.n2), pd(corr ? 2 * d.n2 : 0);
     double2* tmp = nullptr;
+    if (corr && hipMalloc(&tmp, sizeof(double2) * 2 * d.n2) != hipSuccess) return G12_E_NOMEM;
+    hipError_t e = hipSuccess;  // first failure; the stream is drained before tmp and the host buffers go
     if (corr) {
-        HIPCHK(hipMalloc(&tmp, sizeof(double2) * 2 * d.n2));
         for (int s = 0; s < 2; ++s) g12::launch_pair_diag(d.S2[s], tmp + s * d.n2, d.N, r.stream);
-        HIPCHK(hipMemcpyAsync(pd.data(), tmp, sizeof(double2) * 2 * d.n2, hipMemcpyDeviceToHost, r.stream));
+        e = hipMemcpyAsync(pd.data(), tmp, sizeof(double2) * 2 * d.n2, hipMemcpyDeviceToHost, r.stream);
     }
-    HIPCHK(hipMemcpyAsync(g1.data(), d.S1, sizeof(double2) * d.n2, hipMemcpyDeviceToHost, r.stream));
-    const hipError_t e = hipStreamSynchronize(r.stream);
-    if (tmp) hipFree(tmp);
-    if (e != hipSuccess) return G12_E_HIP;
+    if (e == hipSuccess) e = hipMemcpyAsync(g1.data(), d.S1, sizeof(double2) * d.n2, hipMemcpyDeviceToHost, r.stream);
+    const hipError_t es = hipStreamSynchronize(r.stream);
+    if (tmp) (void)hipFree(tmp);
+    if (e != hipSuccess || es != hipSuccess) return G12_E_HIP;
     int gate = 0;
     const double w = quench_phi(p, r.t, &gate) * p.quench_strength;
     energy_parts(p, 1.0, w, reinterpret_cast<const double*>(g1.data()),
@@ -1162,7 +1171,8 @@ int g12_h2_commutator(const double* G2, const double* h1x, const double* h1y, do
         (rc = up(dhy, h1y, sizeof(double2) * n2)))
         return rc;
     g12::launch_ref_to_nat(draw, dT, Ns, Nb, api_stream());
-    g12::launch_comm2(dT, dhx, dhy, dC, N, precision, api_stream());
+    if ((rc = g12::launch_comm2(dT, dhx, dhy, dC, N, precision, api_stream())))
+        return rc;
     g12::launch_nat_to_ref(dC, draw, Ns, Nb, api_stream());
     return down(comm, draw, sizeof(double2) * n4);
 }

@@ -129,9 +129,10 @@ int tile_wave_options(int NP, int* opts);  // compiled wave counts for NP; retur
 // routine-level helpers (kernels.hip)
 void launch_ref_to_nat(const double2* in, double2* out, int Ns, int Nb, hipStream_t s);
 void launch_nat_to_ref(const double2* in, double2* out, int Ns, int Nb, hipStream_t s);
-// out = [hx (x) 1 + 1 (x) hx] T - T [hy (x) 1 + 1 (x) hy] (natural layout)
-void launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
-                  hipStream_t s);
+// out = [hx (x) 1 + 1 (x) hx] T - T [hy (x) 1 + 1 (x) hy] (natural layout);
+// synchronous: 0, G12_E_NOMEM or G12_E_HIP
+int launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
+                 hipStream_t s);
 void launch_hf(double2* h, const double2* G1, int Ns, int Nb, const StageParams& sp, hipStream_t s);
 void launch_comm1(const double2* G, const double2* h, double2* out, int N, hipStream_t s);
 void launch_d_gather(const double2* T, double2* D, int N, hipStream_t s);

@@ -1836,8 +1836,8 @@ void launch_nat_to_ref(const double2* in, double2* out, int Ns, int Nb, hipStrea
     hipLaunchKernelGGL(k_nat_to_ref, dim3(N * N), dim3(256), 0, s, in, out, Ns, Nb);
 }
 
-void launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
-                  hipStream_t s) {
+int launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
+                 hipStream_t s) {
     // a batch of one trajectory whose spin-0 state is T and strided-pass
     // output is `out`; four accumulating launches (strided, contiguous) x
     // (left hx, right hy)
@@ -1852,15 +1852,27 @@ void launch_comm2(const double2* T, const double2* hx, const double2* hy, double
     TrajParams* dt = nullptr;
     int* dl = nullptr;
     int64_t* dp = nullptr;
+    auto release = [&] {
+        if (dd) (void)hipFree(dd);
+        if (dt) (void)hipFree(dt);
+        if (dl) (void)hipFree(dl);
+        if (dp) (void)hipFree(dp);
+    };
     if (hipMalloc(&dd, sizeof d) != hipSuccess || hipMalloc(&dt, sizeof tp) != hipSuccess ||
-        hipMalloc(&dl, sizeof(int)) != hipSuccess || hipMalloc(&dp, 2 * sizeof(int64_t)) != hipSuccess)
-        return;
+        hipMalloc(&dl, sizeof(int)) != hipSuccess || hipMalloc(&dp, 2 * sizeof(int64_t)) != hipSuccess) {
+        release();
+        return G12_E_NOMEM;
+    }
     const int zero = 0;
     const int64_t pre[2] = {0, (int64_t)N * N};
-    hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(dt, &tp, sizeof tp, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(dl, &zero, sizeof zero, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(dp, pre, sizeof pre, hipMemcpyHostToDevice, s);
+    if (hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dt, &tp, sizeof tp, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dl, &zero, sizeof zero, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dp, pre, sizeof pre, hipMemcpyHostToDevice, s) != hipSuccess) {
+        (void)hipStreamSynchronize(s);
+        release();
+        return G12_E_HIP;
+    }
     BTileArgs a{};
     a.b.ds = dd;
     a.b.tp = dt;
@@ -1888,11 +1900,10 @@ void launch_comm2(const double2* T, const double2* hx, const double2* hy, double
             first = 0;
             launch_tile(a, precision, s);
         }
-    hipStreamSynchronize(s);
-    hipFree(dd);
-    hipFree(dt);
-    hipFree(dl);
-    hipFree(dp);
+    const hipError_t launched = hipGetLastError();
+    const hipError_t e = hipStreamSynchronize(s);  // the tiles read dd..dp until here
+    release();
+    return (launched == hipSuccess && e == hipSuccess) ? 0 : G12_E_HIP;
 }
 
 void launch_hf(double2* h, const double2* G1, int Ns, int Nb, const StageParams& sp, hipStream_t s) {
