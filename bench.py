@@ -258,6 +258,39 @@ def mixed_error_run(g, rk_steps):
     return out
 
 
+def observables_leg(g, G1, Gxx, Gxy, reps=5):
+    """Cost of the observables and full-precision I/O at the N = 64 workload:
+    the device energy entry (G1 and the N^2 pair diagonals leave the device)
+    and a binary checkpoint save and load of (t, G1, G2xx, G2xy)."""
+    import shutil
+    import tempfile
+    p = workload_params(g, "fp64")
+    out = {}
+    with g.Propagator(p) as prop:
+        prop.set_state(G1, Gxx, Gxy)
+        prop.advance(2)
+        prop.energy()  # first call: host buffers
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            prop.energy()
+        out["energy_ms"] = round((time.perf_counter() - t0) * 1e3 / reps, 3)
+        d = tempfile.mkdtemp()
+        try:
+            path = os.path.join(d, "state.bin")
+            t0 = time.perf_counter()
+            prop.save(path)
+            t1 = time.perf_counter()
+            prop.load(path)
+            t2 = time.perf_counter()
+            size = os.path.getsize(path)
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    out.update({"checkpoint_bytes": size, "save_seconds": round(t1 - t0, 3), "load_seconds": round(t2 - t1, 3),
+                "save_GB_per_s": round(size / (t1 - t0) / 1e9, 2), "load_GB_per_s": round(size / (t2 - t1) / 1e9, 2),
+                "note": "host file system of the run; includes the device layout permutation and PCIe copies"})
+    return out
+
+
 def roofline(N, precision, k, steps):
     # dominant kernel: the tile pass with the larger summed time; one launch of
     # either pass applies two of the four one-particle terms of [H2,G2] to both
@@ -291,6 +324,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-trajectory ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=3)
     ap.add_argument("--no-extrapolator", action="store_true", help="skip the learned-extrapolator leg")
+    ap.add_argument("--no-observables", action="store_true", help="skip the energy / checkpoint timing leg")
     ap.add_argument("--mixed-steps", type=int, default=200,
                     help="production steps of the mixed-precision error run (5000: the full workload)")
     a = ap.parse_args()
@@ -327,6 +361,7 @@ def main():
         mixed["error_run"] = mixed_error_run(g, a.mixed_steps)
     ensemble = None if a.no_ensemble else ensemble_leg(g, a.ensemble_steps, 1)
     extrap = None if a.no_extrapolator else extrapolator_leg(g)
+    observables = None if a.no_observables else observables_leg(g, G1, Gxx, Gxy)
     cpu = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
@@ -342,7 +377,7 @@ def main():
                                   "dt=0.01", "N": N, "precision": a.precision,
                       "algorithmic_flop_per_step": 256.0 * N ** 5},
            "roofline": roofline(N, a.precision, k_head, a.steps), "mixed": mixed, "ensemble": ensemble, "extrapolator": extrap,
-           "cpu_baseline": cpu}
+           "observables": observables, "cpu_baseline": cpu}
     print(json.dumps(out))
 
 
