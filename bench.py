@@ -25,10 +25,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "G2 RK4 steps/s at N=64 (FP64 vs mixed @≤1e-6 G1 err); batched traj/s"
-# HBM bytes per launch of the dominant kernel (read + written), measured with
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload and calibrated with a
-# plain copy kernel (bytes read = 2 x FETCH_SIZE KiB, written = WRITE_SIZE KiB).
-TRAFFIC_BYTES = {("fp64", "tile_fused"): 3.01e9, ("fp64", "tile_strided"): 1.10e9}
+# HBM bytes per launch of each tile pass (read + written, averaged over the four
+# stages), measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
+# and calibrated with a plain copy kernel (bytes read = 2 x FETCH_SIZE KiB,
+# written = WRITE_SIZE KiB).
+TRAFFIC_BYTES = {("fp64", "tile_fused"): 3.01e9, ("fp64", "tile_strided"): 1.10e9,
+                 ("fp32_3m", "tile_fused"): 2.50e9, ("fp32_3m", "tile_strided"): 0.557e9,
+                 ("bf16x3", "tile_fused"): 2.34e9, ("bf16x3", "tile_strided"): 0.559e9}
 
 
 def workload_params(g, precision="fp64", time_steps=5000):
