@@ -86,7 +86,7 @@ struct DevState {
 // padded tile size.
 // Table launches: one block per (X, group of G12_TABLE_GROUP values of the
 // other index) of each correlated trajectory.
-#define G12_TABLE_GROUP 16
+#define G12_TABLE_GROUP 32
 
 struct BatchDev {
     const DevState* ds;      // device array [ntraj]

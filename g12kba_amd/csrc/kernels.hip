@@ -259,7 +259,7 @@ __device__ inline bool active(const DevState& d, const TrajParams& t, const Stag
 }
 
 // Stage set-up for G1: Y1 = S1 + c K1, the pairs (Y1[X][Y], conj Y1[Y][X])
-// read by the strided pass, h = h_HF[Y1], W tables.  Blocks of 256 elements
+// read by the strided pass, h = h_HF[Y1], W tables.  Blocks of blockDim elements
 // (`chunk`) per trajectory; each block forms the diagonal of Y1 itself (for
 // the Hartree sums, in `diag`) and takes the Y1 elements of its Fock terms
 // from S1 and K1, so the blocks need no ordering among themselves.
@@ -446,12 +446,16 @@ __device__ inline int table_block(const BatchDev& b, int64_t blk, int& e) {
 // Stage set-up of every trajectory in one launch: the G1 part (Y1 = S1 + c K1,
 // the G1 pairs, h = h_HF[Y1], W tables) and the two table families, which
 // form Y1 and W themselves, so that none of the three waits for another.
-// Blocks [0, gA * ntraj): G1 part, gA blocks of 256 elements per trajectory;
+// Blocks [0, gA * ntraj): G1 part, gA blocks of SETUP_THREADS elements per trajectory;
 // then tblocks Qp blocks and tblocks F blocks.
-constexpr int SETUP_LDS = TMAX * TMAX * (int)sizeof(double2) + TG * TMAX * (int)sizeof(double);
-static_assert(TMAX * TMAX * sizeof(double) + 2 * TG * TMAX * sizeof(double2) <= SETUP_LDS, "F part LDS");
+// 1024 threads per block: one workgroup fills a CU's 16 waves, and each thread
+// owns one (Qp: column, pair of Z; F: column, pair of Y) entry at TG = 32, N = 64.
+constexpr int SETUP_THREADS = 1024;
+constexpr int Q_LDS = TMAX * TMAX * (int)sizeof(double2) + TG * TMAX * (int)sizeof(double);
+constexpr int F_LDS = TMAX * TMAX * (int)sizeof(double) + 2 * TG * TMAX * (int)sizeof(double2);
+constexpr int SETUP_LDS = Q_LDS > F_LDS ? Q_LDS : F_LDS;
 
-__global__ __launch_bounds__(256) void k_stage_setup(BatchDev b, StageRef r, int herm, int gA) {
+__global__ __launch_bounds__(SETUP_THREADS) void k_stage_setup(BatchDev b, StageRef r, int herm, int gA) {
     __shared__ StageParams sp;
     __shared__ __align__(16) char lds[SETUP_LDS];
     const StageShared ss = stage_of(r);
@@ -1766,9 +1770,9 @@ inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b
 // launchers
 
 void launch_b_stage_setup(const BatchDev& b, const StageRef& r, int precision, hipStream_t s) {
-    const int gA = (b.maxN * b.maxN + 255) / 256;
+    const int gA = (b.maxN * b.maxN + SETUP_THREADS - 1) / SETUP_THREADS;
     const int64_t blocks = (int64_t)gA * b.ntraj + 2 * b.tblocks;
-    hipLaunchKernelGGL(k_stage_setup, dim3((unsigned)blocks), dim3(256), 0, s, b, r,
+    hipLaunchKernelGGL(k_stage_setup, dim3((unsigned)blocks), dim3(SETUP_THREADS), 0, s, b, r,
                        precision != G12_PREC_FP64 ? 1 : 0, gA);
 }
 
