@@ -85,8 +85,8 @@ struct DevState {
 // constants, and (for the tile passes) buckets of trajectories with the same
 // padded tile size.
 // Table launches: one block per (X, group of G12_TABLE_GROUP values of the
-// other index) of each correlated trajectory.
-#define G12_TABLE_GROUP 32
+// other index) of each correlated trajectory; 64 (N <= 64): one block per X.
+#define G12_TABLE_GROUP 64
 
 struct BatchDev {
     const DevState* ds;      // device array [ntraj]

@@ -293,204 +293,6 @@ __device__ inline void stage_g1_block(const DevState& d, const StageShared& ss, 
     }
 }
 
-// ---------------------------------------------------------------------------
-// Stage tables.  A block handles TG values of one index for one X, so that
-// the operand every output needs (the stage G1 for Qp, W_xy for F) is read
-// from L2 once per block into LDS instead of once per output row.  N <= 64.
-constexpr int TG = G12_TABLE_GROUP;
-constexpr int TMAX = 64;
-
-__device__ inline int table_groups(int N) { return (N + TG - 1) / TG; }
-
-// Qp^s[X][Y][Z] = Q^s[Z][X][Y] = sum_w Y1[X][w] W^s[Z][w] Y1[w][Y] for
-// s = xx (index 0, W_xx) and xy (index 1, W_xy), Z in [Z0, Z0 + TG).  The
-// stage G1 Y1 = S1 + c K1 and W are formed here as the G1 part forms them.
-// Gs: TMAX x TMAX complex (Y1), wz: TG x TMAX (the W rows of the block's Z).
-__device__ inline void q_tables_block(const DevState& d, const StageShared& ss, const StageParams& sp, int X, int Z0,
-                                      double2* Gs, double* wz) {
-    const int N = d.N;
-    const double2* S1 = uni_ptr(d.S1);
-    const double2* K1 = uni_ptr(d.K1);
-    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
-        double2 y = gload(S1, e);
-        if (!ss.first) y = cadd(y, cscale(gload(K1, e), ss.c));
-        Gs[e] = y;
-    }
-    for (int e = threadIdx.x; e < TG * N; e += blockDim.x) {
-        const int Z = Z0 + e / N, w = e % N;
-        wz[e] = Z < N ? w_xy(w, Z, d.Ns, sp) : 0.0;
-    }
-    __syncthreads();
-    const double2* u = Gs + X * N;  // Y1[X][w]
-    // a thread owns column Y for the Z values zi and zi + TG/2 of the block:
-    // the product Y1[X][w] Y1[w][Y] is formed once per w for both, and each
-    // output sums its terms in the same order as one output alone would
-    static_assert(TG % 2 == 0, "two Z values per thread");
-    for (int e = threadIdx.x; e < (TG / 2) * N; e += blockDim.x) {
-        const int zi = e / N, Y = e % N;
-        const int Za = Z0 + zi, Zb = Za + TG / 2;
-        const double* wa = wz + zi * N;
-        const double* wb = wz + (zi + TG / 2) * N;  // zero rows past N
-        double2 axy = make_double2(0.0, 0.0), axx = axy, bxy = axy, bxx = axy;
-        for (int w = 0; w < N; ++w) {
-            const double va = wa[w], vb = wb[w];
-            if (va == 0.0 && vb == 0.0) continue;
-            const double2 p = cmul(u[w], Gs[w * N + Y]);
-            if (va != 0.0) {
-                const double2 t = cscale(p, va);
-                axy = cadd(axy, t);
-                if (w != Za) axx = cadd(axx, t);
-            }
-            if (vb != 0.0) {
-                const double2 t = cscale(p, vb);
-                bxy = cadd(bxy, t);
-                if (w != Zb) bxx = cadd(bxx, t);
-            }
-        }
-        if (Za < N) {
-            const int64_t qp = ((int64_t)X * N + Y) * N + Za;
-            d.Qp[0][qp] = axx;
-            d.Qp[1][qp] = axy;
-        }
-        if (Zb < N) {
-            const int64_t qp = ((int64_t)X * N + Y) * N + Zb;
-            d.Qp[0][qp] = bxx;
-            d.Qp[1][qp] = bxy;
-        }
-    }
-}
-
-// F^xx[X][Y][Z] = -i sum_w ( W_xy[Z][w] D^xy[X][Y][w] + W_xx[Z][w] D^xx[X][Y][w] )
-// F^xy[X][Y][Z] = -i sum_w ( W_xy[Z][w] D^xx[X][Y][w] + W_xx[Z][w] D^xy[X][Y][w] )
-// (Pi_xyxy with the swapped G2 pair, pi.h:25-95 / RK_solver.h:163-164) and
-// I[X][Y] = F^xx[X][Y][X] (collision_int, collision_integral.h:27-69), for Y
-// in [Y0, Y0 + TG), with W_xy formed from the stage parameters.  Ws: TMAX x
-// TMAX doubles, dxx and dxy: TG x TMAX complex (the D rows of the block's Y).
-// With `herm` (mixed-precision runs) D enters through its Hermitian part
-// (D[X][Y][Z] + conj D[Y][X][Z]) / 2.  A Hermitian G2 (the physical states:
-// G2 evolved from zero stays Hermitian) has D[X][Y][Z] = conj D[Y][X][Z], and
-// that identity makes Re tr I, and with it the change of tr G1, vanish; FP32
-// or BF16 tile products break it slightly, the average restores particle-number
-// conservation to FP64 level.  FP64 runs use D as it is, as the reference does,
-// for any input.
-__device__ inline void f_tables_block(const DevState& d, const StageParams& sp, int X, int Y0, bool herm, double* Ws,
-                                      double2* dxx, double2* dxy) {
-    const int N = d.N;
-    for (int e = threadIdx.x; e < N * N; e += blockDim.x) Ws[e] = w_xy(e / N, e % N, d.Ns, sp);
-    for (int e = threadIdx.x; e < TG * N; e += blockDim.x) {
-        const int Y = Y0 + e / N, w = e % N;
-        double2 a = make_double2(0.0, 0.0), b = a;
-        if (Y < N) {
-            const int64_t xy = ((int64_t)X * N + Y) * N + w, yx = ((int64_t)Y * N + X) * N + w;
-            a = d.D[0][xy];
-            b = d.D[1][xy];
-            if (herm) {
-                a = cscale(cadd(a, cconj(d.D[0][yx])), 0.5);
-                b = cscale(cadd(b, cconj(d.D[1][yx])), 0.5);
-            }
-        }
-        dxx[e] = a;
-        dxy[e] = b;
-    }
-    __syncthreads();
-    // a thread owns column Z for the Y values yi and yi + TG/2 of the block:
-    // the W entries are read once per w for both, and each output sums its
-    // terms in the same order as one output alone would
-    for (int e = threadIdx.x; e < (TG / 2) * N; e += blockDim.x) {
-        const int yi = e / N, Z = e % N;
-        const int Ya = Y0 + yi, Yb = Ya + TG / 2;
-        const double2* axxr = dxx + yi * N;
-        const double2* axyr = dxy + yi * N;
-        const double2* bxxr = dxx + (yi + TG / 2) * N;  // zero rows past N
-        const double2* bxyr = dxy + (yi + TG / 2) * N;
-        double2 fxx = make_double2(0.0, 0.0), fxy = fxx, gxx = fxx, gxy = fxx;
-        for (int w = 0; w < N; ++w) {
-            const double a = Ws[w * N + Z];  // W_xy is symmetric
-            if (a == 0.0) continue;
-            const double b = (w == Z) ? 0.0 : a;
-            const double2 vxx = axxr[w], vxy = axyr[w];
-            fxx = cadd(fxx, cadd(cscale(vxy, a), cscale(vxx, b)));
-            fxy = cadd(fxy, cadd(cscale(vxx, a), cscale(vxy, b)));
-            const double2 uxx = bxxr[w], uxy = bxyr[w];
-            gxx = cadd(gxx, cadd(cscale(uxy, a), cscale(uxx, b)));
-            gxy = cadd(gxy, cadd(cscale(uxx, a), cscale(uxy, b)));
-        }
-        if (Ya < N) {
-            const int64_t f = ((int64_t)X * N + Ya) * N + Z;
-            d.F[0][f] = cmul_mi(fxx);
-            d.F[1][f] = cmul_mi(fxy);
-            if (Z == X) d.I[(int64_t)X * N + Ya] = cmul_mi(fxx);
-        }
-        if (Yb < N) {
-            const int64_t f = ((int64_t)X * N + Yb) * N + Z;
-            d.F[0][f] = cmul_mi(gxx);
-            d.F[1][f] = cmul_mi(gxy);
-            if (Z == X) d.I[(int64_t)X * N + Yb] = cmul_mi(gxx);
-        }
-    }
-}
-
-// Table launches enumerate blocks over the correlated trajectories only:
-// block blk is entry (blk - tprefix[j]) of the N x ceil(N / TG) block space
-// of listed trajectory j.
-__device__ inline int table_block(const BatchDev& b, int64_t blk, int& e) {
-    int lo = 0, hi = b.ncorr - 1;
-    while (lo < hi) {  // last correlated trajectory whose blocks start at or before blk
-        const int mid = (lo + hi + 1) >> 1;
-        if (b.tprefix[mid] <= blk) lo = mid; else hi = mid - 1;
-    }
-    e = (int)(blk - b.tprefix[lo]);
-    return b.tlist[lo];
-}
-
-// Stage set-up of every trajectory in one launch: the G1 part (Y1 = S1 + c K1,
-// the G1 pairs, h = h_HF[Y1], W tables) and the two table families, which
-// form Y1 and W themselves, so that none of the three waits for another.
-// Blocks [0, gA * ntraj): G1 part, gA blocks of SETUP_THREADS elements per trajectory;
-// then tblocks Qp blocks and tblocks F blocks.
-// 1024 threads per block: one workgroup fills a CU's 16 waves, and each thread
-// owns one (Qp: column, pair of Z; F: column, pair of Y) entry at TG = 32, N = 64.
-constexpr int SETUP_THREADS = 1024;
-constexpr int Q_LDS = TMAX * TMAX * (int)sizeof(double2) + TG * TMAX * (int)sizeof(double);
-constexpr int F_LDS = TMAX * TMAX * (int)sizeof(double) + 2 * TG * TMAX * (int)sizeof(double2);
-constexpr int SETUP_LDS = Q_LDS > F_LDS ? Q_LDS : F_LDS;
-
-__global__ __launch_bounds__(SETUP_THREADS) void k_stage_setup(BatchDev b, StageRef r, int herm, int gA) {
-    __shared__ StageParams sp;
-    __shared__ __align__(16) char lds[SETUP_LDS];
-    const StageShared ss = stage_of(r);
-    const int64_t nA = (int64_t)gA * b.ntraj, blk = blockIdx.x;
-    int ti, e, role;
-    if (blk < nA) {
-        role = 0;
-        ti = (int)(blk / gA);
-        e = (int)(blk % gA);
-    } else {
-        role = blk < nA + b.tblocks ? 1 : 2;
-        ti = table_block(b, role == 1 ? blk - nA : blk - nA - b.tblocks, e);
-    }
-    const DevState& d = b.ds[ti];
-    const TrajParams& tp = b.tp[ti];
-    if (!active(d, tp, ss)) return;
-    if (role == 0 && e * (int)blockDim.x >= d.N * d.N) return;
-    // built per block in LDS: the V table is indexed at run time and would
-    // otherwise live in per-lane scratch
-    fill_sp(sp, tp, ss);
-    __syncthreads();
-    if (role == 0) {
-        stage_g1_block(d, ss, sp, e, reinterpret_cast<double2*>(lds));
-    } else {
-        const int ng = table_groups(d.N);
-        if (role == 1)
-            q_tables_block(d, ss, sp, e / ng, (e % ng) * TG, reinterpret_cast<double2*>(lds),
-                           reinterpret_cast<double*>(lds + TMAX * TMAX * sizeof(double2)));
-        else
-            f_tables_block(d, sp, e / ng, (e % ng) * TG, herm != 0, reinterpret_cast<double*>(lds),
-                           reinterpret_cast<double2*>(lds + TMAX * TMAX * sizeof(double)),
-                           reinterpret_cast<double2*>(lds + TMAX * TMAX * sizeof(double) + TG * TMAX * sizeof(double2)));
-    }
-}
-
 // G1 right-hand side: comm = [h, Y1], K = -i(comm + I + I^dag), RK
 // accumulation (G1_step, G1_G2_t_step.h:24-47) and the final update.  One
 // block per row X: the rows X of h and Y1 are staged in LDS, and four groups
@@ -682,6 +484,174 @@ __device__ __forceinline__ void tile_mfma_fp64(const TileLds<NP>& L, int hmode, 
         ar[u] = p1[u] - p2[u];
         ai[u] = p3[u] - p1[u] - p2[u];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Stage tables on FP64 MFMA.  One block per (trajectory, X) and table family
+// forms a 64 x 64 x 64 complex product with the tile product of the
+// commutator passes (operands zero-padded to 64, N <= 64):
+//   Qp: O[Z][Y] = sum_w A[Z][w] Y1[w][Y],  A[Z][w] = Y1[X][w] W_xy[Z][w];
+//       Qp^xy[X][Y][Z] = O[Z][Y],  Qp^xx[X][Y][Z] = O[Z][Y] - A[Z][Z] Y1[Z][Y]
+//       (W_xx equals W_xy off the diagonal and is 0 on it);
+//   F:  S[Y][Z] = sum_w (D^xx + D^xy)[X][Y][w] W_xy[w][Z],
+//       F^xx[X][Y][Z] = -i (S[Y][Z] - W_xy[Z][Z] D^xx[X][Y][Z]),
+//       F^xy[X][Y][Z] = -i (S[Y][Z] - W_xy[Z][Z] D^xy[X][Y][Z]),
+//       I[X][Y] = F^xx[X][Y][X]
+// with Qp^s[X][Y][Z] = Q^s[Z][X][Y] = sum_w Y1[X][w] W^s[Z][w] Y1[w][Y]
+// (psi.h:26-277), F as in Pi_xyxy with the swapped G2 pair (pi.h:25-95,
+// RK_solver.h:163-164) and I as in collision_int (collision_integral.h:27-69).
+// The stage G1 Y1 = S1 + c K1 and W are formed here as the G1 part forms
+// them.  With `herm` (mixed-precision runs) D enters through its Hermitian
+// part (D[X][Y][Z] + conj D[Y][X][Z]) / 2.  A Hermitian G2 (the physical
+// states: G2 evolved from zero stays Hermitian) has D[X][Y][Z] =
+// conj D[Y][X][Z], and that identity makes Re tr I, and with it the change of
+// tr G1, vanish; FP32 or BF16 tile products break it slightly, the average
+// restores particle-number conservation to FP64 level.  FP64 runs use D as it
+// is, as the reference does, for any input.
+constexpr int TG = G12_TABLE_GROUP;
+static_assert(TG == 64, "one table block per X (N <= 64)");
+constexpr int SETUP_THREADS = 1024;
+typedef TileCfg<64, 16> SetupCfg;
+static_assert(SetupCfg::NTHR == SETUP_THREADS && SetupCfg::PW == 1, "one 16x16 output block per wave");
+
+// (re, im) of this wave's 16x16 block of A.B, A in (hr, hi), B in (xr, xi);
+// register j of lane l holds row 4j + l/16, column l%16 of the block
+__device__ inline void setup_product(const TileLds<64>& L, d4& re, d4& im, int wave, int lane) {
+    d4 p1[1] = {d4{0.0, 0.0, 0.0, 0.0}}, p2[1] = {p1[0]}, p3[1] = {p1[0]};
+    tile_product_3m<64, 16, false>(L.hr, L.hi, L.xr, L.xi, p1, p2, p3, wave, lane);
+    re = p1[0] - p2[0];
+    im = p3[0] - p1[0] - p2[0];
+}
+
+__device__ inline void q_tables_block(const DevState& d, const StageShared& ss, const StageParams& sp, int X,
+                                      TileLds<64>& L) {
+    constexpr int P = TileLds<64>::P;
+    const int N = d.N;
+    const double2* S1 = uni_ptr(d.S1);
+    const double2* K1 = uni_ptr(d.K1);
+    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {  // B = Y1
+        const int m = e >> 6, n = e & 63;
+        double2 y = make_double2(0.0, 0.0);
+        if (m < N && n < N) {
+            y = gload(S1, (int64_t)m * N + n);
+            if (!ss.first) y = cadd(y, cscale(gload(K1, (int64_t)m * N + n), ss.c));
+        }
+        L.xr[m * P + n] = y.x;
+        L.xi[m * P + n] = y.y;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {  // A[Z][w] = Y1[X][w] W_xy[Z][w]
+        const int Z = e >> 6, w = e & 63;
+        double2 a = make_double2(0.0, 0.0);
+        if (Z < N && w < N) a = cscale(make_double2(L.xr[X * P + w], L.xi[X * P + w]), w_xy(w, Z, d.Ns, sp));
+        L.hr[Z * P + w] = a.x;
+        L.hi[Z * P + w] = a.y;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    d4 re, im;
+    setup_product(L, re, im, wave, lane);
+    const int Y = (wave & 3) * 16 + (lane & 15);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int Z = (wave >> 2) * 16 + 4 * j + (lane >> 4);
+        if (Z >= N || Y >= N) continue;
+        const double2 o = make_double2(re[j], im[j]);
+        const double2 wzz = cmul(make_double2(L.hr[Z * P + Z], L.hi[Z * P + Z]),
+                                 make_double2(L.xr[Z * P + Y], L.xi[Z * P + Y]));  // the w = Z term
+        const int64_t qp = ((int64_t)X * N + Y) * N + Z;
+        d.Qp[0][qp] = csub(o, wzz);
+        d.Qp[1][qp] = o;
+    }
+}
+
+__device__ inline void f_tables_block(const DevState& d, const StageParams& sp, int X, bool herm, TileLds<64>& L) {
+    constexpr int P = TileLds<64>::P;
+    const int N = d.N;
+    auto dval = [&](int s, int Y, int w) {  // D^s[X][Y][w], its Hermitian part with herm
+        double2 v = d.D[s][((int64_t)X * N + Y) * N + w];
+        if (herm) v = cscale(cadd(v, cconj(d.D[s][((int64_t)Y * N + X) * N + w])), 0.5);
+        return v;
+    };
+    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {  // A = D^xx + D^xy, B = W_xy (real)
+        const int m = e >> 6, n = e & 63;
+        double2 a = make_double2(0.0, 0.0);
+        double wv = 0.0;
+        if (m < N && n < N) {
+            a = cadd(dval(0, m, n), dval(1, m, n));
+            wv = w_xy(m, n, d.Ns, sp);
+        }
+        L.hr[m * P + n] = a.x;
+        L.hi[m * P + n] = a.y;
+        L.xr[m * P + n] = wv;
+        L.xi[m * P + n] = 0.0;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    d4 re, im;
+    setup_product(L, re, im, wave, lane);
+    const int Z = (wave & 3) * 16 + (lane & 15);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int Y = (wave >> 2) * 16 + 4 * j + (lane >> 4);
+        if (Y >= N || Z >= N) continue;
+        const double2 sv = make_double2(re[j], im[j]);
+        const double wzz = L.xr[Z * P + Z];  // the w = Z weight W_xx lacks
+        const double2 fxx = cmul_mi(csub(sv, cscale(dval(0, Y, Z), wzz)));
+        const double2 fxy = cmul_mi(csub(sv, cscale(dval(1, Y, Z), wzz)));
+        const int64_t f = ((int64_t)X * N + Y) * N + Z;
+        d.F[0][f] = fxx;
+        d.F[1][f] = fxy;
+        if (Z == X) d.I[(int64_t)X * N + Y] = fxx;
+    }
+}
+
+// Table launches enumerate blocks over the correlated trajectories only:
+// block blk is entry (blk - tprefix[j]) of the N x ceil(N / TG) block space
+// of listed trajectory j.
+__device__ inline int table_block(const BatchDev& b, int64_t blk, int& e) {
+    int lo = 0, hi = b.ncorr - 1;
+    while (lo < hi) {  // last correlated trajectory whose blocks start at or before blk
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.tprefix[mid] <= blk) lo = mid; else hi = mid - 1;
+    }
+    e = (int)(blk - b.tprefix[lo]);
+    return b.tlist[lo];
+}
+
+// Stage set-up of every trajectory in one launch: the G1 part (Y1 = S1 + c K1,
+// the G1 pairs, h = h_HF[Y1], W tables) and the two table families, which
+// form Y1 and W themselves, so that none of the three waits for another.
+// Blocks [0, gA * ntraj): G1 part, gA blocks of SETUP_THREADS elements per
+// trajectory; then tblocks Qp blocks and tblocks F blocks (one per X).
+__global__ __launch_bounds__(SETUP_THREADS) void k_stage_setup(BatchDev b, StageRef r, int herm, int gA) {
+    __shared__ StageParams sp;
+    __shared__ __align__(16) TileLds<64> L;
+    const StageShared ss = stage_of(r);
+    const int64_t nA = (int64_t)gA * b.ntraj, blk = blockIdx.x;
+    int ti, e, role;
+    if (blk < nA) {
+        role = 0;
+        ti = (int)(blk / gA);
+        e = (int)(blk % gA);
+    } else {
+        role = blk < nA + b.tblocks ? 1 : 2;
+        ti = table_block(b, role == 1 ? blk - nA : blk - nA - b.tblocks, e);
+    }
+    const DevState& d = b.ds[ti];
+    const TrajParams& tp = b.tp[ti];
+    if (!active(d, tp, ss)) return;
+    if (role == 0 && e * (int)blockDim.x >= d.N * d.N) return;
+    // built per block in LDS: the V table is indexed at run time and would
+    // otherwise live in per-lane scratch
+    fill_sp(sp, tp, ss);
+    __syncthreads();
+    if (role == 0)
+        stage_g1_block(d, ss, sp, e, reinterpret_cast<double2*>(L.xr));
+    else if (role == 1)
+        q_tables_block(d, ss, sp, e, L);
+    else
+        f_tables_block(d, sp, e, herm != 0, L);
 }
 
 // ---- precision engines ------------------------------------------------------
