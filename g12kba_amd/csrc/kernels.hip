@@ -139,12 +139,13 @@ __device__ __forceinline__ void lds_sync() {
 // Interaction tables W_xy / W_xx (N x N):  U on one site, V_n at distance
 // n <= nV, 0 beyond; W_xx(X,X) = 0.  (The loops of HF.h, psi.h, pi.h and
 // collision_integral.h over r and n enumerate exactly these entries.)
-__device__ inline double w_xy(int x, int z, int Ns, const StageParams& sp) {
-    int d = x % Ns - z % Ns;
+__device__ inline double site_w(int sx, int sz, const StageParams& sp) {  // W_xy of sites sx, sz
+    int d = sx - sz;
     d = d < 0 ? -d : d;
     if (d == 0) return sp.U;
     return d <= sp.nV ? sp.V[d - 1] : 0.0;
 }
+__device__ inline double w_xy(int x, int z, int Ns, const StageParams& sp) { return site_w(x % Ns, z % Ns, sp); }
 
 // h = h_HF[G] for one element (X, Y) (h_HFx_quench, HF.h:57-104):
 //   diagonal: eps + quench + Hartree  -iU g_XX - 2i sum_{Z != X} W_xy(X,Z) g_ZZ
@@ -161,11 +162,13 @@ __device__ inline double2 hf_element(const double2* G, int X, int Y, int Ns, int
         const double2 gxx = G[(int64_t)X * N + X];
         double2 acc = cscale(gxx, sp.U);  // -i U g_XX
         double2 lr = make_double2(0.0, 0.0);
-        for (int Z = 0; Z < N; ++Z) {
-            if (Z == X) continue;
-            const double w = w_xy(X, Z, Ns, sp);
-            if (w != 0.0) lr = cadd(lr, cscale(G[(int64_t)Z * N + Z], w));
-        }
+        // Z = site + orb * Ns in ascending order, without a division per term
+        for (int oz = 0, Z = 0; oz < Nb; ++oz)
+            for (int sz = 0; sz < Ns; ++sz, ++Z) {
+                if (Z == X) continue;
+                const double w = site_w(sx, sz, sp);
+                if (w != 0.0) lr = cadd(lr, cscale(G[(int64_t)Z * N + Z], w));
+            }
         acc = cadd(acc, cscale(lr, 2.0));
         v = cadd(v, cmul_mi(acc));
     } else {
@@ -199,11 +202,12 @@ __device__ inline double2 hf_element_with(GV gv, const double2* diag, int X, int
         if (sx < sp.Nq) v.x += sp.wq;
         double2 acc = cscale(diag[X], sp.U);  // -i U g_XX
         double2 lr = make_double2(0.0, 0.0);
-        for (int Z = 0; Z < N; ++Z) {
-            if (Z == X) continue;
-            const double w = w_xy(X, Z, Ns, sp);
-            if (w != 0.0) lr = cadd(lr, cscale(diag[Z], w));
-        }
+        for (int oz = 0, Z = 0; oz < Nb; ++oz)
+            for (int sz = 0; sz < Ns; ++sz, ++Z) {
+                if (Z == X) continue;
+                const double w = site_w(sx, sz, sp);
+                if (w != 0.0) lr = cadd(lr, cscale(diag[Z], w));
+            }
         acc = cadd(acc, cscale(lr, 2.0));
         v = cadd(v, cmul_mi(acc));
     } else {
