@@ -1144,10 +1144,10 @@ __device__ __forceinline__ double2 to_d2(fv2 v) { return make_double2(v.x, v.y);
 //   fused tile, fixed (p1, q1), element (m, n) = (p2, q2):
 //     g22 (e[m] - e[n]),  e[x] = i Qp[p1][q1][x] + F[p1][q1][x].
 // g11 and conj g(q1,p1) (strided) and g22 (fused) are read per element from
-// the G1 pair table, which every tile shares; the vectors hold one element
-// per lane (N <= 64) and reach the lanes that need them through cross-lane
-// permutes.  The tables depend only on the stage G1 and on the partial traces
-// of the stage input, so the strided pass can run after them.
+// the G1 pair table, which every tile shares; the vectors are loaded by one
+// wave, one element per lane (N <= 64), and reach the lanes that need them
+// through LDS.  The tables depend only on the stage G1 and on the partial
+// traces of the stage input, so the strided pass can run after them.
 struct SrcVec {
     double2 q, f, g;  // Qp row, F row, g22 (strided only)
     double w0, w1;    // W rows at p2 and q2 (strided only)
@@ -1181,6 +1181,12 @@ template <int NP, int NW, int PREC, bool FUSED, bool FULL>
 __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     typedef Engine<NP, NW, PREC> E;
     __shared__ typename E::Lds L;
+    // source vectors of the current tile, one element per row/column index:
+    // strided (a, b, c), fused (e).  The FP64 fused pass keeps them in
+    // registers and permutes them across lanes instead (measured 4% faster
+    // there; through LDS everywhere else)
+    constexpr bool SRC_LDS = !(FUSED && PREC == G12_PREC_FP64);
+    __shared__ double2 srcv[FUSED ? 1 : 3][NP];
     constexpr int NT = TileCfg<NP, NW>::NT, TT = TileCfg<NP, NW>::TT, PW = TileCfg<NP, NW>::PW;
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
     constexpr int TILING = FUSED ? 1 : -1;
@@ -1233,8 +1239,11 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // source vectors of the current tile: loaded after the previous tile's
     // epilogue, used in this tile's
+    // the first wave loads them (one element per lane) and writes them to LDS
+    // with the tile's stage input
+    const bool lead = uni(threadIdx.x >> 6) == 0 || !SRC_LDS;
     SrcVec sv{};
-    if (a.sources) {
+    if (a.sources && lead) {
         View v0;
         load_view<FUSED>(v0, a, ss, tr.traj, tr.spin);
         src_load<FUSED>(v0, tr, lane, sv);
@@ -1271,6 +1280,15 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
             if (EXACT || e < N * N) E::put_x(L, e / N, e % N, to_d2(Pc[i]));
+        }
+        if (SRC_LDS && a.sources && lead && lane < N) {
+            if constexpr (FUSED) {
+                srcv[0][lane] = cadd(cmul_i(sv.q), sv.f);
+            } else {
+                srcv[0][lane] = cadd(cscale(sv.g, sv.w0), cmul_i(sv.q));
+                srcv[1][lane] = cadd(cscale(sv.g, sv.w1), cmul_i(sv.q));
+                srcv[2][lane] = cconj(sv.f);
+            }
         }
         lds_sync();
         ++it;
@@ -1339,10 +1357,8 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
 
         if constexpr (!FUSED) {
             double2* out = v.R;
-            // strided-pass sources: g11 (va[m] - vb[n]) - conj g(q1,p1) (vc[n] - vc[m])
-            const double2 va = cadd(cscale(sv.g, sv.w0), cmul_i(sv.q));
-            const double2 vb = cadd(cscale(sv.g, sv.w1), cmul_i(sv.q));
-            const double2 vc = cconj(sv.f);
+            // strided-pass sources: g11 (va[m] - vb[n]) - conj g(q1,p1) (vc[n] - vc[m]),
+            // va, vb, vc in srcv[0..2]
             auto store = [&](auto compact) {
                 constexpr bool C = decltype(compact)::value;
 #pragma unroll
@@ -1351,17 +1367,17 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                     if (!ALLB && t >= TT) continue;
                     const int n = (t % NT) * 16 + (lane & 15);
                     double2 bn{}, cn{};
-                    if (a.sources) {  // permutes in wave-uniform control flow
-                        bn = lane_value(vb, n);
-                        cn = lane_value(vc, n);
+                    if (a.sources) {
+                        bn = srcv[1][n];
+                        cn = srcv[2][n];
                     }
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
                         double2 am{}, cm{};
                         if (a.sources) {
-                            am = lane_value(va, m);
-                            cm = lane_value(vc, m);
+                            am = srcv[0][m];
+                            cm = srcv[2][m];
                         }
                         if (inside(t, m, n)) {
                             const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
@@ -1382,12 +1398,12 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
             else
                 store(std::false_type{});
         } else {
+            const double2 ve = cadd(cmul_i(sv.q), sv.f);  // e = i Qp + F (register form)
             // fused update for tile (p1, q1) = (t0, t1), element (p2, q2) = (m, n):
             // the accumulators hold R + O (R: the strided-pass terms and their
             // sources); with this pass's sources g22 (ve[m] - ve[n]),
             // K = -i (R + O + sources); then A += w K and the next stage input
             // Y = S + c_next K, or at the last stage S += dt (A + w K) / 6.
-            const double2 ve = cadd(cmul_i(sv.q), sv.f);
             auto update = [&](auto compact) {
                 constexpr bool C = decltype(compact)::value;
 #pragma unroll
@@ -1395,11 +1411,11 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                     const int t = wave + NW * u;
                     if (!ALLB && t >= TT) continue;
                     const int n = (t % NT) * 16 + (lane & 15);
-                    const double2 en = a.sources ? lane_value(ve, n) : double2{};
+                    const double2 en = !a.sources ? double2{} : SRC_LDS ? srcv[0][n] : lane_value(ve, n);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
-                        const double2 em = a.sources ? lane_value(ve, m) : double2{};
+                        const double2 em = !a.sources ? double2{} : SRC_LDS ? srcv[0][m] : lane_value(ve, m);
                         if (!inside(t, m, n)) continue;
                         const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
                         double2 rr = make_double2(ar[u][j], ai[u][j]);  // R + O
@@ -1432,7 +1448,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
             else
                 update(std::false_type{});
         }
-        if (a.sources && gnext >= 0) {
+        if (a.sources && lead && gnext >= 0) {
             if (nt.traj == v.traj && nt.spin == v.spin) {
                 src_load<FUSED>(v, nt, lane, sv);
             } else {
@@ -1501,7 +1517,8 @@ int device_cus() {
 
 template <int NP, int NW, int PREC, bool FULL>
 void launch_tile_cfg(const BTileArgs& a, hipStream_t s) {
-    constexpr int lds = (int)sizeof(typename Engine<NP, NW, PREC>::Lds);
+    // the engine's tile images plus the source vectors (3 x NP complex at most)
+    constexpr int lds = (int)sizeof(typename Engine<NP, NW, PREC>::Lds) + 3 * NP * (int)sizeof(double2);
     int per_cu = (160 * 1024) / lds;
     const int by_waves = 32 / NW;  // at most 32 waves per CU
     if (per_cu > by_waves) per_cu = by_waves;
