@@ -31,7 +31,7 @@ METRIC = "G2 RK4 steps/s at N=64 (FP64 vs mixed @≤1e-6 G1 err); batched traj/s
 # written = WRITE_SIZE KiB).
 TRAFFIC_BYTES = {("fp64", "tile_fused"): 3.01e9, ("fp64", "tile_strided"): 1.10e9,
                  ("fp32_3m", "tile_fused"): 2.50e9, ("fp32_3m", "tile_strided"): 0.557e9,
-                 ("bf16x3", "tile_fused"): 2.34e9, ("bf16x3", "tile_strided"): 0.559e9}
+                 ("bf16x3", "tile_fused"): 2.34e9, ("bf16x3", "tile_strided"): 0.571e9}
 
 
 def workload_params(g, precision="fp64", time_steps=5000):

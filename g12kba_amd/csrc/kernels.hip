@@ -1528,9 +1528,10 @@ void launch_tile_cfg(const BTileArgs& a, hipStream_t s) {
     if (grid > a.k.tiles) grid = a.k.tiles;
     if (grid >= 8) grid &= ~7;
     if (grid < 1) return;
-    // the FP64 fused pass keeps the general kernel: its full-tile form spills
-    // registers (9% more HBM reads) and ran no faster
-    constexpr bool FULL_FUSED = FULL && PREC != G12_PREC_FP64;
+    // the full-tile form serves the fused pass in FP32-3M only: in FP64 it
+    // spills registers (9% more HBM reads) and ran no faster, in BF16x3 it ran
+    // 20% slower
+    constexpr bool FULL_FUSED = FULL && PREC == G12_PREC_FP32_3M;
     if (a.fused)
         hipLaunchKernelGGL((k_tile<NP, NW, PREC, true, FULL_FUSED>), dim3((unsigned)grid),
                            dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
@@ -1549,14 +1550,11 @@ void launch_tile_np(const BTileArgs& a, hipStream_t s) {
     constexpr int W0 = base_waves(NP);
     if (a.k.waves >= 4 * W0) {
         // buckets whose trajectories all have N = NP take the full-tile
-        // kernels, except BF16x3, whose fused pass measured 20% slower in them
-        if constexpr (PREC != G12_PREC_BF16X3) {
-            if (a.k.full) {
-                launch_tile_cfg<NP, 4 * W0, PREC, true>(a, s);
-                return;
-            }
-        }
-        launch_tile_cfg<NP, 4 * W0, PREC, false>(a, s);
+        // kernels (for the fused pass where they measured faster)
+        if (a.k.full)
+            launch_tile_cfg<NP, 4 * W0, PREC, true>(a, s);
+        else
+            launch_tile_cfg<NP, 4 * W0, PREC, false>(a, s);
     } else if (a.k.waves >= 2 * W0) {
         launch_tile_cfg<NP, 2 * W0, PREC, false>(a, s);
     } else {
