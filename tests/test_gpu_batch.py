@@ -4,7 +4,7 @@ the tile autotuner must not change a single bit."""
 import numpy as np
 import pytest
 
-from tests.helpers import oracle_cfg
+from tests.helpers import oracle_cfg, physical_g2
 
 pytestmark = pytest.mark.gpu
 
@@ -200,3 +200,25 @@ def test_batch_checkpoint_resume_is_bit_exact(tmp_path, prec):
         assert np.array_equal(np.concatenate([first[i], rest[i]]), full[i]), f"trajectory {i}"
         for x, y in zip(end[i], end2[i]):
             assert np.array_equal(x, y), f"trajectory {i}"
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_3m", "bf16x3"])
+def test_full_tile_kernels_match_general_kernels(prec):
+    """A bucket whose trajectories all have N = NP runs the full-tile kernels
+    (no element bounds checks); sharing the bucket with a smaller N makes the
+    same trajectory run the general ones.  The N = 48 trajectory must come out
+    the same bit for bit (16 waves: the default at NP = 48, autotuner off)."""
+    sizes = [(24, 2), (47, 1)]
+    ps = ensemble(11, 2, 6, sizes=sizes, precision=prec)
+    rng = np.random.default_rng(12)
+    G1 = g.make_G0(ps[0])
+    s48 = (G1, physical_g2(rng, 24, 2), physical_g2(rng, 24, 2))
+    s47 = start(ps[1])
+    a1, rc1, traj1, fin1, b1 = run_batch([ps[0]], [s48], autotune=False)
+    a2, rc2, traj2, fin2, b2 = run_batch(ps, [s48, s47], autotune=False)
+    assert rc1 == 0 and rc2 == 0
+    assert [k[0] for k in b1] == [48] and [k[0] for k in b2] == [48]
+    assert a1[0] == a2[0] and len(traj1[0]) == len(traj2[0]) > 0
+    assert np.array_equal(traj1[0], traj2[0])
+    for x, y in zip(fin1[0], fin2[0]):
+        assert np.array_equal(x, y)
