@@ -86,6 +86,7 @@ def _load():
         "g12_hf_hamiltonian": [p, p, p, i, i, d, d, d, d, p, p, i, i],
         "g12_h1_commutator": [p, p, p, i],
         "g12_h2_commutator": [p, p, p, p, i, i, i],
+        "g12_h2_commutator_noalloc": [vp, vp, vp, vp, i, i, i, vp],
         "g12_collision": [p, p, p, i, i, d, p, i],
         "g12_psi_xxxx": [p, p, i, i, d, p, i],
         "g12_psi_xyxy": [p, p, i, i, d, p, i],
@@ -155,7 +156,7 @@ EXPORTED = [
     "g12_params_default", "g12_parse_input", "g12_make_g0", "g12_init_g_test", "g12_create", "g12_destroy",
     "g12_set_state", "g12_get_state", "g12_adiabatic_switching", "g12_rk4", "g12_rk4_advance",
     "g12_get_time", "g12_rk4_record", "g12_timing_enable", "g12_timing_get", "g12_memory_bytes", "g12_synchronize",
-    "g12_hf_hamiltonian", "g12_h1_commutator", "g12_h2_commutator", "g12_collision",
+    "g12_hf_hamiltonian", "g12_h1_commutator", "g12_h2_commutator", "g12_h2_commutator_noalloc", "g12_collision",
     "g12_psi_xxxx", "g12_psi_xyxy", "g12_pi", "g12_g1_step", "g12_g2_step", "g12_fermi",
     "g12_dipole", "g12_energy", "g12_state_energy", "g12_batch_state_energy", "g12_write_outputs", "g12_save_state", "g12_load_state", "g12_batch_save_state", "g12_batch_load_state", "g12_version",
     "g12_configure", "g12_batch_create", "g12_batch_destroy", "g12_batch_size",

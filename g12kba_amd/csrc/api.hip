@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <tuple>
 #include <vector>
@@ -36,6 +37,39 @@ namespace {
         }                                                                              \
     } while (0)
 
+// ---- scratch of the no-allocation entry point -------------------------------
+// g12_h2_commutator_noalloc keeps its launch descriptors and the natural-layout
+// copies of its input and result in this array; every GPU the library runs on
+// holds its own copy (at its own address) until the process ends.  Host code
+// never uses the array's name as an address: scratch_address() looks the
+// device address up once per device, with that device current.
+constexpr int64_t NOALLOC_N4 = (int64_t)G12_NOALLOC_MAX_N * G12_NOALLOC_MAX_N * G12_NOALLOC_MAX_N * G12_NOALLOC_MAX_N;
+constexpr int64_t NOALLOC_BYTES = g12::COMM2_DESC_BYTES + 2 * NOALLOC_N4 * (int64_t)sizeof(double2);
+static_assert(NOALLOC_BYTES == G12_NOALLOC_SCRATCH_BYTES, "scratch size as declared in g12kba.h");
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+void* scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on device dev (which must be current), or
+// nullptr with a message when the lookup fails.
+void* scratch_address(int dev) {
+    if (dev < 0 || dev >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            std::fprintf(stderr, "g12kba: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+                         hipGetErrorString(e));
+            return nullptr;
+        }
+        scratch_ptr[dev] = p;
+    }
+    return scratch_ptr[dev];
+}
+
 int select_device() {
     static int dev = -1;
     if (dev >= 0) return dev;
@@ -45,6 +79,8 @@ int select_device() {
     dev = s ? std::atoi(s) : 0;
     if (dev < 0 || dev >= count) dev = 0;
     if (hipSetDevice(dev) != hipSuccess) return -1;
+    // library set-up on the selected device: the no-allocation scratch address
+    (void)scratch_address(dev);
     return dev;
 }
 
@@ -1265,6 +1301,47 @@ int g12_pi(double* pi, const double* G1y, const double* G2xyxy, const double* G2
     g12::launch_pi(dg, dF, dT, N, api_stream());
     g12::launch_nat_to_ref(dT, draw, Ns, Nb, api_stream());
     return down(pi, draw, sizeof(double2) * n4);
+}
+
+int g12_h2_commutator_noalloc(const double* G2, const double* h1x, const double* h1y, double* comm, int Ns, int Nb,
+                              int precision, void* stream) {
+    if (!G2 || !h1x || !h1y || !comm || Ns < 1 || Nb < 1 || Nb > G12_MAX_NB) return G12_E_ARG;
+    if (precision < G12_PREC_FP64 || precision > G12_PREC_BF16X3) return G12_E_ARG;
+    const int N = Ns * Nb;
+    if (N > G12_NOALLOC_MAX_N) {
+        std::fprintf(stderr,
+                     "g12_h2_commutator_noalloc: N = Ns*Nb = %d exceeds the no-allocation limit %d "
+                     "(use g12_h2_commutator)\n",
+                     N, G12_NOALLOC_MAX_N);
+        return G12_E_SIZE;
+    }
+    // the thread's last-error state may hold an earlier, unrelated call's
+    // result (a stream or event query that returned "not ready"); the launch
+    // checks below must see only this call's launches
+    (void)hipGetLastError();
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "g12_h2_commutator_noalloc: hipGetDevice failed: %s\n", hipGetErrorString(e));
+        return G12_E_HIP;
+    }
+    char* base = static_cast<char*>(scratch_address(dev));
+    if (!base) return G12_E_HIP;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int64_t n4 = (int64_t)N * N * N * N;
+    // scratch: descriptors, natural-layout input, natural-layout result
+    double2* nat_in = reinterpret_cast<double2*>(base + g12::COMM2_DESC_BYTES);
+    double2* nat_out = nat_in + n4;
+    g12::launch_ref_to_nat(reinterpret_cast<const double2*>(G2), nat_in, Ns, Nb, s);
+    (void)g12::enqueue_comm2(nat_in, reinterpret_cast<const double2*>(h1x), reinterpret_cast<const double2*>(h1y),
+                             nat_out, N, precision, base, s);
+    g12::launch_nat_to_ref(nat_out, reinterpret_cast<double2*>(comm), Ns, Nb, s);
+    e = hipGetLastError();
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "g12_h2_commutator_noalloc: launch failed: %s\n", hipGetErrorString(e));
+        return G12_E_HIP;
+    }
+    return 0;
 }
 
 int g12_g1_step(const double* comm, const double* I, double* K, double* RK_result, double w, int Ns, int Nb) {

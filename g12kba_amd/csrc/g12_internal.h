@@ -133,6 +133,12 @@ void launch_nat_to_ref(const double2* in, double2* out, int Ns, int Nb, hipStrea
 // synchronous: 0, G12_E_NOMEM or G12_E_HIP
 int launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
                  hipStream_t s);
+// The same enqueued on s without waiting, its launch descriptors written (by a
+// kernel) into desc_area, COMM2_DESC_BYTES of device memory that must stay
+// untouched until the launches complete; allocates nothing.
+constexpr int COMM2_DESC_BYTES = 4096;
+int enqueue_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
+                  void* desc_area, hipStream_t s);
 void launch_hf(double2* h, const double2* G1, int Ns, int Nb, const StageParams& sp, hipStream_t s);
 void launch_comm1(const double2* G, const double2* h, double2* out, int N, hipStream_t s);
 void launch_d_gather(const double2* T, double2* D, int N, hipStream_t s);

@@ -1829,8 +1829,27 @@ void launch_nat_to_ref(const double2* in, double2* out, int Ns, int Nb, hipStrea
     hipLaunchKernelGGL(k_nat_to_ref, dim3(N * N), dim3(256), 0, s, in, out, Ns, Nb);
 }
 
-int launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
-                 hipStream_t s) {
+// Launch descriptors of a one-trajectory batch in a device area of
+// COMM2_DESC_BYTES: DevState at 0, TrajParams at 512, the bucket list at 2048,
+// the tile prefix at 2304.  Written by a kernel (kernel arguments by value),
+// so the whole enqueue needs no host copy and can be captured in a graph.
+constexpr int COMM2_DS = 0, COMM2_TP = 512, COMM2_LIST = 2048, COMM2_PREFIX = 2304;
+static_assert(sizeof(DevState) <= COMM2_TP - COMM2_DS, "DevState fits its slot");
+static_assert(COMM2_TP + sizeof(TrajParams) <= COMM2_LIST, "TrajParams fits its slot");
+static_assert(COMM2_PREFIX + 2 * sizeof(int64_t) <= COMM2_DESC_BYTES, "descriptors fit");
+
+static __global__ void k_comm2_desc(char* desc, DevState d, TrajParams tp, int64_t tiles) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *reinterpret_cast<DevState*>(desc + COMM2_DS) = d;
+    *reinterpret_cast<TrajParams*>(desc + COMM2_TP) = tp;
+    *reinterpret_cast<int*>(desc + COMM2_LIST) = 0;
+    int64_t* pre = reinterpret_cast<int64_t*>(desc + COMM2_PREFIX);
+    pre[0] = 0;
+    pre[1] = tiles;
+}
+
+int enqueue_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
+                  void* desc_area, hipStream_t s) {
     // a batch of one trajectory whose spin-0 state is T and strided-pass
     // output is `out`; four accumulating launches (strided, contiguous) x
     // (left hx, right hy)
@@ -1841,39 +1860,16 @@ int launch_comm2(const double2* T, const double2* hx, const double2* hy, double2
     TrajParams tp{};
     tp.N = N;
     tp.corr = 1;
-    DevState* dd = nullptr;
-    TrajParams* dt = nullptr;
-    int* dl = nullptr;
-    int64_t* dp = nullptr;
-    auto release = [&] {
-        if (dd) (void)hipFree(dd);
-        if (dt) (void)hipFree(dt);
-        if (dl) (void)hipFree(dl);
-        if (dp) (void)hipFree(dp);
-    };
-    if (hipMalloc(&dd, sizeof d) != hipSuccess || hipMalloc(&dt, sizeof tp) != hipSuccess ||
-        hipMalloc(&dl, sizeof(int)) != hipSuccess || hipMalloc(&dp, 2 * sizeof(int64_t)) != hipSuccess) {
-        release();
-        return G12_E_NOMEM;
-    }
-    const int zero = 0;
-    const int64_t pre[2] = {0, (int64_t)N * N};
-    if (hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(dt, &tp, sizeof tp, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(dl, &zero, sizeof zero, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(dp, pre, sizeof pre, hipMemcpyHostToDevice, s) != hipSuccess) {
-        (void)hipStreamSynchronize(s);
-        release();
-        return G12_E_HIP;
-    }
+    char* desc = static_cast<char*>(desc_area);
+    hipLaunchKernelGGL(k_comm2_desc, dim3(1), dim3(64), 0, s, desc, d, tp, (int64_t)N * N);
     BTileArgs a{};
-    a.b.ds = dd;
-    a.b.tp = dt;
+    a.b.ds = reinterpret_cast<const DevState*>(desc + COMM2_DS);
+    a.b.tp = reinterpret_cast<const TrajParams*>(desc + COMM2_TP);
     a.b.ntraj = 1;
     a.b.maxN = N;
     a.k.NP = N <= 16 ? 16 : N <= 32 ? 32 : N <= 48 ? 48 : 64;
-    a.k.list = dl;
-    a.k.prefix = dp;
+    a.k.list = reinterpret_cast<const int*>(desc + COMM2_LIST);
+    a.k.prefix = reinterpret_cast<const int64_t*>(desc + COMM2_PREFIX);
     a.k.n = 1;
     a.k.tiles = (int64_t)N * N;
     a.k.waves = default_waves(a.k.NP);
@@ -1893,10 +1889,17 @@ int launch_comm2(const double2* T, const double2* hx, const double2* hy, double2
             first = 0;
             launch_tile(a, precision, s);
         }
-    const hipError_t launched = hipGetLastError();
-    const hipError_t e = hipStreamSynchronize(s);  // the tiles read dd..dp until here
-    release();
-    return (launched == hipSuccess && e == hipSuccess) ? 0 : G12_E_HIP;
+    return hipGetLastError() == hipSuccess ? 0 : G12_E_HIP;
+}
+
+int launch_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
+                 hipStream_t s) {
+    void* desc = nullptr;
+    if (hipMalloc(&desc, COMM2_DESC_BYTES) != hipSuccess) return G12_E_NOMEM;
+    const int launched = enqueue_comm2(T, hx, hy, out, N, precision, desc, s);
+    const hipError_t e = hipStreamSynchronize(s);  // the tiles read the descriptors until here
+    (void)hipFree(desc);
+    return (launched == 0 && e == hipSuccess) ? 0 : G12_E_HIP;
 }
 
 void launch_hf(double2* h, const double2* G1, int Ns, int Nb, const StageParams& sp, hipStream_t s) {

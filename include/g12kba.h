@@ -272,6 +272,25 @@ int g12_h1_commutator(const double* G, const double* h, double* comm, int N);
 int g12_h2_commutator(const double* G2, const double* h1x, const double* h1y, double* comm, int Ns,
                       int Nb, int precision);
 
+/* No reference counterpart: a variant of g12_h2_commutator for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  G2, h1x, h1y and comm are DEVICE pointers
+ * (reference layouts, as g12_h2_commutator); `stream` is a hipStream_t (NULL:
+ * the null stream).  The work is enqueued on the stream and the call returns
+ * without waiting; it calls no hipMalloc/hipFree and no host copy, so it can
+ * be captured in a graph.  Its scratch (launch descriptors and the
+ * natural-layout copies of G2 and of the result) is one static device array of
+ * G12_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, which limits
+ * N = Ns*Nb to G12_NOALLOC_MAX_N (G12_E_SIZE beyond).  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (on different
+ * streams, or with the scratch of an earlier call still in flight elsewhere)
+ * race.  The call clears the calling thread's HIP last-error state before its
+ * launches and reports only their errors. */
+#define G12_NOALLOC_MAX_N 24
+#define G12_NOALLOC_SCRATCH_BYTES (4096 + 2LL * 24 * 24 * 24 * 24 * 16)
+int g12_h2_commutator_noalloc(const double* G2, const double* h1x, const double* h1y, double* comm, int Ns,
+                              int Nb, int precision, void* stream);
+
 /* Collision integral (collision_int, collision_integral.h:27-69); note the
  * reference's argument order (xyxy, xxxx). */
 int g12_collision(double* Ix, const double* G2xyxy, const double* G2xxxx, int Ns, int Nb, double U,
