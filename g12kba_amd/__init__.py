@@ -499,16 +499,20 @@ class Propagator:
         rc = _check(lib.g12_rk4(self.h, _p(traj), traj_cap, ctypes.byref(n)), "g12_rk4")
         return rc, traj[: min(n.value, traj_cap)]
 
-    def advance(self, nsteps):
+    def advance(self, nsteps, done=False):
+        """Advance nsteps from the current time; returns rc, or (rc, iterations
+        begun) with done=True (fewer when the density guard trips)."""
         n = ctypes.c_int64(0)
-        return _check(lib.g12_rk4_advance(self.h, nsteps, ctypes.byref(n)), "g12_rk4_advance")
+        rc = _check(lib.g12_rk4_advance(self.h, nsteps, ctypes.byref(n)), "g12_rk4_advance")
+        return (rc, n.value) if done else rc
 
     def record(self, nsteps):
-        """Advance nsteps from the current time; returns (rc, G1 before each step)."""
+        """Advance nsteps from the current time; returns (rc, G1 before each
+        step begun: up to and including the failing one if the guard trips)."""
         traj = np.zeros((max(nsteps, 1), self.N, self.N), np.complex128)
         n = ctypes.c_int64(0)
         rc = _check(lib.g12_rk4_record(self.h, nsteps, _p(traj), nsteps, ctypes.byref(n)), "g12_rk4_record")
-        return rc, traj[:nsteps]
+        return rc, traj[: n.value]
 
     def time(self):
         t = ctypes.c_double(0)

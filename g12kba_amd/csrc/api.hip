@@ -120,6 +120,9 @@ bool same_protocol(const g12_params& a, const g12_params& b) {
 // U_AS being the value set by the last stage of the previous iteration (the
 // first iteration never counts).  Depends on U through the rounding of
 // fermi(t) * U.
+// Returns -1 when the switching would not end within MAX_AS_STEPS steps (the
+// exact test can fail forever, e.g. for a ramp that never reaches 1 in double).
+constexpr int MAX_AS_STEPS = 10000000;
 int as_step_count(const g12_params& p) {
     int count = 0, n = 0;
     bool have = false;
@@ -133,9 +136,30 @@ int as_step_count(const g12_params& p) {
         }
         have = true;
         t += p.AS_dt;
-        if (++n > 100000000) break;  // unreachable for finite parameters
+        if (++n > MAX_AS_STEPS) return -1;
     }
     return n;
+}
+
+// Parameters that would make a driver loop forever or compute with
+// non-finite numbers.  The reference's parser gives 0 for a missing key
+// (read_input.h:53-83), and those runs end: AS_rate = 0 makes the ramp a step
+// (exp of an infinite argument), dt = 0 gives tmax = 0 and no production step.
+// A negative AS_rate (a ramp that falls: the exact U_AS == U test of
+// AS.h:96-100 never succeeds), AS_dt <= 0 (the switching clock stands still)
+// and a production clock that never reaches tmax are rejected.
+bool valid_protocol(const g12_params& p) {
+    const double v[] = {p.t1, p.t2, p.U, p.dt, p.quench_strength, p.AS_rate, p.AS_midpoint, p.quench_rate,
+                        p.quench_on_midpoint, p.quench_off_midpoint, p.decay_rate, p.AS_dt, p.quench_gate_time};
+    for (double x : v)
+        if (!std::isfinite(x)) return false;
+    for (int i = 0; i < G12_MAX_NB; ++i)
+        if (!std::isfinite(p.eps[i])) return false;
+    if (p.V_law == G12_VLAW_EXPLICIT)
+        for (int i = 0; i < p.nV_profile && i < G12_MAX_SHELLS; ++i)
+            if (!std::isfinite(p.V_profile[i])) return false;
+    const bool clock_ends = p.dt > 0 || double(p.time_steps) * p.dt <= 0;  // while (t < tmax), RK_solver.h:96
+    return clock_ends && p.AS_dt > 0 && p.AS_rate >= 0 && p.AS_tail >= 0 && p.Nq >= 0;
 }
 
 // Scalars of RK4 stage k of a step of length dt_full.
@@ -322,6 +346,18 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
     for (int i = 0; i < ntraj; ++i) {
         if (!valid_geometry(p[i].Ns, p[i].Nb)) return G12_E_SIZE;
         if (!same_protocol(p[0], p[i])) return G12_E_ARG;
+        if (!valid_protocol(p[i])) {
+            std::fprintf(stderr,
+                         "g12kba: trajectory %d: a non-finite parameter, AS_rate < 0, AS_dt <= 0 or a time step "
+                         "that never reaches time_steps * dt\n",
+                         i);
+            return G12_E_ARG;
+        }
+        if (as_step_count(p[i]) < 0) {
+            std::fprintf(stderr, "g12kba: trajectory %d: adiabatic switching does not end within %d steps\n", i,
+                         MAX_AS_STEPS);
+            return G12_E_ARG;
+        }
     }
     if (select_device() < 0) return G12_E_HIP;
     g12::device_cu_count();  // cached before any graph capture
@@ -352,7 +388,7 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
         t.qstr = q.quench_strength;
         t.nV = v_profile(q, t.Vprof);
         for (int o = 0; o < G12_MAX_NB; ++o) t.eps[o] = o < q.Nb ? q.eps[o] : 0.0;
-        r.as_steps[i] = as_step_count(q);
+        r.as_steps[i] = as_step_count(q);  // finite: checked above
         t.as_steps = r.as_steps[i];
         r.max_as = std::max(r.max_as, t.as_steps);
         r.maxN = std::max(r.maxN, d.N);
@@ -653,7 +689,8 @@ void production_blocks(const g12_params& p, double t, int64_t n, int slot, Stage
 // (capacity cap matrices, from step index `rec0`) when traj is given.
 int run_production(Run& r, int64_t nsteps, double* const* traj, int64_t cap, int64_t rec0) {
     const g12_params& p = r.p[0];
-    double t = r.t;
+    const double t0 = r.t;
+    double t = t0;
     const int64_t s0 = r.steps;
     int rc = run_windows(
         r, 1, nsteps,
@@ -680,10 +717,30 @@ int run_production(Run& r, int64_t nsteps, double* const* traj, int64_t cap, int
             HIPCHK(hipStreamSynchronize(r.stream));
             return all_failed ? 1 : 0;
         });
+    r.timing.resolve();
+    if (rc) return rc;
+    // A run of one trajectory whose guard tripped at step f - 1 (fail = f)
+    // holds the state before that step: its clock stops there.  The
+    // trajectories of a larger run share one clock, which goes on.
+    const int f = r.h_fail[0];
+    if (r.ntraj == 1 && f > s0) {
+        t = t0;
+        for (int64_t i = s0; i < f - 1; ++i) t += p.dt;  // the accumulated time of RK_solver.h:216
+        r.t = t;
+        r.steps = f - 1;
+        return 0;
+    }
     r.t = t;
     r.steps = s0 + nsteps;
-    r.timing.resolve();
-    return rc;
+    return 0;
+}
+
+// Production iterations begun by a run of one trajectory in a call that
+// started at step s0 and asked for n: n, or up to and including the step
+// whose guard tripped.
+int64_t iterations_begun(const Run& r, int64_t s0, int64_t n) {
+    const int f = r.h_fail[0];
+    return (f > s0 && f - s0 < n) ? f - s0 : n;
 }
 
 int64_t production_step_count(const g12_params& p) {
@@ -696,6 +753,7 @@ int64_t production_step_count(const g12_params& p) {
 int reset_guard(Run& r) {
     HIPCHK(hipMemsetAsync(r.d_fail, 0, sizeof(int) * r.ntraj, r.stream));
     HIPCHK(hipStreamSynchronize(r.stream));
+    for (int i = 0; i < r.ntraj; ++i) r.h_fail[i] = 0;
     return 0;
 }
 
@@ -750,6 +808,7 @@ int set_state(Run& r, int i, const double* G1, const double* G2xx, const double*
     const hipError_t es = hipStreamSynchronize(r.stream);
     if (tmp) (void)hipFree(tmp);
     if (e != hipSuccess || es != hipSuccess) return G12_E_HIP;
+    r.h_fail[i] = 0;
     r.t = 0.0;
     r.steps = 0;
     return 0;
@@ -880,20 +939,29 @@ int g12_rk4(g12_ctx* c, double* traj, int64_t traj_cap, int64_t* steps) {
 
 int g12_rk4_advance(g12_ctx* c, int64_t nsteps, int64_t* done) {
     if (!c || nsteps < 0) return G12_E_ARG;
+    if (c->r.h_fail[0]) {  // stopped by the guard: nothing runs until set_state / load_state
+        if (done) *done = 0;
+        return G12_GUARD;
+    }
+    const int64_t s0 = c->r.steps;
     const int rc = run_advance(c->r, nsteps);
     if (rc < 0) return rc;
-    if (done) *done = nsteps;
+    if (done) *done = iterations_begun(c->r, s0, nsteps);
     return rc;
 }
 
 int g12_rk4_record(g12_ctx* c, int64_t nsteps, double* traj, int64_t traj_cap, int64_t* done) {
     if (!c || nsteps < 0) return G12_E_ARG;
     Run& r = c->r;
+    if (r.h_fail[0]) {  // stopped by the guard: nothing runs until set_state / load_state
+        if (done) *done = 0;
+        return G12_GUARD;
+    }
     double* tr[1] = {traj};
     const int64_t s0 = r.steps;
     const int rc = run_production(r, nsteps, traj ? tr : nullptr, traj_cap, s0);
     if (rc) return rc;
-    if (done) *done = nsteps;
+    if (done) *done = iterations_begun(r, s0, nsteps);
     return r.h_fail[0] ? G12_GUARD : 0;
 }
 

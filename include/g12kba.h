@@ -111,6 +111,10 @@ int g12_init_g_test(const g12_params* p, double* G1);
 
 /* ---- context, state, drivers --------------------------------------------- */
 
+/* G12_E_SIZE for sizes the kernels do not support; G12_E_ARG for a
+ * non-finite parameter, AS_rate < 0, AS_dt <= 0, a production clock that
+ * never reaches time_steps * dt, or a switching ramp that does not reach U
+ * exactly (AS.h:96-100) within 10^7 steps. */
 int g12_create(const g12_params* p, g12_ctx** out);
 int g12_destroy(g12_ctx* ctx);
 
@@ -129,12 +133,16 @@ int g12_adiabatic_switching(g12_ctx* ctx, int* steps);
 int g12_rk4(g12_ctx* ctx, double* traj, int64_t traj_cap, int64_t* steps);
 
 /* Advance n RK4 steps of the production protocol from the context's current
- * time without recording (benchmarks, checkpoints).  Same guard semantics. */
+ * time without recording (benchmarks, checkpoints).  Same guard semantics:
+ * when the guard trips, *steps_done counts the iterations begun up to and
+ * including the failing one, and the context's time and step count stop at
+ * the state it holds (the state before the failing step); later calls return
+ * G12_GUARD with nothing done until g12_set_state or g12_load_state. */
 int g12_rk4_advance(g12_ctx* ctx, int64_t n, int64_t* steps_done);
 
 /* Advance n production steps from the context's current time, writing the G1
  * matrix before each step to traj (capacity traj_cap matrices, may be NULL).
- * Same guard semantics as g12_rk4.  No reference counterpart: the reference's
+ * Guard semantics of g12_rk4_advance.  No reference counterpart: the reference's
  * RK4 always starts at t = 0; this resumes a run (checkpoints, extrapolation
  * fallback). */
 int g12_rk4_record(g12_ctx* ctx, int64_t n, double* traj, int64_t traj_cap, int64_t* steps_done);

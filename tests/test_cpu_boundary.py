@@ -156,3 +156,38 @@ def test_init_g_test_domain_wall(Ns, Nb):
     assert np.array_equal(G, expect)
     P = -1j * G
     assert np.allclose(P @ P, P) and abs(np.trace(P) - Nb * (Ns // 2)) == 0
+
+
+@pytest.mark.parametrize("bad", [dict(AS_rate=-0.5), dict(U=float("nan")), dict(t2=float("inf")), dict(AS_dt=0.0),
+                                 dict(dt=-0.01, time_steps=-100), dict(AS_rate=1e6)])
+def test_create_rejects_protocols_that_never_end(bad):
+    """A falling or stalled switching ramp, a clock that never reaches its end
+    and non-finite parameters are G12_E_ARG at create time, before any device
+    work (AS_rate = 1e6: fermi(t) U would reach U exactly only after ~4e8
+    switching steps)."""
+    import ctypes
+    import time
+    p = g.make_params(Ns=3, Nb=1, U=1.0, dt=0.02, time_steps=10)
+    for k, v in bad.items():
+        setattr(p, k, v)
+    h = ctypes.c_void_p()
+    t0 = time.perf_counter()
+    assert g.lib.g12_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert time.perf_counter() - t0 < 5
+    assert g.lib.g12_batch_create(ctypes.byref(p), 1, ctypes.byref(h)) == -1
+
+
+@pytest.mark.parametrize("ok", [dict(), dict(AS_rate=0.0, AS_midpoint=0.0), dict(dt=0.0)])
+def test_create_accepts_what_the_reference_runs(ok):
+    """Missing keys parse as 0 in the reference (read_input.h:53-83): AS_rate = 0
+    (a step ramp) and dt = 0 (no production step) still run there.  Without a
+    GPU creation stops at device selection (G12_E_HIP), after validation."""
+    import ctypes
+    p = g.make_params(Ns=3, Nb=1, U=1.0, dt=0.02, time_steps=10)
+    for k, v in ok.items():
+        setattr(p, k, v)
+    h = ctypes.c_void_p()
+    rc = g.lib.g12_create(ctypes.byref(p), ctypes.byref(h))
+    if rc == 0:
+        g.lib.g12_destroy(h)
+    assert rc in (0, -2)

@@ -173,9 +173,12 @@ def test_n64_invariants(prec, tol):
 
 
 def n64_params(prec):
+    """The N = 64 workload with its quench ramp moved to t = 0 (gate open,
+    midpoint 0.01, rate 2): the two steps run through the rising ramp, so the
+    quench shift acts on the first 8 sites at every stage."""
     return g.make_params(Ns=32, Nb=2, t1=1.0, t2=0.5, U=1.5, dt=0.01, time_steps=2, quench_strength=1.0, Nq=8,
-                         q_type="full", EHM=True, V_law="coulomb", quench_rate=2.0, quench_on_midpoint=30.0,
-                         AS_rate=0.5, AS_midpoint=10.0, precision=prec)
+                         q_type="full", EHM=True, V_law="coulomb", quench_rate=2.0, quench_on_midpoint=0.01,
+                         quench_gate_time=0.0, AS_rate=0.5, AS_midpoint=10.0, precision=prec)
 
 
 @pytest.fixture(scope="module")
