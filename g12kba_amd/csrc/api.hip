@@ -298,11 +298,13 @@ int64_t layout(Run& r, char* base) {
         d.Y1 = cv.take<double2>(d.n2);
         d.h = cv.take<double2>(d.n2);
         d.I = cv.take<double2>(d.n2);
-        d.G1p = cv.take<double2>(2 * d.n2);
+        d.G1t = cv.take<double2>(d.n2);
         d.Wxy = cv.take<double>(d.n2);
         d.Wxx = cv.take<double>(d.n2);
-        // mixed-precision runs keep Y2 and R2 as complex64 (half the elements' bytes)
-        const int64_t n4c = r.precision != G12_PREC_FP64 ? (n4 + 1) / 2 : n4;
+        // mixed-precision runs keep Y2 and R2 as complex64 in the blocked
+        // layout (N^2 Nq^2 elements, Nq = N rounded up to 4; see kernels.hip)
+        const int64_t nq = (d.N + 3) & ~3;
+        const int64_t n4c = r.precision != G12_PREC_FP64 ? (corr ? ((int64_t)d.N * d.N * nq * nq + 1) / 2 : 1) : n4;
         for (int s = 0; s < 2; ++s) {
             d.S2[s] = cv.take<double2>(d.n4);  // the state is kept even when frozen
             d.Y2[s] = cv.take<double2>(n4c);

@@ -68,7 +68,7 @@ struct DevState {
     int Ns, Nb, N;
     int64_t n2, n3, n4;
     double2 *S1, *K1, *A1, *Y1, *h, *I;
-    double2* G1p;                      // pairs (Y1[X][Y], conj Y1[Y][X]) of the stage G1
+    double2* G1t;                      // conj Y1[Y][X] at [X][Y]: the conjugate transpose of the stage G1
     double *Wxy, *Wxx;                 // N x N interaction tables (xy, xx)
     // state, next stage input, RK accumulator, strided-pass result; in
     // mixed-precision runs Y2 and R2 hold complex64 elements

@@ -262,8 +262,8 @@ __device__ inline bool active(const DevState& d, const TrajParams& t, const Stag
     return !(ss.as_phase && ss.step >= t.as_steps);
 }
 
-// Stage set-up for G1: Y1 = S1 + c K1, the pairs (Y1[X][Y], conj Y1[Y][X])
-// read by the strided pass, h = h_HF[Y1], W tables.  Blocks of blockDim elements
+// Stage set-up for G1: Y1 = S1 + c K1 and its conjugate transpose G1t (read
+// by the strided pass), h = h_HF[Y1], W tables.  Blocks of blockDim elements
 // (`chunk`) per trajectory; each block forms the diagonal of Y1 itself (for
 // the Hartree sums, in `diag`) and takes the Y1 elements of its Fock terms
 // from S1 and K1, so the blocks need no ordering among themselves.
@@ -288,8 +288,7 @@ __device__ inline void stage_g1_block(const DevState& d, const StageShared& ss, 
         const int X = e / N, Y = e % N;
         const double2 y = yv(e);
         gstore(d.Y1, e, y);
-        gstore(d.G1p, 2 * (int64_t)e, y);
-        gstore(d.G1p, 2 * (int64_t)e + 1, cconj(yv((int64_t)Y * N + X)));
+        gstore(d.G1t, e, cconj(yv((int64_t)Y * N + X)));
         gstore(d.h, e, hf_element_with(yv, diag, X, Y, d.Ns, d.Nb, sp));
         const double w = w_xy(X, Y, d.Ns, sp);
         ((GLOBAL double*)d.Wxy)[e] = w;
@@ -1007,7 +1006,8 @@ struct View {
     double2* R;         // unfused output / strided-pass result (compact in mixed runs)
     double2* D;         // fused: partial traces of the next stage input
     double2 *A, *Y;     // fused: RK accumulator, next stage input (compact in mixed runs)
-    const double2* G1p; // pairs (g[X][Y], conj g[Y][X]) of the stage G1 g
+    const double2* G1;  // the stage G1 g (Y1)
+    const double2* G1t; // its conjugate transpose: conj g[Y][X] at [X][Y]
     const double2 *Qp, *F;  // source tables of the spin
     const double* W;        // W_xx (spin 0) or W_xy (spin 1)
 };
@@ -1032,7 +1032,8 @@ __device__ inline void load_view(View& v, const BTileArgs& a, const StageShared&
     v.xc = input_compact(a, ss);
     v.S = uni_ptr(d.S2[spin]);
     v.R = uni_ptr(d.R2[spin]);
-    v.G1p = uni_ptr(d.G1p);
+    v.G1 = uni_ptr(d.Y1);
+    v.G1t = uni_ptr(d.G1t);
     v.Qp = uni_ptr(d.Qp[spin]);
     v.F = uni_ptr(d.F[spin]);
     v.W = uni_ptr(spin == 0 ? d.Wxx : d.Wxy);
@@ -1043,22 +1044,74 @@ __device__ inline void load_view(View& v, const BTileArgs& a, const StageShared&
     }
 }
 
-// Element offsets of a tile within one tensor (32-bit: N <= 64).
-__device__ __forceinline__ void tile_geometry(int tiling, int N, int tile, uint32_t& base, uint32_t& sm,
-                                              uint32_t& sn, int& t0, int& t1) {
-    const uint32_t n = (uint32_t)N, N2 = n * n, N3 = N2 * n;
-    t0 = tile / N;
-    t1 = tile % N;
-    if (tiling == 0) {  // strided: fixed (p2, q2) = (t0, t1); m = p1, n = q1
-        base = (uint32_t)t0 * N2 + t1;
-        sm = N3;
-        sn = n;
-    } else {  // contiguous: fixed (p1, q1) = (t0, t1); m = p2, n = q2
-        base = (uint32_t)t0 * N3 + (uint32_t)t1 * n;
-        sm = N2;
-        sn = 1;
+// Element offsets of a tile within one tensor (32-bit: N <= 64).  Element
+// (m, n) of a tile lies at base + m sm + n B in the natural layout
+// T[p1][p2][q1][q2] (the FP64 buffers: state S, RK accumulator A, and Y, R of
+// FP64 runs), and at base + m sm + (n >> 2) A + (n & 3) B in the blocked
+// layout of the complex64 buffers of mixed-precision runs (stage input Y,
+// strided-pass result R).  Blocked: each (p1, p2) slice is an Nq x Nq array
+// (Nq = N rounded up to a multiple of 4) stored as 4 x 4 blocks over
+// (q1, q2), one 128-byte line each, q2 fastest inside a block:
+//   (p1 N + p2) Nq^2 + (q1 >> 2) 4 Nq + (q2 >> 2) 16 + (q1 & 3) 4 + (q2 & 3).
+// A strided tile (fixed p2, q2; rows p1, columns q1) then reads and writes
+// four lanes per line (32-byte runs at a 32-byte pitch) instead of one lane
+// per line, and a contiguous tile (fixed p1, q1; rows p2, columns q2) 32-byte
+// runs of four q2.
+// Which complex64 buffers use the blocked layout (mixed-precision runs).
+#ifndef G12_BLOCKED_Y
+#define G12_BLOCKED_Y 1
+#endif
+#ifndef G12_BLOCKED_R
+#define G12_BLOCKED_R 1
+#endif
+constexpr bool BLK_Y = G12_BLOCKED_Y != 0, BLK_R = G12_BLOCKED_R != 0;
+
+struct TileGeom {
+    uint32_t base, sm, A, B;
+    int t0, t1;  // the tile's fixed pair: (p2, q2) strided, (p1, q1) contiguous
+    __device__ __forceinline__ uint32_t nat(int m, int n) const { return base + (uint32_t)m * sm + (uint32_t)n * B; }
+    __device__ __forceinline__ uint32_t blk(int m, int n) const {
+        return base + (uint32_t)m * sm + (uint32_t)(n >> 2) * A + (uint32_t)(n & 3) * B;
     }
+    template <bool BLK>
+    __device__ __forceinline__ uint32_t at(int m, int n) const { return BLK ? blk(m, n) : nat(m, n); }
+};
+
+__device__ __forceinline__ TileGeom tile_geom(int tiling, int N, int tile, bool blocked) {
+    TileGeom g;
+    const uint32_t n = (uint32_t)N;
+    g.t0 = tile / N;
+    g.t1 = tile % N;
+    const uint32_t t0 = (uint32_t)g.t0, t1 = (uint32_t)g.t1;
+    if (!blocked) {
+        const uint32_t N2 = n * n, N3 = N2 * n;
+        if (tiling == 0) {  // strided: fixed (p2, q2) = (t0, t1); m = p1, n = q1
+            g.base = t0 * N2 + t1;
+            g.sm = N3;
+            g.B = n;
+        } else {  // contiguous: fixed (p1, q1) = (t0, t1); m = p2, n = q2
+            g.base = t0 * N3 + t1 * n;
+            g.sm = N2;
+            g.B = 1;
+        }
+        g.A = 4 * g.B;
+    } else {
+        const uint32_t nq = (n + 3) & ~3u, Q2 = nq * nq;
+        if (tiling == 0) {
+            g.base = t0 * Q2 + (t1 >> 2) * 16 + (t1 & 3);
+            g.sm = n * Q2;
+            g.A = 4 * nq;
+            g.B = 4;
+        } else {
+            g.base = t0 * n * Q2 + (t1 >> 2) * 4 * nq + (t1 & 3) * 4;
+            g.sm = Q2;
+            g.A = 16;
+            g.B = 1;
+        }
+    }
+    return g;
 }
+
 
 // Stage input of one tile into registers (TILING: 0 strided, 1 contiguous,
 // -1 given at run time).  Plain loads: nothing waits for them before the
@@ -1070,24 +1123,24 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
     // every slot holds a tile element only when the threads divide NP^2
     constexpr bool EXACT = FULL && (NP * NP) % NTHR == 0;
     const int N = FULL ? NP : t.N;
-    uint32_t base, sm, sn;
-    int t0, t1;
-    tile_geometry(TILING < 0 ? a.tiling : TILING, N, t.tile, base, sm, sn, t0, t1);
+    const int tiling = TILING < 0 ? a.tiling : TILING;
     // one branch on the storage form, outside the unrolled loads
     if (xc) {
+        const TileGeom g = tile_geom(tiling, N, t.tile, BLK_Y);
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
             double2 y = make_double2(0.0, 0.0);
-            if (EXACT || e < N * N) y = cld(X, base + (uint32_t)(e / N) * sm + (uint32_t)(e % N) * sn, true);
+            if (EXACT || e < N * N) y = cld(X, g.at<BLK_Y>(e / N, e % N), true);
             pre[i] = y;
         }
     } else {
+        const TileGeom g = tile_geom(tiling, N, t.tile, false);
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
             double2 y = make_double2(0.0, 0.0);
-            if (EXACT || e < N * N) y = gld(X, base + (uint32_t)(e / N) * sm + (uint32_t)(e % N) * sn);
+            if (EXACT || e < N * N) y = gld(X, g.nat(e / N, e % N));
             pre[i] = y;
         }
     }
@@ -1104,19 +1157,16 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
     // every slot holds a tile element only when the threads divide NP^2
     constexpr bool EXACT = FULL && (NP * NP) % NTHR == 0;
     const int N = FULL ? NP : t.N;
-    uint32_t base, sm, sn;
-    int t0, t1;
-    tile_geometry(TILING < 0 ? a.tiling : TILING, N, t.tile, base, sm, sn, t0, t1);
+    const TileGeom g = tile_geom(TILING < 0 ? a.tiling : TILING, N, t.tile, xc && BLK_Y);
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
         const int e = threadIdx.x + NTHR * i;
         fv2 y = fv2{0.f, 0.f};
         if (EXACT || e < N * N) {
-            const uint32_t off = base + (uint32_t)(e / N) * sm + (uint32_t)(e % N) * sn;
             if (xc) {
-                y = *(const GLOBAL fv2*)((const GLOBAL char*)X + (off << 3));
+                y = *(const GLOBAL fv2*)((const GLOBAL char*)X + (g.at<BLK_Y>(e / N, e % N) << 3));
             } else {
-                const double2 d = gld(X, off);
+                const double2 d = gld(X, g.nat(e / N, e % N));
                 y = fv2{(float)d.x, (float)d.y};
             }
         }
@@ -1144,7 +1194,8 @@ __device__ __forceinline__ double2 to_d2(fv2 v) { return make_double2(v.x, v.y);
 //   fused tile, fixed (p1, q1), element (m, n) = (p2, q2):
 //     g22 (e[m] - e[n]),  e[x] = i Qp[p1][q1][x] + F[p1][q1][x].
 // g11 and conj g(q1,p1) (strided) and g22 (fused) are read per element from
-// the G1 pair table, which every tile shares; the vectors are loaded by one
+// the stage G1 and its conjugate transpose (two N x N planes, contiguous along
+// a tile's columns), which every tile shares; the vectors are loaded by one
 // wave, one element per lane (N <= 64), and reach the lanes that need them
 // through LDS.  The tables depend only on the stage G1 and on the partial
 // traces of the stage input, so the strided pass can run after them.
@@ -1167,7 +1218,7 @@ __device__ __forceinline__ void src_load(const View& v, const TileRef& t, int la
         sv.f = gld(v.F, ((uint32_t)t1 * n + t0) * n + x);
         sv.w0 = gld1(v.W, (uint32_t)t0 * n + x);  // W is symmetric
         sv.w1 = gld1(v.W, (uint32_t)t1 * n + x);
-        sv.g = gld(v.G1p, 2 * ((uint32_t)t0 * n + t1));
+        sv.g = gld(v.G1, (uint32_t)t0 * n + t1);
     }
 }
 
@@ -1273,9 +1324,12 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                 lds_sync();
             }
         }
-        uint32_t base, sm, sn;
-        int t0, t1;
-        tile_geometry(TILING < 0 ? a.tiling : TILING, N, tr.tile, base, sm, sn, t0, t1);
+        // natural geometry (S, A; Y and R of FP64 runs) and the blocked one of
+        // the complex64 buffers (Y, R) of mixed-precision runs
+        const int tiling = TILING < 0 ? a.tiling : TILING;
+        const TileGeom gn = tile_geom(tiling, N, tr.tile, false);
+        const TileGeom gc = tile_geom(tiling, N, tr.tile, true);
+        const int t0 = gn.t0, t1 = gn.t1;
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
@@ -1323,9 +1377,9 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
                         rpre[u][j] = spre[u][j] = make_double2(0.0, 0.0);
                         if (inside(t, m, n)) {
-                            const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
-                            rpre[u][j] = cld(v.R, off, decltype(compact)::value);
-                            spre[u][j] = gld(v.S, off);
+                            constexpr bool C = decltype(compact)::value;
+                            rpre[u][j] = cld(v.R, (C && BLK_R) ? gc.blk(m, n) : gn.nat(m, n), C);
+                            spre[u][j] = gld(v.S, gn.nat(m, n));
                         }
                     }
                 }
@@ -1348,8 +1402,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
                         add(u, j, rpre[u][j]);
                         apre[u][j] = make_double2(0.0, 0.0);
-                        if (!ss.first && inside(t, m, n))
-                            apre[u][j] = gld(v.A, base + (uint32_t)m * sm + (uint32_t)n * sn);
+                        if (!ss.first && inside(t, m, n)) apre[u][j] = gld(v.A, gn.nat(m, n));
                     }
                 }
             }
@@ -1380,11 +1433,11 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                             cm = srcv[2][m];
                         }
                         if (inside(t, m, n)) {
-                            const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
+                            const uint32_t off = (C && BLK_R) ? gc.blk(m, n) : gn.nat(m, n);
                             double2 val = make_double2(ar[u][j], ai[u][j]);
                             if (a.sources) {
-                                const uint32_t gi = 2 * (uint32_t)(m * N + n);
-                                const double2 g11 = gld(v.G1p, gi), gt = gld(v.G1p, gi + 1);
+                                const uint32_t gi = (uint32_t)(m * N + n);
+                                const double2 g11 = gld(v.G1, gi), gt = gld(v.G1t, gi);
                                 val = cadd(val, csub(cmul(g11, csub(am, bn)), cmul(gt, csub(cn, cm))));
                             }
                             if (a.accumulate) val = cadd(cld(out, off, C), val);
@@ -1417,9 +1470,10 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                         const int m = (t / NT) * 16 + E::row_of(lane, j);
                         const double2 em = !a.sources ? double2{} : SRC_LDS ? srcv[0][m] : lane_value(ve, m);
                         if (!inside(t, m, n)) continue;
-                        const uint32_t off = base + (uint32_t)m * sm + (uint32_t)n * sn;
+                        const uint32_t off = gn.nat(m, n);                        // S, A
+                        const uint32_t offc = (C && BLK_Y) ? gc.blk(m, n) : off;  // Y
                         double2 rr = make_double2(ar[u][j], ai[u][j]);  // R + O
-                        if (a.sources) rr = cadd(rr, cmul(gld(v.G1p, 2 * (uint32_t)(m * N + n)), csub(em, en)));
+                        if (a.sources) rr = cadd(rr, cmul(gld(v.G1, (uint32_t)(m * N + n)), csub(em, en)));
                         const double2 K = cmul_mi(rr);
                         const double2 wK = cscale(K, ss.wk);
                         const double2 s = spre[u][j];
@@ -1430,12 +1484,12 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                             else
                                 gst(v.A, off, cadd(apre[u][j], wK));
                             next = cadd(s, cscale(K, ss.cn));
-                            cst(v.Y, off, next, C);
+                            cst(v.Y, offc, next, C);
                         } else {
                             const double2 A = cadd(apre[u][j], wK);
                             next = cadd(s, cdiv6(cscale(A, ss.dt)));
                             gst(v.S, off, next);
-                            if constexpr (C) cst(v.Y, off, next, true);  // next step's stage-0 input
+                            if constexpr (C) cst(v.Y, offc, next, true);  // next step's stage-0 input
                         }
                         // partial trace of the next stage input, FP64:
                         // D[X=p1][Y=q1][Z=p2] for p2 = q2
@@ -1495,13 +1549,15 @@ __global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
         const int64_t dst = ((int64_t)X * N + Y) * N + Z;
         for (int s = 0; s < 2; ++s) gstore(d.D[s], dst, gload(d.S2[s], src));
     }
-    if (a.compact)  // elements (X, p2, Y, q2): N rows of N contiguous q2
+    if (a.compact) {  // the contiguous tile (p1, q1) = (X, Y): elements (p2, q2), blocked layout
+        const TileGeom gc = tile_geom(1, N, tile, BLK_Y);
         for (int s = 0; s < 2; ++s)
             for (int p2 = 0; p2 < N; ++p2)
                 for (int q2 = threadIdx.x; q2 < N; q2 += blockDim.x) {
                     const int64_t e = (int64_t)X * N2 * N + (int64_t)p2 * N2 + (int64_t)Y * N + q2;
-                    cstore(d.Y2[s], e, gload(d.S2[s], e), true);
+                    cst(d.Y2[s], gc.at<BLK_Y>(p2, q2), gload(d.S2[s], e), true);
                 }
+    }
 }
 
 int device_cus() {
