@@ -28,10 +28,15 @@ METRIC = "G2 RK4 steps/s at N=64 (FP64 vs mixed @≤1e-6 G1 err); batched traj/s
 # HBM bytes per launch of each tile pass (read + written, averaged over the four
 # stages), measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
 # and calibrated with a plain copy kernel (bytes read = 2 x FETCH_SIZE KiB,
-# written = WRITE_SIZE KiB).
+# written = WRITE_SIZE KiB); TRAFFIC_SOURCE names the summary each comes from.
 TRAFFIC_BYTES = {("fp64", "tile_fused"): 3.01e9, ("fp64", "tile_strided"): 1.10e9,
                  ("fp32_3m", "tile_fused"): 2.50e9, ("fp32_3m", "tile_strided"): 0.557e9,
                  ("bf16x3", "tile_fused"): 2.34e9, ("bf16x3", "tile_strided"): 0.571e9}
+
+
+TRAFFIC_SOURCE = {"fp64": "profiles/tile_kernels_n64_fp64_hbm_traffic.txt",
+                  "fp32_3m": "profiles/tile_kernels_n64_mixed_hbm_traffic.txt",
+                  "bf16x3": "profiles/tile_kernels_n64_mixed_hbm_traffic.txt"}
 
 
 def workload_params(g, precision="fp64", time_steps=5000):
@@ -62,6 +67,26 @@ def synthetic_state(g, p, seed=20261203):
     return G1, g2(), g2()
 
 
+def host_cpu():
+    """CPU model, logical CPUs and physical cores of the host (Linux)."""
+    model, cores = "unknown", set()
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model == "unknown":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+                cores.add((phys, core))
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "physical_cores": len(cores) or None}
+
+
 def cpu_baseline(p, G1, Gxx, Gxy, threads):
     """One RK4 stage of the same workload through the oracle (C++, OpenMP)."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
@@ -77,7 +102,9 @@ def cpu_baseline(p, G1, Gxx, Gxy, threads):
     t1 = time.perf_counter()
     return {"value": 1.0 / (4 * (t1 - t0)), "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": f"1 of the 4 RK4 stages at N=64 (stage input, full derivative, RK accumulation), "
-                      f"{t1 - t0:.1f} s, x4 per step"}
+                      f"{t1 - t0:.1f} s, x4 per step",
+            "host": host_cpu(),
+            "threads_note": "OpenMP threads capped at 16, the host-CPU share of one GPU on the measuring box"}
 
 
 PEAK = {"fp64": (78.6, "FP64 dense"), "fp32_3m": (157.3, "FP32 dense"), "bf16x3": (2500.0, "BF16 dense")}
@@ -145,9 +172,10 @@ def production_steps(p):
     return n
 
 
-def ensemble_run(g, precision, steps, warmup):
-    """Times `steps` RK4 steps of the whole ensemble (one batch); returns
-    (ms/step, G1 of every trajectory after warmup + steps, batch facts)."""
+def ensemble_run(g, precision, steps, warmup, extra=0):
+    """Times `steps` RK4 steps of the whole ensemble (one batch), then runs
+    `extra` more untimed; returns (ms/step, G1 of every trajectory after
+    warmup + steps + extra, batch facts)."""
     draws = json.load(open(ENSEMBLE_DRAWS))["draws"]
     ps = g.ensemble_params(draws, **ensemble_protocol(precision))
     b = g.Batch(ps)
@@ -163,6 +191,8 @@ def ensemble_run(g, precision, steps, warmup):
     t1 = time.perf_counter()
     if rc != 0:
         raise SystemExit(f"ensemble: density guard tripped ({precision})")
+    if extra and b.advance(extra) != 0:
+        raise SystemExit(f"ensemble: density guard tripped ({precision})")
     G1 = [b.get_G1(i) for i in range(len(ps))]
     facts = {"buckets": b.buckets(), "memory_GB": round(b.memory_bytes() / 1e9, 2),
              "as_steps": max(switching_steps(g, p) for p in ps), "rk_steps": production_steps(ps[0]),
@@ -171,9 +201,11 @@ def ensemble_run(g, precision, steps, warmup):
     return 1e3 * (t1 - t0) / steps, G1, facts
 
 
-def ensemble_leg(g, steps, warmup, check_fp64=True):
+def ensemble_leg(g, steps, warmup, check_fp64=True, err_steps=100):
     import numpy as np
-    ms, G1m, f = ensemble_run(g, "fp32_3m", steps, warmup)
+    extra = max(0, err_steps - warmup - steps)
+    ms, G1m, f = ensemble_run(g, "fp32_3m", steps, warmup, extra)
+    progress("  ensemble FP32-3M done; FP64 reference")
     per_run = f["as_steps"] + f["rk_steps"]
     out = {"metric": "trajectories/s", "value": round(256.0 / (per_run * ms * 1e-3), 4),
            "unit": "trajectories/s", "trajectories": 256, "precision": "fp32_3m",
@@ -188,44 +220,48 @@ def ensemble_leg(g, steps, warmup, check_fp64=True):
                                                   "dt=0.02, 4000 steps, FP32-3M tile products"}
     out["achieved_tflops"] = round(out["algorithmic_flop_per_step"] / (ms * 1e-3) / 1e12, 3)
     if check_fp64:
-        ms64, G1d, _ = ensemble_run(g, "fp64", steps, warmup)
+        ms64, G1d, _ = ensemble_run(g, "fp64", steps, warmup, extra)
         errs = [float(np.abs(np.diagonal(a).imag - np.diagonal(b).imag).max()) for a, b in zip(G1m, G1d)]
         out["fp64_ms_per_step"] = round(ms64, 3)
         out["max_density_err_vs_fp64"] = max(errs)
-        out["err_horizon_steps"] = warmup + steps
+        out["median_density_err_vs_fp64"] = float(np.median(errs))
+        out["err_horizon_steps"] = warmup + steps + extra
+        out["err_note"] = ("per trajectory: largest |Im G1_ii| difference after the horizon (production steps "
+                           "from G1(0), G2 = 0), FP32-3M batch against the FP64 batch")
     return out
 
 
-def extrapolator_leg(g, total=2000, prefix=400):
-    """Learned extrapolator on the N = 48 workload over a shortened horizon
-    (quench onset at t = 8, inside it), judged by plain RK4."""
+def extrapolator_leg(g, prefix=4000):
+    """Learned collision closure on the N = 48 workload over its full horizon
+    (switching, 4000 RK4 steps, then 16001 closure steps with RK4 fallback),
+    judged by plain RK4 over the same horizon."""
     import numpy as np
     from g12kba_amd import extrapolator as xp
     if not os.path.exists(xp.DEFAULT_MODEL):
         return {"error": "trained network missing"}
-    p = xp.config5_params(time_steps=total)
-    p.quench_gate_time = p.quench_on_midpoint = 8.0
-    judge, t_judge = xp.rk4_run(p, total=total)
-    out = {"workload": f"Ns=24 Nb=2 (N=48) EHM V=U/(1+n) U=2 t1=1 t2=0.4 full quench 1.2 on 6 sites dt=0.01, "
-                       f"{total} steps (RK4 prefix {prefix}), quench onset at t=8",
-           "judge_rk4_seconds": round(t_judge, 3)}
+    p = xp.config5_params()
+    judge, t_judge = xp.rk4_run(p)
+    progress(f"  judge done ({t_judge:.1f} s); closure run")
+    out = {"workload": "Ns=24 Nb=2 (N=48) EHM V=U/(1+n) U=2 t1=1 t2=0.4 full quench 1.2 on 6 sites dt=0.01, "
+                       f"{len(judge)} production steps (RK4 prefix {prefix}), quench gate and midpoint t=50",
+           "judge_rk4_seconds": round(t_judge, 3), "rk4_steps_per_s": round((len(judge) + 304) / t_judge, 1)}
     with xp.Extrapolator() as m:
         m.load(xp.DEFAULT_MODEL)
-        for name, kw in (("strict", dict(trace_tol=1e-6, herm_tol=1e-5)),
-                         ("projected", dict(trace_tol=1e-3, herm_tol=1e-3, project=True))):
-            r = xp.extrapolated_run(p, m, prefix=prefix, total=total, **kw)
-            dens = np.abs(np.diagonal(r["traj"] - judge, axis1=1, axis2=2).imag).max()
-            out[name] = {"thresholds": [kw["trace_tol"], kw["herm_tol"]], "rk4_steps": r["rk_steps"],
-                         "network_steps": int(r["by_net"].sum()), "rejected_blocks": r["rejected_blocks"],
-                         "seconds": round(r["seconds"], 3), "speedup_vs_rk4": round(t_judge / r["seconds"], 3),
-                         "max_density_err_vs_rk4": float(dens)}
-        # network throughput alone: 64 blocks fed back from the judge's history
-        w = xp.quench_shift(p, np.arange(prefix - m.hist, prefix + 64 * m.horizon))
-        m.rollout(judge[prefix - m.hist:prefix], p.Ns, p.Nq, w, 2)
-        t0 = time.perf_counter()
-        m.rollout(judge[prefix - m.hist:prefix], p.Ns, p.Nq, w, 64, project=True)
-        out["rollout_steps_per_s"] = round(64 * m.horizon / (time.perf_counter() - t0), 1)
-    out["rk4_steps_per_s"] = round((total + 304) / t_judge, 1)
+        r = xp.closure_run(p, m, prefix=prefix)
+        tr = r["traj"]
+        post = r["by_net"][prefix:]
+        dens = np.abs(np.diagonal(tr - judge, axis1=1, axis2=2).imag).max(axis=1)
+        out["closure"] = {
+            "thresholds": {"trace": 1e-6, "hermiticity": 1e-5,
+                           "energy_drift": "0.25 |E_corr| of the state each closure segment starts from"},
+            "largest_checks_accepted": r["checks"], "rk4_steps": r["rk_steps"],
+            "network_steps": int(post.sum()), "post_prefix_coverage": round(float(post.mean()), 4),
+            "rejected_chunks": r["rejected_chunks"], "seconds": round(r["seconds"], 3),
+            "speedup_vs_rk4": round(t_judge / r["seconds"], 3),
+            "max_density_err_vs_rk4": float(dens.max()),
+            "max_dipole_err_vs_rk4": float(np.abs(g.dipole(tr) - g.dipole(judge)).max())}
+        out["closure_steps_per_s"] = (round(r["closure_steps_per_s"], 1) if r["closure_steps_per_s"] else None)
+        out["closure_phase_seconds"] = {k: round(v, 3) for k, v in r["closure_phase_seconds"].items()}
     return out
 
 
@@ -236,6 +272,7 @@ def mixed_error_run(g, rk_steps):
     import numpy as np
     out, trajs = {}, {}
     for prec in ("fp64", "fp32_3m", "bf16x3"):
+        progress(f"  error run {prec}")
         p = workload_params(g, prec, time_steps=rk_steps)
         with g.Propagator(p) as prop:
             prop.set_state(g.make_G0(p))
@@ -307,11 +344,19 @@ def roofline(N, precision, k, steps):
     peak, peak_name = PEAK[precision]
     return {"bound": "mfma", "kernel": name, "achieved": round(achieved, 3), "peak": peak, "peak_type": peak_name,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-            "traffic": TRAFFIC_BYTES.get((precision, name)),
+            "traffic": TRAFFIC_BYTES.get((precision, name)), "traffic_source": TRAFFIC_SOURCE.get(precision),
             "algorithmic_flop_per_launch": flops, "avg_launch_ms": round(avg_ms, 4),
             "breakdown_ms_per_step": {"tile_strided": round(k[0][1] / steps, 3),
                                       "tile_fused": round(k[1][1] / steps, 3),
                                       "other": round(k[2][1] / steps, 3)}}
+
+
+T_START = time.time()
+
+
+def progress(msg):
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.time() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -325,11 +370,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-trajectory ensemble leg")
-    ap.add_argument("--ensemble-steps", type=int, default=3)
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--ensemble-err-steps", type=int, default=100,
+                    help="production steps over which the ensemble's FP32-3M error against FP64 is taken")
     ap.add_argument("--no-extrapolator", action="store_true", help="skip the learned-extrapolator leg")
     ap.add_argument("--no-observables", action="store_true", help="skip the energy / checkpoint timing leg")
-    ap.add_argument("--mixed-steps", type=int, default=200,
-                    help="production steps of the mixed-precision error run (5000: the full workload)")
+    ap.add_argument("--mixed-steps", type=int, default=5000,
+                    help="production steps of the mixed-precision error run (5000: the full workload, whose "
+                         "quench acts from t = 50)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {a.gpus} is not supported "
@@ -345,6 +393,7 @@ def main():
     legs = [a.precision] + ([] if a.no_mixed else [x for x in ("fp64", "fp32_3m", "bf16x3") if x != a.precision])
     res = {}
     for prec in legs:
+        progress(f"N=64 leg {prec}: {a.warmup} warmup + {a.steps} timed steps")
         res[prec] = run_leg(g, prec, G1, Gxx, Gxy, a.steps, a.warmup)
     ms_step, G1_head, k_head = res[a.precision]
     if not np.isfinite(G1_head).all():
@@ -361,12 +410,20 @@ def main():
                                "err_horizon_steps": a.warmup + a.steps,
                                "roofline": roofline(N, prec, km, a.steps)}
     if mixed and a.mixed_steps > 0:
+        progress(f"mixed-precision error run: switching + {a.mixed_steps} production steps in three precisions")
         mixed["error_run"] = mixed_error_run(g, a.mixed_steps)
-    ensemble = None if a.no_ensemble else ensemble_leg(g, a.ensemble_steps, 1)
+    if not a.no_ensemble:
+        progress("ensemble leg: 256 trajectories, FP32-3M and FP64")
+    ensemble = None if a.no_ensemble else ensemble_leg(g, a.ensemble_steps, 1, err_steps=a.ensemble_err_steps)
+    if not a.no_extrapolator:
+        progress("extrapolator leg: RK4 judge and closure over the N=48 horizon")
     extrap = None if a.no_extrapolator else extrapolator_leg(g)
+    if not a.no_observables:
+        progress("observables leg")
     observables = None if a.no_observables else observables_leg(g, G1, Gxx, Gxy)
     cpu = None
     if not a.no_cpu:
+        progress("CPU baseline")
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         try:
             cpu = cpu_baseline(p, G1, Gxx, Gxy, threads)
@@ -381,6 +438,7 @@ def main():
                       "algorithmic_flop_per_step": 256.0 * N ** 5},
            "roofline": roofline(N, a.precision, k_head, a.steps), "mixed": mixed, "ensemble": ensemble, "extrapolator": extrap,
            "observables": observables, "cpu_baseline": cpu}
+    progress("done")
     print(json.dumps(out))
 
 

@@ -53,7 +53,7 @@ class Params(ctypes.Structure):
 
 class XPConfig(ctypes.Structure):
     _fields_ = [("hist", ctypes.c_int), ("horizon", ctypes.c_int), ("hidden", ctypes.c_int),
-                ("dt", ctypes.c_double), ("seed", ctypes.c_uint64)]
+                ("dt", ctypes.c_double), ("seed", ctypes.c_uint64), ("noise", ctypes.c_double)]
 
 
 def _load():
@@ -78,6 +78,9 @@ def _load():
         "g12_rk4": [vp, p, i64, pi64],
         "g12_rk4_advance": [vp, i64, pi64],
         "g12_rk4_record": [vp, i64, p, i64, pi64],
+        "g12_rk4_record_collision": [vp, i64, p, p, i64, pi64],
+        "g12_set_collision": [vp, p, i64],
+        "g12_set_time": [vp, d, i64],
         "g12_get_time": [vp, p, pi64],
         "g12_timing_enable": [vp, i],
         "g12_timing_get": [vp, i, pi64, p],
@@ -113,6 +116,7 @@ def _load():
         "g12_batch_rk4": [vp, ctypes.POINTER(p), i64, pi64],
         "g12_batch_rk4_advance": [vp, i64],
         "g12_batch_rk4_record": [vp, i64, ctypes.POINTER(p), i64],
+        "g12_batch_rk4_record_collision": [vp, i64, ctypes.POINTER(p), ctypes.POINTER(p), i64],
         "g12_batch_guard": [vp, ctypes.POINTER(i)],
         "g12_batch_timing_enable": [vp, i],
         "g12_batch_timing_get": [vp, i, pi64, p],
@@ -138,6 +142,7 @@ def _load():
         "g12_xp_forward": [vp, fp, i, fp],
         "g12_xp_features": [vp, p, i, i, i, p, fp, p, p],
         "g12_xp_rollout": [vp, p, i, i, i, p, p, i, i, p, p],
+        "g12_xp_rollout_hermitian": [vp, p, i, i, i, p, p, i, i, p, p],
     })
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -155,18 +160,18 @@ lib = _load()
 EXPORTED = [
     "g12_params_default", "g12_parse_input", "g12_make_g0", "g12_init_g_test", "g12_create", "g12_destroy",
     "g12_set_state", "g12_get_state", "g12_adiabatic_switching", "g12_rk4", "g12_rk4_advance",
-    "g12_get_time", "g12_rk4_record", "g12_timing_enable", "g12_timing_get", "g12_memory_bytes", "g12_synchronize",
+    "g12_get_time", "g12_rk4_record", "g12_set_time", "g12_rk4_record_collision", "g12_set_collision", "g12_timing_enable", "g12_timing_get", "g12_memory_bytes", "g12_synchronize",
     "g12_hf_hamiltonian", "g12_h1_commutator", "g12_h2_commutator", "g12_h2_commutator_noalloc", "g12_collision",
     "g12_psi_xxxx", "g12_psi_xyxy", "g12_pi", "g12_g1_step", "g12_g2_step", "g12_fermi",
     "g12_dipole", "g12_energy", "g12_state_energy", "g12_batch_state_energy", "g12_write_outputs", "g12_save_state", "g12_load_state", "g12_batch_save_state", "g12_batch_load_state", "g12_version",
     "g12_configure", "g12_batch_create", "g12_batch_destroy", "g12_batch_size",
     "g12_batch_memory_bytes", "g12_batch_set_state", "g12_batch_get_state",
-    "g12_batch_adiabatic_switching", "g12_batch_rk4", "g12_batch_rk4_advance", "g12_batch_rk4_record", "g12_batch_guard",
+    "g12_batch_adiabatic_switching", "g12_batch_rk4", "g12_batch_rk4_advance", "g12_batch_rk4_record", "g12_batch_rk4_record_collision", "g12_batch_guard",
     "g12_batch_timing_enable", "g12_batch_timing_get", "g12_batch_configure", "g12_batch_tune",
     "g12_batch_buckets", "g12_xp_default_config", "g12_xp_create", "g12_xp_destroy",
     "g12_xp_param_count", "g12_xp_get_params", "g12_xp_set_params", "g12_xp_save", "g12_xp_load",
     "g12_xp_set_data", "g12_xp_train", "g12_xp_loss_grad", "g12_xp_forward", "g12_xp_features",
-    "g12_xp_rollout",
+    "g12_xp_rollout", "g12_xp_rollout_hermitian",
 ]
 
 
@@ -514,6 +519,29 @@ class Propagator:
         rc = _check(lib.g12_rk4_record(self.h, nsteps, _p(traj), nsteps, ctypes.byref(n)), "g12_rk4_record")
         return rc, traj[: n.value]
 
+    def record_collision(self, nsteps):
+        """record() that also returns the collision term J = I + I^dagger of each
+        step run: (rc, G1 before each step, J of each step)."""
+        traj = np.zeros((max(nsteps, 1), self.N, self.N), np.complex128)
+        J = np.zeros((max(nsteps, 1), self.N, self.N), np.complex128)
+        n = ctypes.c_int64(0)
+        rc = _check(lib.g12_rk4_record_collision(self.h, nsteps, _p(traj), _p(J), nsteps, ctypes.byref(n)),
+                    "g12_rk4_record_collision")
+        ran = n.value - (1 if rc == G12_GUARD else 0)
+        return rc, traj[: n.value], J[:ran]
+
+    def set_collision(self, J):
+        """Uncorrelated (HF) contexts: the collision term at the step times of
+        the next len(J) - 1 production steps (None: off)."""
+        if J is None:
+            _check(lib.g12_set_collision(self.h, None, 0), "g12_set_collision")
+            return
+        J = _c(J)
+        _check(lib.g12_set_collision(self.h, _p(J), J.shape[0]), "g12_set_collision")
+
+    def set_time(self, t, steps):
+        _check(lib.g12_set_time(self.h, float(t), int(steps)), "g12_set_time")
+
     def time(self):
         t = ctypes.c_double(0)
         n = ctypes.c_int64(0)
@@ -639,6 +667,15 @@ class Batch:
         ptrs = (_dp * self.n)(*[_p(o[offset:]) for o in out])
         rc = _check(lib.g12_batch_rk4_record(self.h, nsteps, ptrs, nsteps), "g12_batch_rk4_record")
         return rc, out
+
+    def record_collision(self, nsteps, out, jout, offset=0):
+        """record() that also writes each correlated trajectory's collision
+        terms J = I + I^dagger to jout[i][offset:] (jout[i] None: skipped)."""
+        ptrs = (_dp * self.n)(*[None if o is None else _p(o[offset:]) for o in out])
+        jptrs = (_dp * self.n)(*[None if j is None else _p(j[offset:]) for j in jout])
+        rc = _check(lib.g12_batch_rk4_record_collision(self.h, nsteps, ptrs, jptrs, nsteps),
+                    "g12_batch_rk4_record_collision")
+        return rc
 
     def guard(self):
         f = (ctypes.c_int * self.n)()

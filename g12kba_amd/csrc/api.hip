@@ -269,6 +269,7 @@ struct Run {
     int64_t bytes = 0;
     double t = 0.0;     // production time (shared by all trajectories)
     int64_t steps = 0;  // production steps begun
+    int64_t jext_steps = -1;  // closure runs: steps the given collision term covers (-1: none given)
 };
 
 }  // namespace g12
@@ -316,6 +317,9 @@ int64_t layout(Run& r, char* base) {
         }
         d.traj_slots = r.window;
         d.traj = cv.take<double2>(d.n2 * r.window);
+        d.jtraj = corr ? cv.take<double2>(d.n2 * r.window) : nullptr;
+        d.jext = corr ? nullptr : cv.take<double2>(d.n2 * (r.window + 1));
+        d.jext_on = corr ? nullptr : cv.take<int>(1);
     }
     r.d_ds = cv.take<DevState>(r.ntraj);
     r.d_tp = cv.take<TrajParams>(r.ntraj);
@@ -689,8 +693,19 @@ void production_blocks(const g12_params& p, double t, int64_t n, int slot, Stage
 // Production RK4 of every trajectory (RK_solver.h:64-226), `nsteps` steps from
 // the run's current time.  The G1 matrix before each step goes to traj[i]
 // (capacity cap matrices, from step index `rec0`) when traj is given.
-int run_production(Run& r, int64_t nsteps, double* const* traj, int64_t cap, int64_t rec0) {
+int run_production(Run& r, int64_t nsteps, double* const* traj, int64_t cap, int64_t rec0,
+                   double* const* jtraj = nullptr) {
     const g12_params& p = r.p[0];
+    // a context given a collision term (closure runs) runs at most the steps
+    // it covers, once
+    if (r.ntraj == 1 && r.jext_steps >= 0) {
+        if (nsteps > r.jext_steps) {
+            std::fprintf(stderr, "g12kba: %lld steps asked, the given collision term covers %lld\n",
+                         (long long)nsteps, (long long)r.jext_steps);
+            return G12_E_ARG;
+        }
+        r.jext_steps = 0;
+    }
     const double t0 = r.t;
     double t = t0;
     const int64_t s0 = r.steps;
@@ -705,16 +720,23 @@ int run_production(Run& r, int64_t nsteps, double* const* traj, int64_t cap, int
             for (int i = 0; i < r.ntraj; ++i) {
                 const int f = r.h_fail[i];
                 if (!f) all_failed = false;
-                if (!traj || !traj[i]) continue;
-                // steps recorded in this window: up to and including the failing one
+                const DevState& d = r.ds[i];
+                const bool want_g = traj && traj[i], want_j = jtraj && jtraj[i] && d.jtraj;
+                if (!want_g && !want_j) continue;
+                // G1 records of this window: up to and including the failing
+                // step; collision terms: the steps that ran
                 int64_t hi = s0 + w0 + n;
                 if (f && f < hi) hi = f;
                 const int64_t lo = s0 + w0;
                 const int64_t top = std::min(hi, rec0 + cap);
                 if (top <= lo || lo < rec0) continue;
-                const DevState& d = r.ds[i];
-                HIPCHK(hipMemcpyAsync(traj[i] + (lo - rec0) * d.n2 * 2, d.traj, sizeof(double2) * d.n2 * (top - lo),
-                                      hipMemcpyDeviceToHost, r.stream));
+                if (want_g)
+                    HIPCHK(hipMemcpyAsync(traj[i] + (lo - rec0) * d.n2 * 2, d.traj,
+                                          sizeof(double2) * d.n2 * (top - lo), hipMemcpyDeviceToHost, r.stream));
+                const int64_t jtop = (f && f - 1 < top) ? f - 1 : top;
+                if (want_j && jtop > lo)
+                    HIPCHK(hipMemcpyAsync(jtraj[i] + (lo - rec0) * d.n2 * 2, d.jtraj,
+                                          sizeof(double2) * d.n2 * (jtop - lo), hipMemcpyDeviceToHost, r.stream));
             }
             HIPCHK(hipStreamSynchronize(r.stream));
             return all_failed ? 1 : 0;
@@ -967,6 +989,45 @@ int g12_rk4_record(g12_ctx* c, int64_t nsteps, double* traj, int64_t traj_cap, i
     return r.h_fail[0] ? G12_GUARD : 0;
 }
 
+int g12_rk4_record_collision(g12_ctx* c, int64_t nsteps, double* traj, double* jtraj, int64_t traj_cap,
+                              int64_t* done) {
+    if (!c || nsteps < 0) return G12_E_ARG;
+    Run& r = c->r;
+    if (!r.tp[0].corr && jtraj) return G12_E_ARG;  // no collision term to record
+    if (r.h_fail[0]) {
+        if (done) *done = 0;
+        return G12_GUARD;
+    }
+    double* tr[1] = {traj};
+    double* jt[1] = {jtraj};
+    const int64_t s0 = r.steps;
+    const int rc = run_production(r, nsteps, traj ? tr : nullptr, traj_cap, s0, jtraj ? jt : nullptr);
+    if (rc) return rc;
+    if (done) *done = iterations_begun(r, s0, nsteps);
+    return r.h_fail[0] ? G12_GUARD : 0;
+}
+
+int g12_set_collision(g12_ctx* c, const double* J, int64_t count) {
+    if (!c) return G12_E_ARG;
+    Run& r = c->r;
+    const DevState& d = r.ds[0];
+    if (r.tp[0].corr || !d.jext) return G12_E_ARG;  // only uncorrelated (HF) contexts take a given term
+    const int on = (J && count >= 2) ? 1 : 0;
+    if (J && (count < 2 || count > r.window + 1)) return G12_E_ARG;
+    if (on) HIPCHK(hipMemcpyAsync(d.jext, J, sizeof(double2) * d.n2 * count, hipMemcpyHostToDevice, r.stream));
+    HIPCHK(hipMemcpyAsync(d.jext_on, &on, sizeof(int), hipMemcpyHostToDevice, r.stream));
+    HIPCHK(hipStreamSynchronize(r.stream));
+    r.jext_steps = on ? count - 1 : -1;
+    return 0;
+}
+
+int g12_set_time(g12_ctx* c, double t, int64_t steps) {
+    if (!c || !std::isfinite(t) || steps < 0) return G12_E_ARG;
+    c->r.t = t;
+    c->r.steps = steps;
+    return 0;
+}
+
 int g12_timing_enable(g12_ctx* c, int on) { return c ? timing_enable(c->r, on) : G12_E_ARG; }
 
 int g12_timing_get(g12_ctx* c, int which, int64_t* launches, double* total_ms) {
@@ -1035,6 +1096,17 @@ int g12_batch_rk4_record(g12_batch* b, int64_t nsteps, double* const* traj, int6
     if (!b || nsteps < 0) return G12_E_ARG;
     Run& r = b->r;
     const int rc = run_production(r, nsteps, traj, traj_cap, r.steps);
+    if (rc) return rc;
+    for (int i = 0; i < r.ntraj; ++i)
+        if (r.h_fail[i]) return G12_GUARD;
+    return 0;
+}
+
+int g12_batch_rk4_record_collision(g12_batch* b, int64_t nsteps, double* const* traj, double* const* jtraj,
+                                    int64_t traj_cap) {
+    if (!b || nsteps < 0) return G12_E_ARG;
+    Run& r = b->r;
+    const int rc = run_production(r, nsteps, traj, traj_cap, r.steps, jtraj);
     if (rc) return rc;
     for (int i = 0; i < r.ntraj; ++i)
         if (r.h_fail[i]) return G12_GUARD;

@@ -212,7 +212,7 @@ __device__ inline uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
 // complex64 series, the drive from coef[s] * drive[t].
 __global__ void k_gather(FeatDims fd, double dt, const float2* __restrict__ series, const float* __restrict__ coef,
                          const float* __restrict__ drive, int64_t n_series, int T, uint64_t seed, int64_t step,
-                         int M, float* X, float* Y) {
+                         int M, float* X, float* Y, float noise) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     const uint64_t r = mix64(seed ^ mix64((uint64_t)step * 0x100000001B3ull + (uint64_t)i));
@@ -227,6 +227,19 @@ __global__ void k_gather(FeatDims fd, double dt, const float2* __restrict__ seri
     double2 ref;
     double sc;
     make_features(fd, dt, h, d, X + (int64_t)i * fd.din, &ref, &sc);
+    if (noise > 0.f) {  // Gaussian noise on the history features (Box-Muller on hashed uniforms)
+        float* x = X + (int64_t)i * fd.din;
+        uint64_t z = mix64(r ^ 0xD1B54A32D192ED03ull);
+        for (int k = 0; k < 2 * fd.hist; k += 2) {
+            z = mix64(z);
+            const double u1 = ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+            z = mix64(z);
+            const double u2 = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+            const double rad = sqrt(-2.0 * log(u1)) * noise, ang = 6.283185307179586 * u2;
+            x[k] += (float)(rad * cos(ang));
+            x[k + 1] += (float)(rad * sin(ang));
+        }
+    }
     float* y = Y + (int64_t)i * fd.dout;
     for (int j = 0; j < fd.horizon; ++j) {
         const float2 g = src[fd.hist + j];
@@ -257,9 +270,11 @@ __global__ void k_roll_in(FeatDims fd, double dt, const double2* __restrict__ ri
 }
 
 // Rollout output: predicted steps written to the ring (slots first + hist + j)
-// and to `out` (horizon x N x N), lower triangle by G1^dagger = -G1.
+// and to `out` (horizon x N x N), lower triangle by G1^dagger = -G1 (herm = 0)
+// or J^dagger = J (herm = 1).
 __global__ void k_roll_out(FeatDims fd, const float* __restrict__ Y, const double2* __restrict__ ref,
-                           const double* __restrict__ scale, double2* ring, int cap, int first, int N, double2* out) {
+                           const double* __restrict__ scale, double2* ring, int cap, int first, int N, double2* out,
+                           int herm) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int n_el = N * (N + 1) / 2;
     if (e >= n_el) return;
@@ -274,23 +289,26 @@ __global__ void k_roll_out(FeatDims fd, const float* __restrict__ Y, const doubl
     const float* y = Y + (int64_t)e * fd.dout;
     for (int j = 0; j < fd.horizon; ++j) {
         const double2 g = make_double2(r.x + s * (double)y[2 * j], r.y + s * (double)y[2 * j + 1]);
+        const double2 mirror = herm ? make_double2(g.x, -g.y) : make_double2(-g.x, g.y);
         double2* dst = ring + (int64_t)((first + fd.hist + j) % cap) * N * N;
         dst[p * N + q] = g;
-        if (p != q) dst[q * N + p] = make_double2(-g.x, g.y);
+        if (p != q) dst[q * N + p] = mirror;
         if (out) {
             double2* o = out + (int64_t)j * N * N;
             o[p * N + q] = g;
-            if (p != q) o[q * N + p] = make_double2(-g.x, g.y);
+            if (p != q) o[q * N + p] = mirror;
         }
     }
 }
 
 // Checks of the `horizon` predicted matrices: max |tr G - tr0| and the
-// anti-hermiticity error max |G + G^dagger| (= 2 max |Re G_ii| here).
+// anti-hermiticity error max |G + G^dagger| (= 2 max |Re G_ii| here), or for
+// Hermitian series (herm = 1) the hermiticity error max |J - J^dagger|
+// (= 2 max |Im J_ii|).
 __global__ void k_roll_check(const double2* __restrict__ ring, int cap, int first, int hist, int horizon, int N,
-                             double2 tr0, double* res) {
+                             double2 tr0, double* res, int herm) {
     __shared__ double dr[256], hr[256];
-    double drift = 0.0, herm = 0.0;
+    double drift = 0.0, hermerr = 0.0;
     for (int j = 0; j < horizon; ++j) {
         const double2* G = ring + (int64_t)((first + hist + j) % cap) * N * N;
         // trace: one thread sums (N <= 64), the others idle
@@ -302,10 +320,11 @@ __global__ void k_roll_check(const double2* __restrict__ ring, int cap, int firs
             }
             drift = fmax(drift, sqrt((tx - tr0.x) * (tx - tr0.x) + (ty - tr0.y) * (ty - tr0.y)));
         }
-        for (int i = threadIdx.x; i < N; i += blockDim.x) herm = fmax(herm, 2.0 * fabs(G[i * N + i].x));
+        for (int i = threadIdx.x; i < N; i += blockDim.x)
+            hermerr = fmax(hermerr, 2.0 * fabs(herm ? G[i * N + i].y : G[i * N + i].x));
     }
     dr[threadIdx.x] = drift;
-    hr[threadIdx.x] = herm;
+    hr[threadIdx.x] = hermerr;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
@@ -322,22 +341,23 @@ __global__ void k_roll_check(const double2* __restrict__ ring, int cap, int firs
 
 // Projection of the `horizon` predicted matrices onto the constraints the
 // dynamics conserve: Re G_ii = 0 (G1^dagger = -G1) and Im tr G = Im tr0, the
-// trace difference spread evenly over the diagonal.  Ring and out alike.
+// trace difference spread evenly over the diagonal; for Hermitian series
+// (herm = 1) Im J_ii = 0 and Re tr J = Re tr0.  Ring and out alike.
 __global__ void k_roll_project(double2* ring, int cap, int first, int hist, int horizon, int N, double2 tr0,
-                               double2* out) {
+                               double2* out, int herm) {
     const int j = blockIdx.x;
     if (j >= horizon) return;
     double2* G = ring + (int64_t)((first + hist + j) % cap) * N * N;
-    __shared__ double tr_im;
+    __shared__ double tr_part;
     if (threadIdx.x == 0) {
         double t = 0.0;
-        for (int i = 0; i < N; ++i) t += G[i * N + i].y;
-        tr_im = t;
+        for (int i = 0; i < N; ++i) t += herm ? G[i * N + i].x : G[i * N + i].y;
+        tr_part = t;
     }
     __syncthreads();
-    const double shift = (tr0.y - tr_im) / N;
+    const double shift = ((herm ? tr0.x : tr0.y) - tr_part) / N;
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        const double2 v = make_double2(0.0, G[i * N + i].y + shift);
+        const double2 v = herm ? make_double2(G[i * N + i].x + shift, 0.0) : make_double2(0.0, G[i * N + i].y + shift);
         G[i * N + i] = v;
         if (out) out[(int64_t)j * N * N + i * N + i] = v;
     }
@@ -491,13 +511,14 @@ int g12_xp_default_config(g12_xp_config* c) {
     c->hidden = 320;
     c->dt = 0.01;
     c->seed = 20261203ull;
+    c->noise = 0.0;
     return 0;
 }
 
 int g12_xp_create(const g12_xp_config* c, g12_xp** out) {
     if (!c || !out) return G12_E_ARG;
     if (c->hist < 2 || c->hist > 128 || c->horizon < 1 || c->horizon > 32 || c->hist + c->horizon > 160 ||
-        c->hidden < 16 || c->hidden > 1024 || !(c->dt > 0))
+        c->hidden < 16 || c->hidden > 1024 || !(c->dt > 0) || !(c->noise >= 0))
         return G12_E_SIZE;
     if (select_xp_device() < 0) return G12_E_HIP;
     g12_xp* x = new (std::nothrow) g12_xp();
@@ -647,7 +668,7 @@ int g12_xp_train(g12_xp* x, int steps, int batch, double lr, float* loss) {
     for (int i = 0; i < steps; ++i) {
         hipLaunchKernelGGL(k_gather, dim3(blocks(batch, 64)), dim3(64), 0, x->s, x->fd, x->cfg.dt, x->series, x->coef,
                            x->drive, x->n_series, x->T_len, (uint64_t)x->cfg.seed, (int64_t)x->adam_t, batch, x->X,
-                           x->T);
+                           x->T, (float)x->cfg.noise);
         forward(x, batch);
         backward(x, batch);
         x->adam_t += 1;
@@ -699,8 +720,8 @@ int g12_xp_forward(g12_xp* x, const float* X, int M, float* Y) {
 // of the network's own output.  With `project`, each block is then projected
 // (Re G_ii = 0, Im tr G = Im tr_ref) before it is fed back, so the checks
 // measure the drift a single block produces.
-int g12_xp_rollout(g12_xp* x, const double* hist, int N, int Ns, int Nq, const double* w, const double* tr_ref,
-                   int project, int nblocks, double* out, double* checks) {
+static int rollout(g12_xp* x, const double* hist, int N, int Ns, int Nq, const double* w, const double* tr_ref,
+                   int project, int nblocks, double* out, double* checks, int herm) {
     if (!x || !hist || !w || !out || N < 1 || N > 64 || Ns < 1 || nblocks < 1) return G12_E_ARG;
     const int H = x->fd.hist, F = x->fd.horizon;
     const int n_el = N * (N + 1) / 2;
@@ -739,11 +760,12 @@ int g12_xp_rollout(g12_xp* x, const double* hist, int N, int Ns, int Nq, const d
                            N, Ns, Nq, x->w, x->X, x->ref, x->scale);
         forward(x, n_el);
         hipLaunchKernelGGL(k_roll_out, dim3(blocks(n_el, 64)), dim3(64), 0, x->s, x->fd, x->Y, x->ref, x->scale,
-                           x->ring, cap, first, N, x->rout);
-        hipLaunchKernelGGL(k_roll_check, dim3(1), dim3(256), 0, x->s, x->ring, cap, first, H, F, N, tr0, x->chk);
+                           x->ring, cap, first, N, x->rout, herm);
+        hipLaunchKernelGGL(k_roll_check, dim3(1), dim3(256), 0, x->s, x->ring, cap, first, H, F, N, tr0, x->chk,
+                           herm);
         if (project)
             hipLaunchKernelGGL(k_roll_project, dim3(F), dim3(64), 0, x->s, x->ring, cap, first, H, F, N, tr0,
-                               x->rout);
+                               x->rout, herm);
         XCHK(hipMemcpyAsync(out + (int64_t)b * F * n2 * 2, x->rout, sizeof(double2) * F * n2, hipMemcpyDeviceToHost,
                             x->s));
         if (checks) XCHK(hipMemcpyAsync(checks + 2 * b, x->chk, sizeof(double) * 2, hipMemcpyDeviceToHost, x->s));
@@ -753,6 +775,22 @@ int g12_xp_rollout(g12_xp* x, const double* hist, int N, int Ns, int Nq, const d
     XCHK(hipGetLastError());
     return 0;
 }
+
+}  // extern "C"
+
+// Rollout of G1 (anti-Hermitian) or of a Hermitian series (the collision term).
+extern "C" int g12_xp_rollout(g12_xp* x, const double* hist, int N, int Ns, int Nq, const double* w,
+                              const double* tr_ref, int project, int nblocks, double* out, double* checks) {
+    return rollout(x, hist, N, Ns, Nq, w, tr_ref, project, nblocks, out, checks, 0);
+}
+
+extern "C" int g12_xp_rollout_hermitian(g12_xp* x, const double* hist, int N, int Ns, int Nq, const double* w,
+                                        const double* tr_ref, int project, int nblocks, double* out,
+                                        double* checks) {
+    return rollout(x, hist, N, Ns, Nq, w, tr_ref, project, nblocks, out, checks, 1);
+}
+
+extern "C" {
 
 // Features of the rollout input for the last `hist` matrices (tests): X
 // (n_el x din, upper-triangle order), ref and scale per element.

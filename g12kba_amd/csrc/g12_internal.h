@@ -79,6 +79,14 @@ struct DevState {
     int* fail;                         // 0 while running, step+1 after the guard trips
     double2* traj;                     // device trajectory ring (traj_slots x N^2) or null
     int traj_slots;
+    // collision term J = I + I^dagger of each production step's first stage,
+    // ring like traj (correlated trajectories), or null
+    double2* jtraj;
+    // a given collision term at the step times of the current window
+    // ((traj_slots + 1) x N^2) that an uncorrelated (HF) trajectory adds to its
+    // G1 equation when jext_on: the learned closure's G1 propagation
+    double2* jext;
+    int* jext_on;
 };
 
 // A set of trajectories advanced together: device arrays of states and

@@ -335,7 +335,19 @@ __global__ __launch_bounds__(256) void k_g1_rhs(BatchDev b, StageRef r) {
         if (grp == 0 && Y < N) {
             const int64_t e = (int64_t)X * N + Y;
             double2 v = cadd(cadd(part[0][col], part[1][col]), cadd(part[2][col], part[3][col]));
-            if (tp.corr) v = cadd(cadd(v, d.I[e]), cconj(d.I[(int64_t)Y * N + X]));
+            if (tp.corr) {
+                const double2 iyx = cconj(d.I[(int64_t)Y * N + X]);
+                // the collision term J = I + I^dagger of a production step's
+                // first stage (whose input is the state) goes to the record ring
+                if (ss.first && !ss.as_phase && d.jtraj) d.jtraj[(int64_t)ss.slot * N * N + e] = cadd(d.I[e], iyx);
+                v = cadd(cadd(v, d.I[e]), iyx);
+            } else if (d.jext && *d.jext_on) {
+                // a given collision term at the step's two ends: stage 0 at t,
+                // stages 1 and 2 at t + dt/2 (their mean), stage 3 at t + dt
+                const double2* J = d.jext + (int64_t)ss.slot * N * N;
+                const double2 a = J[e], b = J[(int64_t)N * N + e];
+                v = cadd(v, ss.first ? a : ss.last ? b : cscale(cadd(a, b), 0.5));
+            }
             const double2 K = cmul_mi(v);
             const double2 wK = cscale(K, ss.wk);
             if (ss.first) {

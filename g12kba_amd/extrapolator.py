@@ -1,14 +1,20 @@
-"""Learned G1 extrapolator: network binding, training on trajectories the method
-generates, and extrapolated runs with plain RK4 as fallback and judge.
+"""Learned collision closure: network binding, training on trajectories the
+method generates, and closure runs with plain RK4 as fallback and judge.
 
-The network (g12kba_amd/csrc/extrap.hip) sees one upper-triangle element of G1
-at a time: `hist` past steps and the quench drive give the next `horizon`
-steps.  An extrapolated run propagates a prefix with RK4, then rolls the
-network out block by block; a block whose trace drift or anti-hermiticity
-error exceeds its threshold is rejected and RK4 resumes from its last full
-(G1, G2) state, which then also refills the history.  A plain RK4 run of the
-whole horizon judges the result.  The reference has no counterpart (its README
-names extrapolation as future work).
+G2 enters the G1 equation of motion only through the collision term:
+dG1/dt = -i([h[G1], G1] + J), J = I + I^dagger (G1_G2_t_step.h:24-47).  The
+network (g12kba_amd/csrc/extrap.hip) sees one upper-triangle element of J at a
+time: `hist` past steps and the quench drive give the next `horizon` steps.
+A closure run propagates a prefix with the full RK4 (recording J), then
+rolls the network out on J and propagates G1 with RK4 on the exact
+Hartree-Fock term plus the predicted J (an HF context given J): O(N^3) per step
+instead of O(N^5).  The predicted J is Hermitian with zero trace by
+construction, so G1 keeps its trace and anti-hermiticity; the checks (trace
+drift, anti-hermiticity, energy drift) run on the propagated G1, and a chunk
+that fails one is replaced by plain RK4 from its last full (G1, G2) state,
+which then refills the J history.  A plain RK4 run of the whole horizon
+judges the result.  The reference has no counterpart (its README names
+extrapolation as future work).
 
 Command line:  python -m g12kba_amd.extrapolator train --out FILE [options]
 """
@@ -20,12 +26,12 @@ import time
 
 import numpy as np
 
-from . import (Batch, G12Error, Propagator, XPConfig, _check, _p, draw_ensemble, ensemble_params, fermi, lib,
+from . import (Batch, G12Error, Params, Propagator, XPConfig, _check, _p, draw_ensemble, ensemble_params, fermi, lib,
                make_G0, make_params)
 
 _fp = ctypes.POINTER(ctypes.c_float)
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_MODEL = os.path.join(HERE, "models", "xp_n48.bin")
+DEFAULT_MODEL = os.path.join(HERE, "models", "xp_closure_n48.bin")
 
 
 def _f(a):
@@ -33,10 +39,10 @@ def _f(a):
 
 
 class Extrapolator:
-    def __init__(self, hist=64, horizon=16, hidden=320, dt=0.01, seed=20261203):
+    def __init__(self, hist=64, horizon=16, hidden=320, dt=0.01, seed=20261203, noise=0.0):
         c = XPConfig()
         lib.g12_xp_default_config(ctypes.byref(c))
-        c.hist, c.horizon, c.hidden, c.dt, c.seed = hist, horizon, hidden, dt, seed
+        c.hist, c.horizon, c.hidden, c.dt, c.seed, c.noise = hist, horizon, hidden, dt, seed, noise
         h = ctypes.c_void_p()
         _check(lib.g12_xp_create(ctypes.byref(c), ctypes.byref(h)), "g12_xp_create")
         self.h = h
@@ -124,6 +130,23 @@ class Extrapolator:
         _check(lib.g12_xp_features(self.h, _p(hist), N, Ns, Nq, _p(w), _f(X), _p(ref), _p(sc)), "g12_xp_features")
         return X, ref, sc
 
+    def rollout_hermitian(self, hist, Ns, Nq, w, nblocks, tr_ref=0.0, project=True):
+        """rollout() for a Hermitian series (the collision term J): lower
+        triangle J^dagger = J; with project, Im J_ii = 0 and Re tr J = tr_ref
+        in each block.  checks[b] = (max |tr J - tr_ref|, max |J - J^dagger|)
+        of the network's own output for block b."""
+        hist = np.ascontiguousarray(np.asarray(hist)[-self.hist:], dtype=np.complex128)
+        N = hist.shape[1]
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        if w.size < self.hist + nblocks * self.horizon:
+            raise ValueError("drive too short")
+        out = np.zeros((nblocks * self.horizon, N, N), np.complex128)
+        chk = np.zeros((nblocks, 2))
+        tr = np.array([complex(tr_ref).real, complex(tr_ref).imag])
+        _check(lib.g12_xp_rollout_hermitian(self.h, _p(hist), N, Ns, Nq, _p(w), _p(tr), int(bool(project)), nblocks,
+                                            _p(out), _p(chk)), "g12_xp_rollout_hermitian")
+        return out, chk
+
     def rollout(self, hist, Ns, Nq, w, nblocks, tr_ref=None, project=False):
         """Predicts nblocks * horizon matrices after `hist` (oldest first).
         w: quench shift for hist + nblocks * horizon steps; tr_ref: conserved
@@ -186,9 +209,9 @@ def training_protocol(dt=0.01, steps=1600, gate=6.0, precision="fp64"):
 
 
 def series_from_trajectory(traj, p, rng, per_traj):
-    """Upper-triangle element series of one trajectory (all diagonal elements
-    and a random subset of the others, per_traj in total) and their drive
-    coefficients strength * (m_p - m_q)."""
+    """Upper-triangle element series of one trajectory of matrices (all
+    diagonal elements and a random subset of the others, per_traj in total)
+    and their drive coefficients strength * (m_p - m_q)."""
     N = traj.shape[1]
     pairs = [(a, b) for a in range(N) for b in range(a + 1, N)]
     k = max(0, min(len(pairs), per_traj - N))
@@ -201,17 +224,17 @@ def series_from_trajectory(traj, p, rng, per_traj):
 
 def build_training_set(n_traj=256, seed=20261204, max_n=48, steps=1600, per_traj=96, holdout=0, log=print):
     """Runs n_traj trajectories of the ensemble distribution (N <= max_n) as one
-    batch (FP64: switching, then `steps` production steps) and returns
-    (series, coef, drive, held_out) for Extrapolator.set_data, the last
-    `holdout` trajectories being left out of the series and returned as
-    [(params, trajectory)] for evaluation."""
+    batch (FP64: switching, then `steps` production steps recording the
+    collision term J of every step) and returns (series, coef, drive, held_out)
+    for Extrapolator.set_data: element series of J, the last `holdout`
+    trajectories being left out of the series and returned as [(params, J)]
+    for evaluation.  Trajectories whose guard tripped are left out."""
     draws = draw_ensemble(seed, n_traj, max_n=max_n)
     proto = training_protocol(steps=steps)
     ps = ensemble_params(draws, **proto)
     t0 = time.time()
     T = steps
-    trajs = [np.zeros((T, p.Ns * p.Nb, p.Ns * p.Nb), np.complex128) for p in ps]
-    rc = 0
+    jt = [np.zeros((T, p.Ns * p.Nb, p.Ns * p.Nb), np.complex128) for p in ps]
     with Batch(ps) as b:
         for i, p in enumerate(ps):
             b.set_state(i, make_G0(p))
@@ -220,60 +243,91 @@ def build_training_set(n_traj=256, seed=20261204, max_n=48, steps=1600, per_traj
         chunk = 200
         for k in range(0, T, chunk):  # in chunks, reporting progress
             n = min(chunk, T - k)
-            rc = max(rc, b.record(n, trajs, k)[0])
+            b.record_collision(n, [None] * len(ps), [j[k:k + n] for j in jt])
             log(f"recorded {k + n}/{T} steps ({time.time() - t0:.1f} s)")
         fail = b.guard()
-    trajs = [tr[:f] if f else tr for tr, f in zip(trajs, fail)]
     log(f"training trajectories: {n_traj} x {T} steps in {time.time() - t0:.1f} s (guard trips: {sum(map(bool, fail))})")
     rng = np.random.default_rng(seed)
     S, C, held = [], [], []
-    for i, (p, tr) in enumerate(zip(ps, trajs)):
-        if len(tr) < T or not np.isfinite(tr).all():  # guard tripped: left out
+    for i, (p, j) in enumerate(zip(ps, jt)):
+        if fail[i] or not np.isfinite(j).all():  # guard tripped: left out
             continue
         if i >= len(ps) - holdout:
-            held.append((p, tr))
+            held.append((p, j))
             continue
-        s, c = series_from_trajectory(tr[:T], p, rng, per_traj)
-        S.append(s)
-        C.append(c)
+        s_, c_ = series_from_trajectory(j, p, rng, per_traj)
+        S.append(s_)
+        C.append(c_)
     drive = quench_shift(ps[0], np.arange(T)) / ps[0].quench_strength  # shared profile
     return np.concatenate(S), np.concatenate(C), drive.astype(np.float32), held
 
 
 def evaluate_blocks(model, held, windows=12, seed=7):
-    """One-block prediction error from true histories on held-out trajectories,
-    against holding the last value and linear extrapolation.  Returns per
-    window the largest absolute element error of each predictor."""
+    """One-block predictions of J from true histories on held-out
+    trajectories, against holding the last value and linear extrapolation.
+    Returns per window (N, step, error of the network, of holding, of linear
+    extrapolation), each the largest absolute element error over the block."""
     rng = np.random.default_rng(seed)
     H, F = model.hist, model.horizon
     rows = []
-    for p, tr in held:
-        w_all = quench_shift(p, np.arange(len(tr)))
-        for k in rng.integers(H, len(tr) - F, size=windows):
-            truth = tr[k:k + F]
-            pred, _ = model.rollout(tr[k - H:k], p.Ns, p.Nq, w_all[k - H:k + F], 1)
-            hold = np.repeat(tr[k - 1][None], F, 0)
-            slope = tr[k - 1] - tr[k - 2]
-            lin = tr[k - 1][None] + slope[None] * np.arange(1, F + 1)[:, None, None]
+    for p, J in held:
+        w_all = quench_shift(p, np.arange(len(J)))
+        for k in rng.integers(H, len(J) - F, size=windows):
+            truth = J[k:k + F]
+            pred, _ = model.rollout_hermitian(J[k - H:k], p.Ns, p.Nq, w_all[k - H:k + F], 1, project=False)
+            hold = np.repeat(J[k - 1][None], F, 0)
+            lin = J[k - 1][None] + (J[k - 1] - J[k - 2])[None] * np.arange(1, F + 1)[:, None, None]
             rows.append((p.Ns * p.Nb, int(k), float(np.abs(pred - truth).max()), float(np.abs(hold - truth).max()),
                          float(np.abs(lin - truth).max())))
     return rows
 
 
 # ---------------------------------------------------------------------------
-# extrapolated runs
+# closure runs
 
 
-def extrapolated_run(p, model, prefix=4000, total=None, G1_0=None, trace_tol=1e-6, herm_tol=1e-5, resume=64,
-                     blocks_per_call=8, project=False, log=None):
-    """RK4 for `prefix` production steps, then network blocks with RK4 fallback
-    up to `total` steps (default: the production length of p).  A block is
-    rejected when the network's trace drift exceeds trace_tol or its
-    anti-hermiticity error herm_tol; with `project` accepted blocks are
-    projected onto the conserved trace and G1^dagger = -G1 (the checks then
-    see each block's own drift).  Returns a dict with the G1 trajectory (total
-    matrices: G1 before each step), which steps the network produced, the RK4
-    steps spent and the wall time."""
+def hf_params(p):
+    """The uncorrelated (HF = 1) copy of parameters p that carries the
+    closure's G1 propagation."""
+    q = Params.from_buffer_copy(p)
+    q.HF = 1
+    return q
+
+
+def hf_energy(p, G1, w):
+    """One-body (with the quench shift w) + Hartree + exchange energy of G1."""
+    from . import energy
+    return float(energy(p, G1, w=w)[:3].sum())
+
+
+def quench_work(p, G1s, ws):
+    """Work of the time-dependent quench shift along a G1 sequence: the sum of
+    (w_{k+1} - w_k) times the mean quenched-site occupation of the two steps
+    (trapezoid), spin-summed as in the one-body energy."""
+    m = site_mask(p).astype(bool)
+    occ = np.array([2 * np.diagonal(G).imag[m].sum() for G in G1s])
+    return float(np.sum(np.diff(ws) * 0.5 * (occ[1:] + occ[:-1])))
+
+
+def closure_run(p, model, prefix=4000, total=None, G1_0=None, trace_tol=1e-6, herm_tol=1e-5, energy_tol=None,
+                chunk_blocks=8, resume=64, log=None, collision="network"):
+    """The full method (adiabatic switching, then `prefix` production steps,
+    recording G1 and J), then the closure up to `total` steps (default: the
+    production length of p): the network rolls J out `chunk_blocks` blocks at
+    a time and an HF context propagates G1 with it.  Each chunk's G1 is
+    checked: trace drift against the prefix's trace (trace_tol), largest
+    |G1 + G1^dagger| (herm_tol), and the energy drift: the HF energy net of
+    the quench's work since the closure started, E'(t) = E_HF(t) - W, may move
+    from its starting value by at most energy_tol (default: a quarter of the
+    correlation energy |E_corr| of the full state the closure started from,
+    the energy the HF part can take from or give to the correlations).  A
+    failing chunk is discarded; the full method resumes from its last state and
+    runs `resume` steps past the chunk's start, refilling the J history, and
+    the closure continues from there.  `collision` = "hold" (J frozen at its
+    last value) or "zero" (the Hartree-Fock dynamics without J) replaces the
+    network by the closures it is measured against.  Returns a dict with the
+    G1 trajectory (G1 before each step), which steps the closure produced, the
+    full-method steps spent, the checks' largest values and the wall time."""
     if total is None:
         total, t = 0, 0.0
         while t < p.time_steps * p.dt:
@@ -281,52 +335,94 @@ def extrapolated_run(p, model, prefix=4000, total=None, G1_0=None, trace_tol=1e-
             total += 1
     N = p.Ns * p.Nb
     H, F = model.hist, model.horizon
-    w_all = quench_shift(p, np.arange(total + H + F * blocks_per_call + 1))
+    w_all = quench_shift(p, np.arange(total + H + F * chunk_blocks + 2))
     traj = np.zeros((total, N, N), np.complex128)
+    J = np.zeros((total, N, N), np.complex128)
     by_net = np.zeros(total, bool)
+    worst = dict(trace=0.0, herm=0.0, energy=0.0)
     t_start = time.time()
-    with Propagator(p) as prop:
-        prop.set_state(make_G0(p) if G1_0 is None else G1_0)
-        prop.adiabatic_switching()
-        rc, rec = prop.record(prefix)
+    with Propagator(p) as full, Propagator(hf_params(p)) as hf:
+        full.set_state(make_G0(p) if G1_0 is None else G1_0)
+        full.adiabatic_switching()
+        rc, g1, jj = full.record_collision(prefix)
         if rc:
             raise G12Error("density guard tripped in the RK4 prefix")
-        traj[:prefix] = rec
-        t_r = prefix      # steps the RK4 state has completed (its G1 is G1(t_{t_r}))
-        k = prefix        # next step to fill; the history is traj[k - H:k]
+        traj[:prefix], J[:prefix] = g1, jj
+        t_r = prefix  # steps the full state has completed
+        k = prefix    # next step to fill
         rk_steps = prefix
-        tr_true = np.trace(traj[prefix - 1])  # conserved by RK4 to round-off
+        tr0 = np.trace(traj[prefix - 1])
         rejected = accepted = 0
+
+        start = {}
+
+        def restart_hf():
+            t, n = full.time()
+            G = full.get_G1()
+            hf.set_state(G)
+            hf.set_time(t, n)
+            start["E"] = hf_energy(p, G, w_all[n])  # E' at the closure's start
+            start["W"] = 0.0                        # quench work since then
+            start["tol"] = energy_tol if energy_tol is not None else 0.25 * abs(full.energy()[3])
+
+        restart_hf()
+        closure_time = 0.0  # wall time of accepted chunks (rollout, propagation, checks)
+        phase = dict(rollout=0.0, propagate=0.0, checks=0.0)
         while k < total:
-            nb = min(blocks_per_call, (total - k + F - 1) // F)
-            pred, chk = model.rollout(traj[k - H:k], p.Ns, p.Nq, w_all[k - H:k + nb * F], nb, tr_ref=tr_true,
-                                      project=project)
-            ok = 0
-            while ok < nb and chk[ok, 0] <= trace_tol and chk[ok, 1] <= herm_tol:
-                ok += 1
-            n_ok = min(ok * F, total - k)
-            traj[k:k + n_ok] = pred[:n_ok]
-            by_net[k:k + n_ok] = True
-            accepted += ok
-            k += n_ok
-            if ok == nb:
+            t_chunk = time.time()
+            n = min(chunk_blocks * F, total - k)  # steps of this chunk
+            nb = (n + 1 + F - 1) // F                # J is needed at n + 1 step times
+            if collision == "network":
+                pred, _ = model.rollout_hermitian(J[k - H:k], p.Ns, p.Nq, w_all[k - H:k + nb * F], nb)
+            elif collision == "hold":
+                pred = np.repeat(J[k - 1][None], nb * F, 0)
+            else:
+                pred = np.zeros((nb * F, N, N), np.complex128)
+            t_a = time.time()
+            hf.set_collision(pred[:n + 1])
+            rc, g1 = hf.record(n)
+            t_b = time.time()
+            tr_d = herm = dE = float("inf")
+            if rc == 0 and len(g1) == n:  # else the density guard tripped: rejected
+                end = hf.get_G1()
+                seq = np.concatenate([g1, end[None]])
+                tr_d = float(np.abs(np.trace(seq, axis1=1, axis2=2) - tr0).max())
+                herm = float(np.abs(seq + np.conj(np.transpose(seq, (0, 2, 1)))).max())
+                ws = w_all[k:k + n + 1]
+                work = start["W"] + quench_work(p, seq, ws)
+                dE = hf_energy(p, end, ws[-1]) - work - start["E"]
+            ok = tr_d <= trace_tol and herm <= herm_tol and abs(dE) <= start["tol"]
+            phase["rollout"] += t_a - t_chunk
+            phase["propagate"] += t_b - t_a
+            phase["checks"] += time.time() - t_b
+            if ok:
+                closure_time += time.time() - t_chunk
+                start["W"] = work
+                traj[k:k + n], J[k:k + n] = g1, pred[:n]
+                by_net[k:k + n] = True
+                worst = dict(trace=max(worst["trace"], tr_d), herm=max(worst["herm"], herm),
+                             energy=max(worst["energy"], abs(dE)))
+                accepted += 1
+                k += n
                 continue
             rejected += 1
             if log:
-                log(f"  block at step {k} rejected: trace drift {chk[ok, 0]:.2e}, herm {chk[ok, 1]:.2e}")
-            # RK4 resumes from its own state and refills `resume` steps past the
-            # rejected block's start (the next history is then all RK4)
+                log(f"  chunk at step {k} rejected: trace {tr_d:.2e}, herm {herm:.2e}, energy {dE:.2e}")
+            # the full method resumes from its own state and runs `resume` steps
+            # past the rejected chunk's start (its J history is then all exact)
             stop = min(total, k + resume)
-            n = stop - t_r
-            rc, rec = prop.record(n)
+            rc, g1, jj = full.record_collision(stop - t_r)
             if rc:
-                raise G12Error("density guard tripped during RK4 fallback")
-            traj[t_r:stop] = rec
+                raise G12Error("density guard tripped during the RK4 fallback")
+            traj[t_r:stop], J[t_r:stop] = g1, jj
             by_net[t_r:stop] = False
-            rk_steps += n
+            rk_steps += stop - t_r
             t_r = k = stop
-    return dict(traj=traj, by_net=by_net, rk_steps=rk_steps, accepted_blocks=accepted, rejected_blocks=rejected,
-                seconds=time.time() - t_start, prefix=prefix, total=total)
+            restart_hf()
+    n_net = int(by_net.sum())
+    return dict(traj=traj, by_net=by_net, rk_steps=rk_steps, accepted_chunks=accepted, rejected_chunks=rejected,
+                checks=worst, seconds=time.time() - t_start, prefix=prefix, total=total,
+                closure_steps_per_s=(n_net / closure_time if closure_time > 0 else None), closure_phase_seconds=phase)
 
 
 def rk4_run(p, total=None, G1_0=None):
@@ -350,10 +446,38 @@ def config5_params(time_steps=20000, precision="fp64"):
                        AS_rate=0.5, AS_midpoint=10.0, precision=precision)
 
 
+def train_on(S, C, D, steps, batch, lr, log=None, seed=20261203, noise=0.0):
+    """A fresh network trained on series S (coefficients C, drive D): Adam with
+    the learning rate lowered tenfold for the last quarter, history features
+    perturbed by Gaussian noise of standard deviation `noise`.  Returns
+    (model, mean loss per 250 steps)."""
+    m = Extrapolator(seed=seed, noise=noise)
+    m.set_data(S, C, D)
+    losses = []
+    t0 = time.time()
+    chunk = 250
+    for i in range(0, steps, chunk):
+        n = min(chunk, steps - i)
+        ls = m.train(n, batch, lr * (0.1 if i >= 0.75 * steps else 1.0))
+        losses.append(float(np.mean(ls)))
+        if log:
+            log(f"step {i + n}: loss {losses[-1]:.3e} ({time.time() - t0:.1f} s)")
+    return m, losses
+
+
+def summarise(rows):
+    r = np.array([x[2:] for x in rows])
+    return {"windows": len(rows), "median_err": {"network": float(np.median(r[:, 0])),
+                                                 "hold": float(np.median(r[:, 1])),
+                                                 "linear": float(np.median(r[:, 2]))},
+            "network_better_than_linear": float(np.mean(r[:, 0] < r[:, 2])),
+            "network_better_than_hold": float(np.mean(r[:, 0] < r[:, 1]))}
+
+
 def _main():
     ap = argparse.ArgumentParser(prog="python -m g12kba_amd.extrapolator")
     sub = ap.add_subparsers(dest="cmd", required=True)
-    tr = sub.add_parser("train", help="generate trajectories with the method and train the network")
+    tr = sub.add_parser("train", help="generate trajectories with the method and train the network on J")
     tr.add_argument("--out", default=DEFAULT_MODEL)
     tr.add_argument("--trajectories", type=int, default=256)
     tr.add_argument("--steps", type=int, default=1600)
@@ -361,41 +485,41 @@ def _main():
     tr.add_argument("--train-steps", type=int, default=4000)
     tr.add_argument("--batch", type=int, default=4096)
     tr.add_argument("--lr", type=float, default=1e-3)
+    tr.add_argument("--noise", type=float, default=0.0, help="history-feature noise during training")
     tr.add_argument("--holdout", type=int, default=16, help="trajectories kept out of training for evaluation")
+    tr.add_argument("--learning-curve", default="", help="comma-separated training-set sizes (trajectories) "
+                                                         "to train and evaluate on the same held-out set")
     tr.add_argument("--log", default=None, help="JSON summary file")
     a = ap.parse_args()
     if a.cmd == "train":
+        say = lambda s: print(s, flush=True)  # noqa: E731
         S, C, D, held = build_training_set(a.trajectories, steps=a.steps, per_traj=a.per_traj, holdout=a.holdout,
-                                           log=lambda s: print(s, flush=True))
-        print(f"series: {S.shape}, drive max {D.max():.3f}", flush=True)
-        with Extrapolator() as m:
-            m.set_data(S, C, D)
-            losses = []
-            t0 = time.time()
-            chunk = 250
-            for i in range(0, a.train_steps, chunk):
-                n = min(chunk, a.train_steps - i)
-                lr = a.lr * (0.1 if i >= 0.75 * a.train_steps else 1.0)
-                ls = m.train(n, a.batch, lr)
-                losses.append(float(np.mean(ls)))
-                print(f"step {i + n}: loss {losses[-1]:.3e} ({time.time() - t0:.1f} s)", flush=True)
-            os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
-            m.save(a.out)
-            rows = evaluate_blocks(m, held) if held else []
-        ev = {}
-        if rows:
-            r = np.array([x[2:] for x in rows])
-            ev = {"windows": len(rows), "median_err": {"network": float(np.median(r[:, 0])),
-                                                       "hold": float(np.median(r[:, 1])),
-                                                       "linear": float(np.median(r[:, 2]))},
-                  "network_better_than_linear": float(np.mean(r[:, 0] < r[:, 2])),
-                  "network_better_than_hold": float(np.mean(r[:, 0] < r[:, 1]))}
-            print(f"held-out blocks: {json.dumps(ev)}", flush=True)
+                                           log=say)
+        say(f"series: {S.shape}, drive max {D.max():.3f}")
+        curve = []
+        if a.learning_curve:
+            per = S.shape[0] / max(1, a.trajectories - a.holdout)
+            for n in [int(x) for x in a.learning_curve.split(",") if x]:
+                k = int(round(n * per))
+                m, _ = train_on(S[:k], C[:k], D, a.train_steps, a.batch, a.lr, noise=a.noise)
+                ev = summarise(evaluate_blocks(m, held))
+                m.close()
+                curve.append({"trajectories": n, "series": k, **ev})
+                say(f"learning curve: {n} trajectories -> {json.dumps(ev)}")
+        m, losses = train_on(S, C, D, a.train_steps, a.batch, a.lr, log=say, noise=a.noise)
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        m.save(a.out)
+        rows = evaluate_blocks(m, held) if held else []
+        m.close()
+        ev = summarise(rows) if rows else {}
+        if ev:
+            say(f"held-out blocks: {json.dumps(ev)}")
         if a.log:
             with open(a.log, "w") as f:
-                json.dump({"series": list(S.shape), "losses": losses, "train_steps": a.train_steps,
-                           "batch": a.batch, "lr": a.lr, "trajectories": a.trajectories, "steps": a.steps,
-                           "holdout": a.holdout, "evaluation": ev, "windows": rows}, f)
+                json.dump({"target": "collision term J = I + I^dagger", "series": list(S.shape), "losses": losses,
+                           "train_steps": a.train_steps, "batch": a.batch, "lr": a.lr, "noise": a.noise,
+                           "trajectories": a.trajectories, "steps": a.steps, "holdout": a.holdout,
+                           "evaluation": ev, "learning_curve": curve, "windows": rows}, f)
 
 
 if __name__ == "__main__":

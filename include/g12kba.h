@@ -150,6 +150,31 @@ int g12_rk4_record(g12_ctx* ctx, int64_t n, double* traj, int64_t traj_cap, int6
 /* Current production time and step count of the context. */
 int g12_get_time(g12_ctx* ctx, double* t, int64_t* steps);
 
+/* Sets the production clock (time and step count) from which the next
+ * production step runs: the quench protocol of RK_solver.h:141-151 is
+ * evaluated at that time.  No reference counterpart (starts a run mid-way,
+ * e.g. the G1 propagation of the learned closure at the end of an RK4
+ * prefix). */
+int g12_set_time(g12_ctx* ctx, double t, int64_t steps);
+
+/* g12_rk4_record that also writes, for every step run, the collision term
+ * J = I + I^dagger of the step's first stage (the term G1_step adds to
+ * [h, G1], G1_G2_t_step.h:41) to jtraj (capacity traj_cap N x N matrices;
+ * NULL: not recorded; G12_E_ARG for an uncorrelated context, which has none).
+ * No reference counterpart: training data of the learned closure. */
+int g12_rk4_record_collision(g12_ctx* ctx, int64_t n, double* traj, double* jtraj, int64_t traj_cap,
+                             int64_t* steps_done);
+
+/* The learned closure's G1 propagation: an uncorrelated context (HF = 1,
+ * G2 frozen) adds a given collision term to its G1 equation,
+ * K = -i([h, G1] + J(t)), J given at the step times of its next production
+ * steps: J[k] at the start of step k (count matrices for count - 1 steps, at
+ * most 257).  Stage 0 takes J[k], stages 1 and 2 (J[k] + J[k+1]) / 2, stage 3
+ * J[k+1].  The next production call may run at most count - 1 steps
+ * (G12_E_ARG beyond) and uses the term up; J = NULL switches it off.
+ * G12_E_ARG for a correlated context.  No reference counterpart. */
+int g12_set_collision(g12_ctx* ctx, const double* J, int64_t count);
+
 /* Energy of the context's current state, as g12_energy with full U and the
  * quench shift at the context's current time.  Only G1 and the N^2
  * pair-diagonal elements of each G2 tensor leave the device.  No reference
@@ -208,6 +233,10 @@ int g12_batch_rk4_advance(g12_batch* b, int64_t n);
 /* n more production steps from the batch's current time; traj[i] (or NULL)
  * receives trajectory i's G1 before each of them (capacity traj_cap). */
 int g12_batch_rk4_record(g12_batch* b, int64_t n, double* const* traj, int64_t traj_cap);
+/* g12_batch_rk4_record that also writes each correlated trajectory's
+ * collision terms (as g12_rk4_record_collision) to jtraj[i] (or NULL). */
+int g12_batch_rk4_record_collision(g12_batch* b, int64_t n, double* const* traj, double* const* jtraj,
+                                   int64_t traj_cap);
 int g12_batch_guard(g12_batch* b, int* fail);
 int g12_batch_timing_enable(g12_batch* b, int on);
 int g12_batch_timing_get(g12_batch* b, int which, int64_t* launches, double* total_ms);
@@ -243,6 +272,9 @@ typedef struct g12_xp_config {
   int hidden;      /* hidden width (320) */
   double dt;       /* time step of the trajectories (0.01) */
   uint64_t seed;   /* weight initialisation and minibatch draws */
+  double noise;    /* training only: standard deviation of the Gaussian noise added to the
+                      normalised history features of each minibatch (0: none), which makes
+                      long autoregressive rollouts less sensitive to their own errors */
 } g12_xp_config;
 
 int g12_xp_default_config(g12_xp_config* c);
@@ -261,6 +293,12 @@ int g12_xp_features(g12_xp* x, const double* hist, int N, int Ns, int Nq, const 
                     double* scale);
 int g12_xp_rollout(g12_xp* x, const double* hist, int N, int Ns, int Nq, const double* w, const double* tr_ref,
                    int project, int nblocks, double* out, double* checks);
+/* The same for a Hermitian series (the collision term J = I + I^dagger of
+ * the learned closure): the lower triangle mirrors the upper as J^dagger = J;
+ * checks report the largest |tr J - tr_ref| and hermiticity error
+ * |J - J^dagger|; project restores Im J_ii = 0 and Re tr J = Re tr_ref. */
+int g12_xp_rollout_hermitian(g12_xp* x, const double* hist, int N, int Ns, int Nq, const double* w,
+                             const double* tr_ref, int project, int nblocks, double* out, double* checks);
 
 /* ---- routine-level entry points ------------------------------------------
  * One entry per reference routine, same argument roles and order, host

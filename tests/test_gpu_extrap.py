@@ -1,7 +1,8 @@
 """Learned extrapolator on the GPU against the CPU oracle (oracle/mlp.py):
 loss and gradient of the hand-written BF16 training kernels, inference,
 features and one rollout block; training on oscillations; and, with the
-committed trained network, an extrapolated run judged by plain RK4."""
+committed trained network (the learned collision closure), a closure run
+judged by plain RK4."""
 import os
 
 import numpy as np
@@ -106,19 +107,25 @@ def test_training_learns_oscillations(tmp_path):
 
 
 @pytest.mark.skipif(not os.path.exists(xp.DEFAULT_MODEL), reason="trained network not built")
-def test_trained_network_against_rk4_judge():
-    """N = 48 (the learned-extrapolator workload) over a shortened horizon with
-    the quench onset at step 800, judged by plain RK4:
-    - one-block predictions from true histories beat linear extrapolation
-      (what training gains over the untrained extrapolators);
-    - at the strict thresholds (trace 1e-6, anti-hermiticity 1e-5) rejected
-      blocks are replaced by RK4, which reproduces the judge bit for bit;
-    - with projection the network covers every step after the prefix, faster
-      than RK4, within a bounded density error."""
-    total, prefix = 1200, 400
+def test_trained_closure_against_rk4_judge():
+    """N = 48 (the learned-extrapolator workload) with the quench onset moved
+    to step 800, judged by plain RK4:
+    - one-block predictions of the collision term J from true histories beat
+      linear extrapolation (what training gains on its own target);
+    - past a 400-step RK4 prefix, the closure propagates G1 with the network's
+      J for 256 steps: trace and anti-hermiticity within the thresholds, every
+      step by the closure, a smaller density error than the Hartree-Fock
+      dynamics without the collision term (J = 0), and within 1.5e-2 (measured
+      9.7e-3; J frozen at its last value gave 6.3e-3 over these 256 steps: the
+      network's one-block gain does not survive a 16-block rollout)."""
+    total, prefix, horizon = 1200, 400, 256
     p = xp.config5_params(time_steps=total)
     p.quench_gate_time = p.quench_on_midpoint = 8.0
-    judge, t_judge = xp.rk4_run(p, total=total)
+    with g.Propagator(p) as prop:
+        prop.set_state(g.make_G0(p))
+        prop.adiabatic_switching()
+        rc, judge, J = prop.record_collision(total)
+    assert rc == 0
     w_all = xp.quench_shift(p, np.arange(total))
     with xp.Extrapolator() as m:
         m.load(xp.DEFAULT_MODEL)
@@ -126,24 +133,23 @@ def test_trained_network_against_rk4_judge():
         better, ratios = 0, []
         ks = range(prefix, total - F, 40)
         for k in ks:
-            pred, _ = m.rollout(judge[k - H:k], p.Ns, p.Nq, w_all[k - H:k + F], 1)
-            lin = judge[k - 1][None] + (judge[k - 1] - judge[k - 2])[None] * np.arange(1, F + 1)[:, None, None]
-            e_net = np.abs(pred - judge[k:k + F]).max()
-            e_lin = np.abs(lin - judge[k:k + F]).max()
+            pred, _ = m.rollout_hermitian(J[k - H:k], p.Ns, p.Nq, w_all[k - H:k + F], 1, project=False)
+            lin = J[k - 1][None] + (J[k - 1] - J[k - 2])[None] * np.arange(1, F + 1)[:, None, None]
+            e_net = np.abs(pred - J[k:k + F]).max()
+            e_lin = np.abs(lin - J[k:k + F]).max()
             better += e_net < e_lin
             ratios.append(e_net / e_lin)
         assert better >= 0.8 * len(ks) and np.median(ratios) < 0.5
-        strict = xp.extrapolated_run(p, m, prefix=prefix, total=total)
-        proj = xp.extrapolated_run(p, m, prefix=prefix, total=total, trace_tol=1e-3, herm_tol=1e-3, project=True)
-    tr = strict["traj"]
-    assert np.isfinite(tr).all() and np.array_equal(tr[:prefix], judge[:prefix])
-    rk_only = ~strict["by_net"]
-    assert np.array_equal(tr[rk_only], judge[rk_only])
-    assert strict["rk_steps"] <= total
-    tp = proj["traj"]
-    assert proj["by_net"][prefix:].all() and proj["rk_steps"] == prefix
-    tr_p = np.trace(tp, axis1=1, axis2=2)
-    assert np.abs(tr_p.imag - tr_p[0].imag).max() < 1e-10
-    assert np.abs(np.diagonal(tp[prefix:], axis1=1, axis2=2).real).max() == 0
-    assert np.abs(np.diagonal(tp - judge, axis1=1, axis2=2).imag).max() < 0.1
-    assert proj["seconds"] < t_judge
+        runs = {c: xp.closure_run(p, m, prefix=prefix, total=prefix + horizon, collision=c)
+                for c in ("network", "hold", "zero")}
+    err = {}
+    for c, r in runs.items():
+        tr = r["traj"]
+        assert np.isfinite(tr).all() and np.array_equal(tr[:prefix], judge[:prefix])
+        err[c] = float(np.abs(np.diagonal(tr[prefix:] - judge[prefix:prefix + horizon], axis1=1, axis2=2).imag).max())
+    print("closure density errors over the horizon:", err, flush=True)
+    net = runs["network"]
+    assert net["by_net"][prefix:].all() and net["rk_steps"] == prefix and net["rejected_chunks"] == 0
+    assert net["checks"]["trace"] <= 1e-6 and net["checks"]["herm"] <= 1e-5
+    assert err["network"] < err["zero"]
+    assert err["network"] < 1.5e-2
