@@ -23,7 +23,8 @@ QUENCH_FULL, QUENCH_PULSE = 0, 1
 VLAW_EXP, VLAW_COULOMB, VLAW_EXPLICIT = 0, 1, 2
 PREC_FP64, PREC_FP32_3M, PREC_BF16X3 = 0, 1, 2
 PRECISIONS = {"fp64": PREC_FP64, "fp32_3m": PREC_FP32_3M, "bf16x3": PREC_BF16X3}
-MAX_NB, MAX_SHELLS = 8, 128
+MAX_N, MAX_N_MIXED, MAX_SHELLS = 128, 64, 128
+MAX_NB = MAX_N
 
 _dp = ctypes.POINTER(ctypes.c_double)
 

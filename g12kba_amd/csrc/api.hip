@@ -103,7 +103,10 @@ int v_profile(const g12_params& p, double* prof) {
 
 bool correlated(const g12_params& p) { return p.U != 0.0 && !p.HF; }
 
-bool valid_geometry(int Ns, int Nb) { return Ns >= 1 && Nb >= 1 && Nb <= G12_MAX_NB && Ns * Nb <= 64; }
+bool valid_geometry(int Ns, int Nb) { return Ns >= 1 && Nb >= 1 && Nb <= G12_MAX_NB && Ns * Nb <= G12_MAX_N; }
+
+// The mixed-precision tile products hold a whole tile in LDS: N <= 64.
+bool valid_precision_size(int N, int precision) { return precision == G12_PREC_FP64 || N <= G12_MAX_N_MIXED; }
 
 // The time protocol (steps, switching, quench ramps) and the precision are
 // shared by every trajectory of a run; the physics (sizes, hoppings,
@@ -351,6 +354,11 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
     if (!p || ntraj < 1) return G12_E_ARG;
     for (int i = 0; i < ntraj; ++i) {
         if (!valid_geometry(p[i].Ns, p[i].Nb)) return G12_E_SIZE;
+        if (!valid_precision_size(p[i].Ns * p[i].Nb, p[i].precision)) {
+            std::fprintf(stderr, "g12kba: trajectory %d: N = %d exceeds %d, the mixed-precision limit (use FP64)\n", i,
+                         p[i].Ns * p[i].Nb, G12_MAX_N_MIXED);
+            return G12_E_SIZE;
+        }
         if (!same_protocol(p[0], p[i])) return G12_E_ARG;
         if (!valid_protocol(p[i])) {
             std::fprintf(stderr,
@@ -401,7 +409,8 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
         r.any_corr = r.any_corr || t.corr;
     }
     // buckets of correlated trajectories by padded tile size, larger N first
-    for (int NP = 16; NP <= 64; NP += 16) {
+    // (NP > 64: the wide-tile kernels)
+    for (int NP = 16; NP <= G12_MAX_N; NP += 16) {
         std::vector<int> l;
         for (int i = 0; i < ntraj; ++i)
             if (r.tp[i].corr && (r.tp[i].N + 15) / 16 * 16 == NP) l.push_back(i);
@@ -419,13 +428,13 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
         r.lists.push_back(l);
         r.prefixes.push_back(pre);
     }
-    // table launches: N x ceil(N / G12_TABLE_GROUP) blocks per correlated trajectory
+    // table launches: N x table_blocks_per_x(N) blocks per correlated trajectory
     r.tab_prefix.assign(1, 0);
     for (int i = 0; i < ntraj; ++i)
         if (r.tp[i].corr) {
-            const int64_t N = r.ds[i].N;
+            const int N = r.ds[i].N;
             r.tab_list.push_back(i);
-            r.tab_prefix.push_back(r.tab_prefix.back() + N * ((N + G12_TABLE_GROUP - 1) / G12_TABLE_GROUP));
+            r.tab_prefix.push_back(r.tab_prefix.back() + (int64_t)N * g12::table_blocks_per_x(N));
         }
     const int64_t total = layout(r, nullptr);
     if (hipMalloc(&r.arena, total) != hipSuccess) {
@@ -1340,6 +1349,7 @@ int g12_h2_commutator(const double* G2, const double* h1x, const double* h1y, do
     int rc = check_routine_geometry(Ns, Nb, 0);
     if (rc) return rc;
     const int N = Ns * Nb;
+    if (!valid_precision_size(N, precision)) return G12_E_SIZE;
     const int64_t n2 = (int64_t)N * N, n4 = n2 * n2;
     Scratch sc;
     double2 *draw = sc.get<double2>(n4), *dT = sc.get<double2>(n4), *dC = sc.get<double2>(n4);

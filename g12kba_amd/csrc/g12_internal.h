@@ -92,9 +92,13 @@ struct DevState {
 // A set of trajectories advanced together: device arrays of states and
 // constants, and (for the tile passes) buckets of trajectories with the same
 // padded tile size.
-// Table launches: one block per (X, group of G12_TABLE_GROUP values of the
-// other index) of each correlated trajectory; 64 (N <= 64): one block per X.
+// Table launches: one block per (X, G12_TABLE_GROUP x G12_TABLE_GROUP output
+// block) of each correlated trajectory (one block per X at N <= 64).
 #define G12_TABLE_GROUP 64
+__host__ __device__ inline int table_blocks_per_x(int N) {
+    const int nb = (N + G12_TABLE_GROUP - 1) / G12_TABLE_GROUP;
+    return nb * nb;
+}
 
 struct BatchDev {
     const DevState* ds;      // device array [ntraj]
@@ -102,8 +106,8 @@ struct BatchDev {
     int ntraj;
     int maxN;
     // correlated trajectories and the prefix of their table blocks
-    // (N x ceil(N / G12_TABLE_GROUP) each), so that a table launch has no
-    // idle blocks
+    // (N x table_blocks_per_x(N) each), so that a table launch has no idle
+    // blocks
     const int* tlist;        // device array [ncorr]
     const int64_t* tprefix;  // device array [ncorr + 1]
     int ncorr;
@@ -111,7 +115,7 @@ struct BatchDev {
 };
 
 struct Bucket {
-    int NP;                  // 16, 32, 48 or 64
+    int NP;                  // padded tile size: N rounded up to a multiple of 16 (wide tiles past 64)
     const int* list;         // device array of trajectory indices
     const int64_t* prefix;   // device array [n + 1]: tiles before each listed trajectory (both spins)
     int n;

@@ -309,7 +309,7 @@ __global__ __launch_bounds__(256) void k_g1_rhs(BatchDev b, StageRef r) {
     const int N = d.N;
     const int X = blockIdx.x;
     if (X >= N) return;
-    __shared__ double2 hrow[G12_MAX_NB * 64], grow[G12_MAX_NB * 64];
+    __shared__ double2 hrow[G12_MAX_N], grow[G12_MAX_N];
     __shared__ double2 part[4][64];
     const double2* h = d.h;
     const double2* Y1 = d.Y1;
@@ -502,9 +502,10 @@ __device__ __forceinline__ void tile_mfma_fp64(const TileLds<NP>& L, int hmode, 
 }
 
 // ---------------------------------------------------------------------------
-// Stage tables on FP64 MFMA.  One block per (trajectory, X) and table family
-// forms a 64 x 64 x 64 complex product with the tile product of the
-// commutator passes (operands zero-padded to 64, N <= 64):
+// Stage tables on FP64 MFMA.  One block per (trajectory, X, 64 x 64 output
+// block) and table family forms its block of an N x N x N complex product with
+// the tile product of the commutator passes, the sum over w taken in chunks of
+// 64 (operands zero-padded; one chunk and one output block per X at N <= 64):
 //   Qp: O[Z][Y] = sum_w A[Z][w] Y1[w][Y],  A[Z][w] = Y1[X][w] W_xy[Z][w];
 //       Qp^xy[X][Y][Z] = O[Z][Y],  Qp^xx[X][Y][Z] = O[Z][Y] - A[Z][Z] Y1[Z][Y]
 //       (W_xx equals W_xy off the diagonal and is 0 on it);
@@ -524,63 +525,74 @@ __device__ __forceinline__ void tile_mfma_fp64(const TileLds<NP>& L, int hmode, 
 // restores particle-number conservation to FP64 level.  FP64 runs use D as it
 // is, as the reference does, for any input.
 constexpr int TG = G12_TABLE_GROUP;
-static_assert(TG == 64, "one table block per X (N <= 64)");
+static_assert(TG == 64, "table output blocks of 64 x 64");
 constexpr int SETUP_THREADS = 1024;
 typedef TileCfg<64, 16> SetupCfg;
 static_assert(SetupCfg::NTHR == SETUP_THREADS && SetupCfg::PW == 1, "one 16x16 output block per wave");
 
-// (re, im) of this wave's 16x16 block of A.B, A in (hr, hi), B in (xr, xi);
-// register j of lane l holds row 4j + l/16, column l%16 of the block
-__device__ inline void setup_product(const TileLds<64>& L, d4& re, d4& im, int wave, int lane) {
+// (re, im) of this wave's 16x16 block of sum over chunks of A.B: fill(chunk)
+// writes A into (hr, hi) and B into (xr, xi); register j of lane l holds row
+// 4j + l/16, column l%16 of the block
+template <class Fill>
+__device__ inline void setup_product(TileLds<64>& L, int nchunks, Fill&& fill, d4& re, d4& im, int wave,
+                                     int lane) {
     d4 p1[1] = {d4{0.0, 0.0, 0.0, 0.0}}, p2[1] = {p1[0]}, p3[1] = {p1[0]};
-    tile_product_3m<64, 16, false>(L.hr, L.hi, L.xr, L.xi, p1, p2, p3, wave, lane);
+    for (int c = 0; c < nchunks; ++c) {
+        if (c) __syncthreads();  // the previous chunk's readers are done
+        fill(c);
+        __syncthreads();
+        tile_product_3m<64, 16, false>(L.hr, L.hi, L.xr, L.xi, p1, p2, p3, wave, lane);
+    }
     re = p1[0] - p2[0];
     im = p3[0] - p1[0] - p2[0];
 }
 
+// Qp block (rows Z in block zb, columns Y in block yb) of row X
 __device__ inline void q_tables_block(const DevState& d, const StageShared& ss, const StageParams& sp, int X,
-                                      TileLds<64>& L) {
+                                      int zb, int yb, TileLds<64>& L) {
     constexpr int P = TileLds<64>::P;
     const int N = d.N;
     const double2* S1 = uni_ptr(d.S1);
     const double2* K1 = uni_ptr(d.K1);
-    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {  // B = Y1
-        const int m = e >> 6, n = e & 63;
-        double2 y = make_double2(0.0, 0.0);
-        if (m < N && n < N) {
-            y = gload(S1, (int64_t)m * N + n);
-            if (!ss.first) y = cadd(y, cscale(gload(K1, (int64_t)m * N + n), ss.c));
-        }
-        L.xr[m * P + n] = y.x;
-        L.xi[m * P + n] = y.y;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {  // A[Z][w] = Y1[X][w] W_xy[Z][w]
-        const int Z = e >> 6, w = e & 63;
-        double2 a = make_double2(0.0, 0.0);
-        if (Z < N && w < N) a = cscale(make_double2(L.xr[X * P + w], L.xi[X * P + w]), w_xy(w, Z, d.Ns, sp));
-        L.hr[Z * P + w] = a.x;
-        L.hi[Z * P + w] = a.y;
-    }
-    __syncthreads();
+    auto y1 = [&](int64_t i) {  // stage G1
+        double2 y = gload(S1, i);
+        if (!ss.first) y = cadd(y, cscale(gload(K1, i), ss.c));
+        return y;
+    };
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     d4 re, im;
-    setup_product(L, re, im, wave, lane);
-    const int Y = (wave & 3) * 16 + (lane & 15);
+    setup_product(L, (N + TG - 1) / TG, [&](int c) {
+        for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {  // B = Y1 rows w, columns Y
+            const int w = c * TG + (e >> 6), Y = yb * TG + (e & 63);
+            const double2 y = (w < N && Y < N) ? y1((int64_t)w * N + Y) : make_double2(0.0, 0.0);
+            L.xr[(e >> 6) * P + (e & 63)] = y.x;
+            L.xi[(e >> 6) * P + (e & 63)] = y.y;
+        }
+        for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {  // A[Z][w] = Y1[X][w] W_xy[Z][w]
+            const int Z = zb * TG + (e >> 6), w = c * TG + (e & 63);
+            double2 a = make_double2(0.0, 0.0);
+            if (Z < N && w < N) a = cscale(y1((int64_t)X * N + w), w_xy(w, Z, d.Ns, sp));
+            L.hr[(e >> 6) * P + (e & 63)] = a.x;
+            L.hi[(e >> 6) * P + (e & 63)] = a.y;
+        }
+    }, re, im, wave, lane);
+    const int Y = yb * TG + (wave & 3) * 16 + (lane & 15);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int Z = (wave >> 2) * 16 + 4 * j + (lane >> 4);
+        const int Z = zb * TG + (wave >> 2) * 16 + 4 * j + (lane >> 4);
         if (Z >= N || Y >= N) continue;
         const double2 o = make_double2(re[j], im[j]);
-        const double2 wzz = cmul(make_double2(L.hr[Z * P + Z], L.hi[Z * P + Z]),
-                                 make_double2(L.xr[Z * P + Y], L.xi[Z * P + Y]));  // the w = Z term
+        const double2 wzz = cmul(cscale(y1((int64_t)X * N + Z), w_xy(Z, Z, d.Ns, sp)),
+                                 y1((int64_t)Z * N + Y));  // the w = Z term
         const int64_t qp = ((int64_t)X * N + Y) * N + Z;
         d.Qp[0][qp] = csub(o, wzz);
         d.Qp[1][qp] = o;
     }
 }
 
-__device__ inline void f_tables_block(const DevState& d, const StageParams& sp, int X, bool herm, TileLds<64>& L) {
+// F block (rows Y in block yb, columns Z in block zb) of row X
+__device__ inline void f_tables_block(const DevState& d, const StageParams& sp, int X, int yb, int zb, bool herm,
+                                      TileLds<64>& L) {
     constexpr int P = TileLds<64>::P;
     const int N = d.N;
     auto dval = [&](int s, int Y, int w) {  // D^s[X][Y][w], its Hermitian part with herm
@@ -588,30 +600,28 @@ __device__ inline void f_tables_block(const DevState& d, const StageParams& sp, 
         if (herm) v = cscale(cadd(v, cconj(d.D[s][((int64_t)Y * N + X) * N + w])), 0.5);
         return v;
     };
-    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {  // A = D^xx + D^xy, B = W_xy (real)
-        const int m = e >> 6, n = e & 63;
-        double2 a = make_double2(0.0, 0.0);
-        double wv = 0.0;
-        if (m < N && n < N) {
-            a = cadd(dval(0, m, n), dval(1, m, n));
-            wv = w_xy(m, n, d.Ns, sp);
-        }
-        L.hr[m * P + n] = a.x;
-        L.hi[m * P + n] = a.y;
-        L.xr[m * P + n] = wv;
-        L.xi[m * P + n] = 0.0;
-    }
-    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     d4 re, im;
-    setup_product(L, re, im, wave, lane);
-    const int Z = (wave & 3) * 16 + (lane & 15);
+    setup_product(L, (N + TG - 1) / TG, [&](int c) {
+        for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {  // A = D^xx + D^xy, B = W_xy (real)
+            const int m = e >> 6, n = e & 63;
+            const int Y = yb * TG + m, w = c * TG + n;     // A[Y][w]
+            const int wb = c * TG + m, Z = zb * TG + n;    // B[w][Z]
+            double2 a = make_double2(0.0, 0.0);
+            if (Y < N && w < N) a = cadd(dval(0, Y, w), dval(1, Y, w));
+            L.hr[m * P + n] = a.x;
+            L.hi[m * P + n] = a.y;
+            L.xr[m * P + n] = (wb < N && Z < N) ? w_xy(wb, Z, d.Ns, sp) : 0.0;
+            L.xi[m * P + n] = 0.0;
+        }
+    }, re, im, wave, lane);
+    const int Z = zb * TG + (wave & 3) * 16 + (lane & 15);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int Y = (wave >> 2) * 16 + 4 * j + (lane >> 4);
+        const int Y = yb * TG + (wave >> 2) * 16 + 4 * j + (lane >> 4);
         if (Y >= N || Z >= N) continue;
         const double2 sv = make_double2(re[j], im[j]);
-        const double wzz = L.xr[Z * P + Z];  // the w = Z weight W_xx lacks
+        const double wzz = w_xy(Z, Z, d.Ns, sp);  // the w = Z weight W_xx lacks
         const double2 fxx = cmul_mi(csub(sv, cscale(dval(0, Y, Z), wzz)));
         const double2 fxy = cmul_mi(csub(sv, cscale(dval(1, Y, Z), wzz)));
         const int64_t f = ((int64_t)X * N + Y) * N + Z;
@@ -622,8 +632,8 @@ __device__ inline void f_tables_block(const DevState& d, const StageParams& sp, 
 }
 
 // Table launches enumerate blocks over the correlated trajectories only:
-// block blk is entry (blk - tprefix[j]) of the N x ceil(N / TG) block space
-// of listed trajectory j.
+// block blk is entry (blk - tprefix[j]) of the N x table_blocks_per_x(N)
+// block space of listed trajectory j.
 __device__ inline int table_block(const BatchDev& b, int64_t blk, int& e) {
     int lo = 0, hi = b.ncorr - 1;
     while (lo < hi) {  // last correlated trajectory whose blocks start at or before blk
@@ -638,7 +648,8 @@ __device__ inline int table_block(const BatchDev& b, int64_t blk, int& e) {
 // the G1 pairs, h = h_HF[Y1], W tables) and the two table families, which
 // form Y1 and W themselves, so that none of the three waits for another.
 // Blocks [0, gA * ntraj): G1 part, gA blocks of SETUP_THREADS elements per
-// trajectory; then tblocks Qp blocks and tblocks F blocks (one per X).
+// trajectory; then tblocks Qp blocks and tblocks F blocks (one per X and 64 x 64
+// output block).
 __global__ __launch_bounds__(SETUP_THREADS) void k_stage_setup(BatchDev b, StageRef r, int herm, int gA) {
     __shared__ StageParams sp;
     __shared__ __align__(16) TileLds<64> L;
@@ -661,12 +672,16 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_stage_setup(BatchDev b, Stage
     // otherwise live in per-lane scratch
     fill_sp(sp, tp, ss);
     __syncthreads();
-    if (role == 0)
+    if (role == 0) {
         stage_g1_block(d, ss, sp, e, reinterpret_cast<double2*>(L.xr));
-    else if (role == 1)
-        q_tables_block(d, ss, sp, e, L);
+        return;
+    }
+    const int nb = (d.N + TG - 1) / TG, per_x = table_blocks_per_x(d.N);
+    const int X = e / per_x, rb = (e % per_x) / nb, cb = (e % per_x) % nb;
+    if (role == 1)
+        q_tables_block(d, ss, sp, X, rb, cb, L);
     else
-        f_tables_block(d, sp, e, herm != 0, L);
+        f_tables_block(d, sp, X, rb, cb, herm != 0, L);
 }
 
 // ---- precision engines ------------------------------------------------------
@@ -1536,6 +1551,226 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wide tiles: 64 < N <= 128, FP64.  An N x N complex128 tile (up to 256 KB)
+// and h do not fit in LDS together, so both products stream their operands
+// through LDS in k-chunks of WK: h.X stages columns [k0, k0 + WK) of h and
+// rows [k0, k0 + WK) of X, then -X.h columns of X (negated as they are
+// staged, so both products accumulate) and rows of h.  The next chunk loads
+// into registers while the current one is multiplied.  The products are 3M on
+// v_mfma_f64_16x16x4f64 as in k_tile, and the epilogues (sources, RK update,
+// partial traces) are those of k_tile's FP64 passes.  The (N/16)^2 output
+// blocks of 16x16 are taken in passes of WNW x WPW blocks, WPW per wave (a
+// wave of sixteen has 128 registers, and the 3M accumulators of two blocks
+// take 48); each pass streams the operands again, from L2.  Workgroups are
+// persistent over the bucket's tile list with k_tile's XCD-aware schedule.
+constexpr int WNP = 128, WNW = 16, WTHR = WNW * 64, WK = 16, WPW = 2;
+static_assert(WNP == G12_MAX_N, "wide tiles cover every N the library takes");
+struct WideLds {
+    static constexpr int PA = WK + 1, PB = WNP + 1;  // padded pitches (doubles)
+    double ar[WNP * PA], ai[WNP * PA];               // A chunk [m][k]
+    double br[WK * PB], bi[WK * PB];                 // B chunk [k][n]
+};
+// staged elements per thread and chunk: WNP x WK of A, WK x WNP of B
+constexpr int WSLOTS = WNP * WK / WTHR;
+static_assert(WSLOTS * WTHR == WNP * WK && WK == 16 && WNP == 128, "slot decomposition below");
+
+template <bool FUSED>
+__global__ __launch_bounds__(WTHR) void k_tile_wide(BTileArgs a) {
+    __shared__ WideLds L;
+    __shared__ double2 srcv[FUSED ? 1 : 3][WNP];
+    constexpr int PA = WideLds::PA, PB = WideLds::PB;
+    const StageShared ss = uni_stage(a.r);
+    const TileSched sched(a.k.tiles, gridDim.x, blockIdx.x);
+    const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63, tid = threadIdx.x;
+    const int r16 = lane & 15, kq = lane >> 4;
+    Cursor c;
+    View v;
+    for (int it = 0;; ++it) {
+        const int64_t g = sched.tile(it, blockIdx.x, gridDim.x);
+        if (g < 0) break;
+        seek(a, ss, c, g);
+        if (!c.on) continue;
+        const TileRef tr = ref_of(c, g);
+        const int N = tr.N;
+        if (tr.traj != v.traj || tr.spin != v.spin) load_view<FUSED>(v, a, ss, tr.traj, tr.spin);
+        const double2* h = uni_ptr(a.h_override ? a.h_override : a.b.ds[tr.traj].h);
+        const double2* X = v.X;
+        const TileGeom gn = tile_geom(FUSED ? 1 : a.tiling, N, tr.tile, false);
+        const int t0 = gn.t0, t1 = gn.t1;
+        const int nt = (N + 15) >> 4, TT = nt * nt;
+        const int nc = (N + WK - 1) / WK;
+        __syncthreads();  // the previous tile's readers of LDS are done
+        if (a.sources && wave == 0) {  // source vectors, elements x = lane + 64 i
+            for (int x = lane; x < N; x += 64) {
+                SrcVec sv;
+                src_load<FUSED>(v, tr, x, sv);
+                if constexpr (FUSED) {
+                    srcv[0][x] = cadd(cmul_i(sv.q), sv.f);
+                } else {
+                    srcv[0][x] = cadd(cscale(sv.g, sv.w0), cmul_i(sv.q));
+                    srcv[1][x] = cadd(cscale(sv.g, sv.w1), cmul_i(sv.q));
+                    srcv[2][x] = cconj(sv.f);
+                }
+            }
+        }
+        // chunks [0, nc): h.X; [nc, 2 nc): -X.h
+        const int c0 = (a.hmode & 1) ? 0 : nc, c1 = (a.hmode & 2) ? 2 * nc : nc;
+        // fused stages 1 and 2 read their input from Y and write the next
+        // stage's input there: with more than one pass, a pass would overwrite
+        // elements a later pass still streams, so each pass writes the next
+        // input into R instead (R of an element is dead once this pass has
+        // added it) and the tile copies R to Y after its last pass
+        const bool park = FUSED && !ss.last && X == v.Y && TT > WNW * WPW;
+        double2 ra[WSLOTS], rb[WSLOTS];
+        auto fetch = [&](int ci) {
+            const bool second = ci >= nc;
+            const int k0 = (second ? ci - nc : ci) * WK;
+#pragma unroll
+            for (int i = 0; i < WSLOTS; ++i) {
+                const int s = tid + WTHR * i;
+                const int m = s >> 4, ka = k0 + (s & 15);   // A slot (m, k)
+                const int kb = k0 + (s >> 7), n = s & 127;  // B slot (k, n)
+                double2 va = make_double2(0.0, 0.0), vb = va;
+                if (m < N && ka < N) {
+                    if (second) {
+                        const double2 x = gld(X, gn.nat(m, ka));
+                        va = make_double2(-x.x, -x.y);
+                    } else {
+                        va = gld(h, (uint32_t)(m * N + ka));
+                    }
+                }
+                if (kb < N && n < N) vb = second ? gld(h, (uint32_t)(kb * N + n)) : gld(X, gn.nat(kb, n));
+                ra[i] = va;
+                rb[i] = vb;
+            }
+        };
+        for (int tb = 0; tb < TT; tb += WNW * WPW) {  // passes over the output blocks
+            d4 p1[WPW], p2[WPW], p3[WPW];
+#pragma unroll
+            for (int u = 0; u < WPW; ++u) {
+                p1[u] = d4{0.0, 0.0, 0.0, 0.0};
+                p2[u] = p1[u];
+                p3[u] = p1[u];
+            }
+            // R (the strided-pass result) enters the fused accumulators between
+            // the two products, as in k_tile: Re = P1 - P2 through P1,
+            // Im = P3 - P1 - P2 through P3
+            auto mid = [&]() {
+                if constexpr (FUSED) {
+#pragma unroll
+                    for (int u = 0; u < WPW; ++u) {
+                        const int t = tb + wave + WNW * u;
+                        if (t >= TT) continue;
+                        const int n = (t % nt) * 16 + r16;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int m = (t / nt) * 16 + 4 * j + kq;
+                            if (m < N && n < N) {
+                                const double2 rv = gld(v.R, gn.nat(m, n));
+                                p1[u][j] += rv.x;
+                                p3[u][j] += rv.x + rv.y;
+                            }
+                        }
+                    }
+                }
+            };
+            if (c0 < c1) fetch(c0);
+            for (int ci = c0; ci < c1; ++ci) {
+                if (ci == nc) mid();
+                __syncthreads();  // the previous chunk's readers are done
+#pragma unroll
+                for (int i = 0; i < WSLOTS; ++i) {
+                    const int s = tid + WTHR * i;
+                    L.ar[(s >> 4) * PA + (s & 15)] = ra[i].x;
+                    L.ai[(s >> 4) * PA + (s & 15)] = ra[i].y;
+                    L.br[(s >> 7) * PB + (s & 127)] = rb[i].x;
+                    L.bi[(s >> 7) * PB + (s & 127)] = rb[i].y;
+                }
+                __syncthreads();
+                if (ci + 1 < c1) fetch(ci + 1);  // in flight during this chunk's MFMAs
+#pragma unroll
+                for (int j = 0; j < WK / 4; ++j) {
+                    const int kk = kq * (WK / 4) + j;
+#pragma unroll
+                    for (int u = 0; u < WPW; ++u) {
+                        const int t = tb + wave + WNW * u;
+                        if (t >= TT) continue;
+                        const int mm = (t / nt) * 16 + r16, nn = (t % nt) * 16 + r16;
+                        const double xr = L.ar[mm * PA + kk], xi = L.ai[mm * PA + kk];
+                        const double yr = L.br[kk * PB + nn], yi = L.bi[kk * PB + nn];
+                        p1[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr, yr, p1[u], 0, 0, 0);
+                        p2[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi, yi, p2[u], 0, 0, 0);
+                        p3[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr + xi, yr + yi, p3[u], 0, 0, 0);
+                    }
+                }
+            }
+            if (c1 <= nc) mid();
+            __syncthreads();  // srcv written by wave 0 is visible (no chunk may have run)
+#pragma unroll
+            for (int u = 0; u < WPW; ++u) {
+                const int t = tb + wave + WNW * u;
+                if (t >= TT) continue;
+                const int n = (t % nt) * 16 + r16;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int m = (t / nt) * 16 + 4 * j + kq;
+                    if (m >= N || n >= N) continue;
+                    const double2 o = make_double2(p1[u][j] - p2[u][j], p3[u][j] - p1[u][j] - p2[u][j]);
+                    const uint32_t off = gn.nat(m, n);
+                    if constexpr (!FUSED) {
+                        // strided-pass sources: g11 (a[m] - b[n]) - conj g(q1,p1) (c[n] - c[m])
+                        double2 val = o;
+                        if (a.sources) {
+                            const uint32_t gi = (uint32_t)(m * N + n);
+                            const double2 g11 = gld(v.G1, gi), gt = gld(v.G1t, gi);
+                            val = cadd(val, csub(cmul(g11, csub(srcv[0][m], srcv[1][n])),
+                                                 cmul(gt, csub(srcv[2][n], srcv[2][m]))));
+                        }
+                        if (a.accumulate) val = cadd(gld(v.R, off), val);
+                        gst(v.R, off, val);
+                    } else {
+                        // fused update (k_tile): K = -i (R + O + g22 (e[m] - e[n]));
+                        // A += w K and Y = S + c_next K, or S += dt (A + w K) / 6
+                        double2 rr = o;
+                        if (a.sources)
+                            rr = cadd(rr, cmul(gld(v.G1, (uint32_t)(m * N + n)), csub(srcv[0][m], srcv[0][n])));
+                        const double2 K = cmul_mi(rr);
+                        const double2 wK = cscale(K, ss.wk);
+                        const double2 sv = gld(v.S, off);
+                        double2 next;
+                        if (!ss.last) {
+                            gst(v.A, off, ss.first ? wK : cadd(gld(v.A, off), wK));
+                            next = cadd(sv, cscale(K, ss.cn));
+                            gst(park ? v.R : v.Y, off, next);
+                        } else {
+                            next = cadd(sv, cdiv6(cscale(cadd(gld(v.A, off), wK), ss.dt)));
+                            gst(v.S, off, next);
+                        }
+                        if (m == n) gst(v.D, (uint32_t)((t0 * N + t1) * N + m), next);
+                    }
+                }
+            }
+        }
+        if (park) {
+            __syncthreads();  // every pass has streamed X (= Y)
+            // each thread moves the elements it wrote itself
+            for (int tb = 0; tb < TT; tb += WNW * WPW)
+#pragma unroll
+                for (int u = 0; u < WPW; ++u) {
+                    const int t = tb + wave + WNW * u;
+                    if (t >= TT) continue;
+                    const int n = (t % nt) * 16 + r16;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int m = (t / nt) * 16 + 4 * j + kq;
+                        if (m < N && n < N) gst(v.Y, gn.nat(m, n), gld(v.R, gn.nat(m, n)));
+                    }
+                }
+        }
+    }
+}
+
 // Start of a run: partial traces of the state (the stage-0 input)
 // D[X][Y][Z] = S[X][Z][Y][Z] for both spins and, in mixed-precision runs, the
 // complex64 copy of S in Y that the tile passes read as the stage-0 input;
@@ -1630,13 +1865,33 @@ void launch_tile_np(const BTileArgs& a, hipStream_t s) {
     }
 }
 
+// Wide tiles (NP = 80 ... 128)
+void launch_tile_wide(const BTileArgs& a, hipStream_t s) {
+    constexpr int lds = (int)sizeof(WideLds) + 3 * WNP * (int)sizeof(double2);
+    int per_cu = (160 * 1024) / lds;
+    if (per_cu > 32 / WNW) per_cu = 32 / WNW;
+    if (a.k.wg_per_cu > 0 && per_cu > a.k.wg_per_cu) per_cu = a.k.wg_per_cu;
+    int64_t grid = (int64_t)device_cus() * per_cu;
+    if (grid > a.k.tiles) grid = a.k.tiles;
+    if (grid >= 8) grid &= ~7;
+    if (grid < 1) return;
+    if (a.fused)
+        hipLaunchKernelGGL(k_tile_wide<true>, dim3((unsigned)grid), dim3(WTHR), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_tile_wide<false>, dim3((unsigned)grid), dim3(WTHR), 0, s, a);
+}
+
 template <int PREC>
 void launch_tile_prec(const BTileArgs& a, hipStream_t s) {
     switch (a.k.NP) {
         case 16: launch_tile_np<16, PREC>(a, s); break;
         case 32: launch_tile_np<32, PREC>(a, s); break;
         case 48: launch_tile_np<48, PREC>(a, s); break;
-        default: launch_tile_np<64, PREC>(a, s); break;
+        case 64: launch_tile_np<64, PREC>(a, s); break;
+        default:
+            // N > 64: FP64 only (the host rejects mixed precision there)
+            if constexpr (PREC == G12_PREC_FP64) launch_tile_wide(a, s);
+            break;
     }
 }
 
@@ -1850,6 +2105,10 @@ int device_cu_count() { return device_cus(); }
 int default_waves(int NP) { return NP >= 48 ? 16 : 4; }
 
 int tile_wave_options(int NP, int* opts) {
+    if (NP > 64) {  // wide tiles: one wave count
+        opts[0] = WNW;
+        return 1;
+    }
     const int w0 = base_waves(NP);
     opts[0] = w0;
     opts[1] = 2 * w0;
@@ -1898,10 +2157,10 @@ void launch_nat_to_ref(const double2* in, double2* out, int Ns, int Nb, hipStrea
 }
 
 // Launch descriptors of a one-trajectory batch in a device area of
-// COMM2_DESC_BYTES: DevState at 0, TrajParams at 512, the bucket list at 2048,
-// the tile prefix at 2304.  Written by a kernel (kernel arguments by value),
+// COMM2_DESC_BYTES: DevState at 0, TrajParams at 512, the bucket list at 3072,
+// the tile prefix at 3328.  Written by a kernel (kernel arguments by value),
 // so the whole enqueue needs no host copy and can be captured in a graph.
-constexpr int COMM2_DS = 0, COMM2_TP = 512, COMM2_LIST = 2048, COMM2_PREFIX = 2304;
+constexpr int COMM2_DS = 0, COMM2_TP = 512, COMM2_LIST = 3072, COMM2_PREFIX = 3328;
 static_assert(sizeof(DevState) <= COMM2_TP - COMM2_DS, "DevState fits its slot");
 static_assert(COMM2_TP + sizeof(TrajParams) <= COMM2_LIST, "TrajParams fits its slot");
 static_assert(COMM2_PREFIX + 2 * sizeof(int64_t) <= COMM2_DESC_BYTES, "descriptors fit");
@@ -1935,7 +2194,7 @@ int enqueue_comm2(const double2* T, const double2* hx, const double2* hy, double
     a.b.tp = reinterpret_cast<const TrajParams*>(desc + COMM2_TP);
     a.b.ntraj = 1;
     a.b.maxN = N;
-    a.k.NP = N <= 16 ? 16 : N <= 32 ? 32 : N <= 48 ? 48 : 64;
+    a.k.NP = (N + 15) / 16 * 16;
     a.k.list = reinterpret_cast<const int*>(desc + COMM2_LIST);
     a.k.prefix = reinterpret_cast<const int64_t*>(desc + COMM2_PREFIX);
     a.k.n = 1;

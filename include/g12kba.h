@@ -52,7 +52,12 @@ extern "C" {
 #define G12_PREC_FP32_3M 1   /* 3M complex FP32 products, FP32 accumulation */
 #define G12_PREC_BF16X3 2    /* BF16 hi/lo split (hh + hl + lh), FP32 accumulation */
 
-#define G12_MAX_NB 8
+/* Sizes: N = Ns*Nb <= G12_MAX_N in FP64 (the reference takes any Ns, Nb:
+ * read_input.h:85-146, Main.cpp:48-50), N <= G12_MAX_N_MIXED in the
+ * mixed-precision modes, whose tile products hold a whole N x N tile in LDS. */
+#define G12_MAX_N 128
+#define G12_MAX_N_MIXED 64
+#define G12_MAX_NB G12_MAX_N
 #define G12_MAX_SHELLS 128
 
 /* Run parameters: every global of read_input.h:27-49 plus the choices the
@@ -111,7 +116,8 @@ int g12_init_g_test(const g12_params* p, double* G1);
 
 /* ---- context, state, drivers --------------------------------------------- */
 
-/* G12_E_SIZE for sizes the kernels do not support; G12_E_ARG for a
+/* G12_E_SIZE for N = Ns*Nb beyond G12_MAX_N (G12_MAX_N_MIXED in the
+ * mixed-precision modes); G12_E_ARG for a
  * non-finite parameter, AS_rate < 0, AS_dt <= 0, a production clock that
  * never reaches time_steps * dt, or a switching ramp that does not reach U
  * exactly (AS.h:96-100) within 10^7 steps. */

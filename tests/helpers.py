@@ -39,3 +39,25 @@ def physical_g2(rng, Ns, Nb, scale=1e-3):
     M = np.empty((N * N, N * N), np.complex128)
     M[np.ix_(pair, pair)] = T.reshape(N * N, N * N)
     return M
+
+
+def product_state(rng, N, rank=3):
+    """T = sum_r a_r (x) b_r in the natural layout, T[p1][p2][q1][q2] =
+    sum_r a_r[p1][q1] b_r[p2][q2] (complex128, N^4 elements)."""
+    a = [rand_c(rng, N, N) for _ in range(rank)]
+    b = [rand_c(rng, N, N) for _ in range(rank)]
+    T = np.zeros((N, N, N, N), np.complex128)
+    for ar, br in zip(a, b):
+        T += np.einsum("ac,bd->abcd", ar, br)
+    return T, a, b
+
+
+def product_state_commutator(a, b, hx, hy):
+    """[H2, T] = (hx(x)1 + 1(x)hx) T - T (hy(x)1 + 1(x)hy) of T = sum_r a_r (x) b_r:
+    sum_r (hx a_r - a_r hy) (x) b_r + a_r (x) (hx b_r - b_r hy), natural layout."""
+    N = hx.shape[0]
+    out = np.zeros((N, N, N, N), np.complex128)
+    for ar, br in zip(a, b):
+        out += np.einsum("ac,bd->abcd", hx @ ar - ar @ hy, br)
+        out += np.einsum("ac,bd->abcd", ar, hx @ br - br @ hy)
+    return out

@@ -191,3 +191,23 @@ def test_create_accepts_what_the_reference_runs(ok):
     if rc == 0:
         g.lib.g12_destroy(h)
     assert rc in (0, -2)
+
+
+@pytest.mark.parametrize("Ns,Nb,prec,ok", [(65, 1, "fp32_3m", False), (33, 2, "bf16x3", False), (129, 1, "fp64", False),
+                                           (1, 129, "fp64", False), (64, 2, "fp64", True), (36, 2, "fp64", True),
+                                           (2, 12, "fp32_3m", True), (32, 2, "bf16x3", True)])
+def test_create_size_limits(Ns, Nb, prec, ok):
+    """N = Ns*Nb up to 128 in FP64 with any split into sites and orbitals
+    (read_input.h:85-146 takes any Ns, Nb); the mixed-precision modes stop at
+    N = 64.  Beyond: G12_E_SIZE before any device work."""
+    import ctypes
+    p = g.make_params(Ns=Ns, Nb=Nb, U=1.0, dt=0.02, time_steps=2, precision=prec)
+    h = ctypes.c_void_p()
+    rc = g.lib.g12_create(ctypes.byref(p), ctypes.byref(h))
+    if rc == 0:
+        g.lib.g12_destroy(h)
+    if ok:
+        assert rc in (0, -2)
+    else:
+        assert rc == -4
+        assert g.lib.g12_batch_create(ctypes.byref(p), 1, ctypes.byref(h)) == -4

@@ -17,7 +17,7 @@ import pytest
 
 from oracle import fast
 from oracle.oracle import REF_SO, Oracle, Ref
-from tests.helpers import GEOMS, rand_c
+from tests.helpers import GEOMS, product_state, product_state_commutator, rand_c
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
@@ -95,6 +95,21 @@ def test_comm2_kron_bruteforce(Ns, Nb):
     Mn = fast.ref_to_nat(M, Ns, Nb).reshape(N * N, N * N)
     want = fast.nat_to_ref((Hx @ Mn - Mn @ Hy).reshape(N, N, N, N), Ns, Nb)
     assert rel(O.comm2(M, hx, hy, Ns, Nb), want) < TOL
+
+
+@pytest.mark.parametrize("Ns,Nb", [(3, 2), (2, 5)])
+def test_comm2_product_state_closed_form(Ns, Nb):
+    """The closed form the N = 128 GPU check uses: for T = sum_r a_r (x) b_r,
+    [H2, T] = sum_r (hx a_r - a_r hy) (x) b_r + a_r (x) (hx b_r - b_r hy);
+    against the oracle's commutator, and failing with hy in place of hx on the
+    left (a transposed-operand mistake)."""
+    rng = np.random.default_rng(Ns * 7 + Nb)
+    N = Ns * Nb
+    T, a, b = product_state(rng, N)
+    hx, hy = rand_c(rng, N, N), rand_c(rng, N, N)
+    got = fast.ref_to_nat(O.comm2(fast.nat_to_ref(T, Ns, Nb), hx, hy, Ns, Nb), Ns, Nb)
+    assert rel(got, product_state_commutator(a, b, hx, hy)) < TOL
+    assert rel(got, product_state_commutator(a, b, hy, hy)) > 0.1
 
 
 @pytest.mark.parametrize("Ns,Nb,nV", GEOMS[:4])
