@@ -1176,16 +1176,6 @@ struct Scratch {
 
 hipStream_t api_stream() { return nullptr; }  // legacy default stream: calls are synchronous
 
-StageParams interaction_params(int Ns, int Nb, double U, const double* V, int nV) {
-    StageParams sp{};
-    sp.U = U;
-    sp.nV = nV < G12_MAX_SHELLS ? nV : G12_MAX_SHELLS;
-    for (int i = 0; i < sp.nV; ++i) sp.V[i] = V[i];
-    (void)Ns;
-    (void)Nb;
-    return sp;
-}
-
 int up(void* dst, const void* src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : G12_E_HIP;
 }
@@ -1200,6 +1190,135 @@ int check_routine_geometry(int Ns, int Nb, int nV) {
     if (!valid_geometry(Ns, Nb) || nV < 0 || nV > G12_MAX_SHELLS) return G12_E_SIZE;
     if (select_device() < 0) return G12_E_HIP;
     return 0;
+}
+
+// A one-trajectory batch on scratch buffers through which the routine-level
+// entries run the production kernels: the stage set-up (G1 part: Y1, its
+// conjugate transpose, h_HF and the W tables; Qp and F tables and the
+// collision integral) and the tile passes with their source epilogues.  The
+// interactions are given as V_n themselves (v_abs), and the stage is stage 0 of
+// a production step unless set otherwise (Y1 = S1, stage input S2).
+struct RoutineBatch {
+    Scratch sc;
+    DevState d{};
+    TrajParams tp{};
+    BatchDev b{};
+    Bucket k{};
+    hipStream_t s = api_stream();
+
+    // with_g2: allocate the G2 tensors and tables and enable the table roles
+    int init(int Ns, int Nb, double U, const double* V, int nV, bool with_g2) {
+        const int N = Ns * Nb;
+        d.Ns = Ns;
+        d.Nb = Nb;
+        d.N = N;
+        d.n2 = (int64_t)N * N;
+        d.n3 = d.n2 * N;
+        d.n4 = d.n2 * d.n2;
+        std::vector<std::pair<void*, int64_t>> zero;
+        auto c2 = [&](int64_t n) {
+            double2* p = sc.get<double2>(n);
+            zero.push_back({p, (int64_t)sizeof(double2) * n});
+            return p;
+        };
+        d.S1 = c2(d.n2);
+        d.K1 = c2(d.n2);
+        d.A1 = c2(d.n2);
+        d.Y1 = c2(d.n2);
+        d.h = c2(d.n2);
+        d.I = c2(d.n2);
+        d.G1t = c2(d.n2);
+        d.Wxy = sc.get<double>(d.n2);
+        d.Wxx = sc.get<double>(d.n2);
+        if (with_g2)
+            for (int sp = 0; sp < 2; ++sp) {
+                d.S2[sp] = c2(d.n4);
+                d.Y2[sp] = c2(d.n4);
+                d.A2[sp] = c2(d.n4);
+                d.R2[sp] = c2(d.n4);
+                d.D[sp] = c2(d.n3);
+                d.F[sp] = c2(d.n3);
+                d.Qp[sp] = c2(d.n3);
+            }
+        for (auto& z : zero)
+            if (!z.first) return G12_E_NOMEM;
+        if (!d.Wxy || !d.Wxx) return G12_E_NOMEM;
+        for (auto& z : zero)
+            if (hipMemsetAsync(z.first, 0, z.second, s) != hipSuccess) return G12_E_HIP;
+        tp.Ns = Ns;
+        tp.Nb = Nb;
+        tp.N = N;
+        tp.corr = with_g2 ? 1 : 0;
+        tp.U = U;
+        tp.nV = nV < G12_MAX_SHELLS ? nV : G12_MAX_SHELLS;
+        tp.v_abs = 1;
+        for (int i = 0; i < tp.nV; ++i) tp.Vprof[i] = V[i];
+        return 0;
+    }
+
+    // descriptors on the device; call after the last change to d and tp
+    int upload() {
+        const int N = d.N;
+        const int64_t tblocks = tp.corr ? (int64_t)N * g12::table_blocks_per_x(N) : 0;
+        DevState* dds = sc.get<DevState>(1);
+        TrajParams* dtp = sc.get<TrajParams>(1);
+        int* list = sc.get<int>(1);
+        int64_t* prefix = sc.get<int64_t>(2);
+        int64_t* tprefix = sc.get<int64_t>(2);
+        DEV_OR_FAIL(dds && dtp && list && prefix && tprefix);
+        const int zero_i = 0;
+        const int64_t pre[2] = {0, 2 * d.n2}, tpre[2] = {0, tblocks};
+        int rc;
+        if ((rc = up(dds, &d, sizeof(DevState))) || (rc = up(dtp, &tp, sizeof(TrajParams))) ||
+            (rc = up(list, &zero_i, sizeof(int))) || (rc = up(prefix, pre, sizeof(pre))) ||
+            (rc = up(tprefix, tpre, sizeof(tpre))))
+            return rc;
+        b.ds = dds;
+        b.tp = dtp;
+        b.ntraj = 1;
+        b.maxN = N;
+        b.tlist = list;
+        b.tprefix = tprefix;
+        b.ncorr = tp.corr ? 1 : 0;
+        b.tblocks = tblocks;
+        k.NP = (N + 15) / 16 * 16;
+        k.list = list;
+        k.prefix = prefix;
+        k.n = 1;
+        k.tiles = 2 * d.n2;
+        k.waves = k.waves_fused = g12::default_waves(k.NP);
+        k.full = N == k.NP ? 1 : 0;
+        return 0;
+    }
+
+    // stage scalars: Y1 = S1, the next stage input S + cn K, A = w K (first) or A + w K
+    static StageRef stage(int first, double wk, double cn) {
+        StageRef r{};
+        r.val.wk = wk;
+        r.val.cn = cn;
+        r.val.first = first;
+        r.val.lam = 1.0;
+        return r;
+    }
+
+    int upload_nat(double2* dst, const double* ref, double2* raw) {  // reference layout -> natural
+        const int rc = up(raw, ref, sizeof(double2) * d.n4);
+        if (!rc) g12::launch_ref_to_nat(raw, dst, d.Ns, d.Nb, s);
+        return rc;
+    }
+    int download_ref(double* ref, const double2* src, double2* raw) {  // natural -> reference layout
+        g12::launch_nat_to_ref(src, raw, d.Ns, d.Nb, s);
+        return down(ref, raw, sizeof(double2) * d.n4);
+    }
+};
+
+// x <- i x for n complex values (exact: a swap and a sign)
+void times_i(double* x, int64_t n) {
+    for (int64_t e = 0; e < n; ++e) {
+        const double re = x[2 * e];
+        x[2 * e] = -x[2 * e + 1];
+        x[2 * e + 1] = re;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1314,20 +1433,23 @@ int g12_hf_hamiltonian(double* h, const double* G1x, const double* G1y, int Ns, 
     if (!h || !G1x || (nV > 0 && !V)) return G12_E_ARG;
     int rc = check_routine_geometry(Ns, Nb, nV);
     if (rc) return rc;
-    const int64_t n2 = (int64_t)Ns * Nb * Ns * Nb;
-    Scratch sc;
-    double2* dh = sc.get<double2>(n2);
-    double2* dg = sc.get<double2>(n2);
-    DEV_OR_FAIL(dh && dg);
-    StageParams sp = interaction_params(Ns, Nb, U, V, nV);
-    sp.t1 = t1;
-    sp.t2 = t2;
-    sp.wq = w;
-    sp.Nq = quench_extent;
-    for (int i = 0; i < Nb; ++i) sp.eps[i] = eps ? eps[i] : 0.0;
-    if ((rc = up(dh, h, sizeof(double2) * n2)) || (rc = up(dg, G1x, sizeof(double2) * n2))) return rc;
-    g12::launch_hf(dh, dg, Ns, Nb, sp, api_stream());
-    return down(h, dh, sizeof(double2) * n2);
+    // the G1 part of the stage set-up (h_HF of Y1 = S1 = G1x)
+    RoutineBatch rb;
+    if ((rc = rb.init(Ns, Nb, U, V, nV, false))) return rc;
+    rb.tp.t1 = t1;
+    rb.tp.t2 = t2;
+    rb.tp.qstr = w;
+    rb.tp.Nq = quench_extent;
+    for (int i = 0; i < Nb; ++i) rb.tp.eps[i] = eps ? eps[i] : 0.0;
+    if ((rc = rb.upload()) || (rc = up(rb.d.S1, G1x, sizeof(double2) * rb.d.n2))) return rc;
+    StageRef sr = RoutineBatch::stage(1, 1.0, 0.0);
+    sr.val.gate = 1;  // quench shift phi * strength with phi = 1: w on sites < quench_extent
+    sr.val.phi = 1.0;
+    g12::launch_b_stage_setup(rb.b, sr, G12_PREC_FP64, rb.s);
+    std::vector<double> hf(2 * rb.d.n2);
+    if ((rc = down(hf.data(), rb.d.h, sizeof(double2) * rb.d.n2))) return rc;
+    for (int64_t e = 0; e < 2 * rb.d.n2; ++e) h[e] += hf[e];  // h accumulates
+    return 0;
 }
 
 int g12_h1_commutator(const double* G, const double* h, double* comm, int N) {
@@ -1370,50 +1492,39 @@ int g12_collision(double* Ix, const double* G2xyxy, const double* G2xxxx, int Ns
     if (!Ix || !G2xyxy || !G2xxxx || (nV > 0 && !V)) return G12_E_ARG;
     int rc = check_routine_geometry(Ns, Nb, nV);
     if (rc) return rc;
-    const int N = Ns * Nb;
-    const int64_t n2 = (int64_t)N * N, n3 = n2 * N, n4 = n2 * n2;
-    Scratch sc;
-    double2 *draw = sc.get<double2>(n4), *dT = sc.get<double2>(n4);
-    double2 *dP = sc.get<double2>(n3), *dQ = sc.get<double2>(n3), *dF = sc.get<double2>(n3);
-    double2* dI = sc.get<double2>(n2);
-    double *wxy = sc.get<double>(n2), *wxx = sc.get<double>(n2);
-    DEV_OR_FAIL(draw && dT && dP && dQ && dF && dI && wxy && wxx);
-    StageParams sp = interaction_params(Ns, Nb, U, V, nV);
-    const double* src[2] = {G2xyxy, G2xxxx};
-    double2* dst[2] = {dP, dQ};
-    for (int s = 0; s < 2; ++s) {
-        if ((rc = up(draw, src[s], sizeof(double2) * n4))) return rc;
-        g12::launch_ref_to_nat(draw, dT, Ns, Nb, api_stream());
-        g12::launch_d_gather(dT, dst[s], N, api_stream());
-    }
-    g12::launch_w_tables(wxy, wxx, Ns, Nb, sp, api_stream());
-    g12::launch_f_tables(dP, dQ, wxy, wxx, dF, nullptr, N, api_stream());
-    g12::launch_collision_from_f(dF, dI, N, api_stream());
-    return down(Ix, dI, sizeof(double2) * n2);
+    // state spins (xxxx, xyxy), their partial traces, and the stage tables:
+    // I[X][Y] = F^xx[X][Y][X]
+    RoutineBatch rb;
+    if ((rc = rb.init(Ns, Nb, U, V, nV, true)) || (rc = rb.upload())) return rc;
+    if ((rc = rb.upload_nat(rb.d.S2[0], G2xxxx, rb.d.Y2[0])) || (rc = rb.upload_nat(rb.d.S2[1], G2xyxy, rb.d.Y2[0])))
+        return rc;
+    g12::launch_b_d_from_state(rb.b, rb.k, G12_PREC_FP64, rb.s);
+    g12::launch_b_stage_setup(rb.b, RoutineBatch::stage(1, 1.0, 0.0), G12_PREC_FP64, rb.s);
+    return down(Ix, rb.d.I, sizeof(double2) * rb.d.n2);
 }
 
 }  // extern "C"
 
+// Psi of the stage G1 = G1 through the production epilogues: the strided pass
+// writes its Psi terms into R (no products), the fused pass adds its own and
+// takes the stage-0 update A = -i (R + sources); Psi = i A.  Spin 0 carries
+// W_xx (Psi_xxxx), spin 1 W_xy (Psi_xyxy).
 static int psi_common(double* psi, const double* G1, int Ns, int Nb, double U, const double* V, int nV,
                       bool same_spin) {
     if (!psi || !G1 || (nV > 0 && !V)) return G12_E_ARG;
     int rc = check_routine_geometry(Ns, Nb, nV);
     if (rc) return rc;
-    const int N = Ns * Nb;
-    const int64_t n2 = (int64_t)N * N, n3 = n2 * N, n4 = n2 * n2;
-    Scratch sc;
-    double2 *dg = sc.get<double2>(n2), *dQ = sc.get<double2>(n3), *dQp = sc.get<double2>(n3);
-    double2 *dnat = sc.get<double2>(n4), *dref = sc.get<double2>(n4);
-    double *wxy = sc.get<double>(n2), *wxx = sc.get<double>(n2);
-    DEV_OR_FAIL(dg && dQ && dQp && dnat && dref && wxy && wxx);
-    StageParams sp = interaction_params(Ns, Nb, U, V, nV);
-    if ((rc = up(dg, G1, sizeof(double2) * n2))) return rc;
-    g12::launch_w_tables(wxy, wxx, Ns, Nb, sp, api_stream());
-    const double* W = same_spin ? wxx : wxy;
-    g12::launch_q_table(dg, W, dQ, dQp, N, api_stream());
-    g12::launch_psi(dg, W, dQ, dQp, dnat, N, api_stream());
-    g12::launch_nat_to_ref(dnat, dref, Ns, Nb, api_stream());
-    return down(psi, dref, sizeof(double2) * n4);
+    RoutineBatch rb;
+    if ((rc = rb.init(Ns, Nb, U, V, nV, true)) || (rc = rb.upload())) return rc;
+    if ((rc = up(rb.d.S1, G1, sizeof(double2) * rb.d.n2))) return rc;
+    const StageRef sr = RoutineBatch::stage(1, 1.0, 0.0);
+    g12::launch_b_stage_setup(rb.b, sr, G12_PREC_FP64, rb.s);
+    g12::launch_b_tile_args(rb.b, rb.k, sr, 0, 0, g12::SRC_PSI, 0, rb.s);
+    g12::launch_b_tile_args(rb.b, rb.k, sr, 1, 0, g12::SRC_PSI, 0, rb.s);
+    const int sp = same_spin ? 0 : 1;
+    if ((rc = rb.download_ref(psi, rb.d.A2[sp], rb.d.R2[sp]))) return rc;
+    times_i(psi, rb.d.n4);
+    return 0;
 }
 
 extern "C" {
@@ -1431,28 +1542,22 @@ int g12_pi(double* pi, const double* G1y, const double* G2xyxy, const double* G2
     if (!pi || !G1y || !G2xyxy || !G2xxxx || (nV > 0 && !V)) return G12_E_ARG;
     int rc = check_routine_geometry(Ns, Nb, nV);
     if (rc) return rc;
-    const int N = Ns * Nb;
-    const int64_t n2 = (int64_t)N * N, n3 = n2 * N, n4 = n2 * n2;
-    Scratch sc;
-    double2 *draw = sc.get<double2>(n4), *dT = sc.get<double2>(n4);
-    double2 *dP = sc.get<double2>(n3), *dQ = sc.get<double2>(n3), *dF = sc.get<double2>(n3);
-    double2* dg = sc.get<double2>(n2);
-    double *wxy = sc.get<double>(n2), *wxx = sc.get<double>(n2);
-    DEV_OR_FAIL(draw && dT && dP && dQ && dF && dg && wxy && wxx);
-    StageParams sp = interaction_params(Ns, Nb, U, V, nV);
-    const double* src[2] = {G2xyxy, G2xxxx};
-    double2* dst[2] = {dP, dQ};
-    for (int s = 0; s < 2; ++s) {
-        if ((rc = up(draw, src[s], sizeof(double2) * n4))) return rc;
-        g12::launch_ref_to_nat(draw, dT, Ns, Nb, api_stream());
-        g12::launch_d_gather(dT, dst[s], N, api_stream());
-    }
-    if ((rc = up(dg, G1y, sizeof(double2) * n2))) return rc;
-    g12::launch_w_tables(wxy, wxx, Ns, Nb, sp, api_stream());
-    g12::launch_f_tables(dP, dQ, wxy, wxx, dF, nullptr, N, api_stream());
-    g12::launch_pi(dg, dF, dT, N, api_stream());
-    g12::launch_nat_to_ref(dT, draw, Ns, Nb, api_stream());
-    return down(pi, draw, sizeof(double2) * n4);
+    // Pi(P, Q) = g22 (F[p1][q1][p2] - F[p1][q1][q2]) with F built from
+    // D^P + D^Q less the diagonal W term on D^Q: the production F table of the
+    // spin whose state is Q.  Q as spin 0, P as spin 1, R = 0, and the fused
+    // pass with the Pi family only: A = -i Pi.
+    RoutineBatch rb;
+    if ((rc = rb.init(Ns, Nb, U, V, nV, true)) || (rc = rb.upload())) return rc;
+    if ((rc = up(rb.d.S1, G1y, sizeof(double2) * rb.d.n2)) ||
+        (rc = rb.upload_nat(rb.d.S2[0], G2xxxx, rb.d.Y2[0])) || (rc = rb.upload_nat(rb.d.S2[1], G2xyxy, rb.d.Y2[0])))
+        return rc;
+    const StageRef sr = RoutineBatch::stage(1, 1.0, 0.0);
+    g12::launch_b_d_from_state(rb.b, rb.k, G12_PREC_FP64, rb.s);
+    g12::launch_b_stage_setup(rb.b, sr, G12_PREC_FP64, rb.s);
+    g12::launch_b_tile_args(rb.b, rb.k, sr, 1, 0, g12::SRC_PI, 0, rb.s);
+    if ((rc = rb.download_ref(pi, rb.d.A2[0], rb.d.R2[0]))) return rc;
+    times_i(pi, rb.d.n4);
+    return 0;
 }
 
 int g12_h2_commutator_noalloc(const double* G2, const double* h1x, const double* h1y, double* comm, int Ns, int Nb,
@@ -1519,24 +1624,21 @@ int g12_g2_step(const double* comm, const double* psi, const double* pi, double*
     if (!comm || !psi || !pi || !K || !RK_result) return G12_E_ARG;
     int rc = check_routine_geometry(Ns, Nb, 0);
     if (rc) return rc;
-    const int N = Ns * Nb;
-    const int64_t n2 = (int64_t)N * N, n4 = n2 * n2;
-    Scratch sc;
-    double2* raw = sc.get<double2>(n4);
-    double2 *dc = sc.get<double2>(n4), *ds = sc.get<double2>(n4), *dp = sc.get<double2>(n4),
-            *dK = sc.get<double2>(n4), *dR = sc.get<double2>(n4);
-    DEV_OR_FAIL(raw && dc && ds && dp && dK && dR);
-    const double* src[4] = {comm, psi, pi, RK_result};
-    double2* dst[4] = {dc, ds, dp, dR};
-    for (int i = 0; i < 4; ++i) {
-        if ((rc = up(raw, src[i], sizeof(double2) * n4))) return rc;
-        g12::launch_ref_to_nat(raw, dst[i], Ns, Nb, api_stream());
-    }
-    g12::launch_g2_step(dc, ds, dp, dK, dR, w, N, api_stream());
-    g12::launch_nat_to_ref(dK, raw, Ns, Nb, api_stream());
-    if ((rc = down(K, raw, sizeof(double2) * n4))) return rc;
-    g12::launch_nat_to_ref(dR, raw, Ns, Nb, api_stream());
-    return down(RK_result, raw, sizeof(double2) * n4);
+    // R = comm + psi + pi - pi^dagger-partner, then the fused epilogue with no
+    // products or sources at a middle stage of weight w: A = RK + w K and the
+    // next stage input S + 1 K = K (S = 0), K = -i R.  Spin 0 carries the
+    // data; spin 1's buffers serve as staging.
+    RoutineBatch rb;
+    if ((rc = rb.init(Ns, Nb, 0.0, nullptr, 0, true)) || (rc = rb.upload())) return rc;
+    DevState& d = rb.d;
+    double2* raw = d.S2[1];
+    if ((rc = rb.upload_nat(d.A2[1], comm, raw)) || (rc = rb.upload_nat(d.R2[1], psi, raw)) ||
+        (rc = rb.upload_nat(d.Y2[1], pi, raw)) || (rc = rb.upload_nat(d.A2[0], RK_result, raw)))
+        return rc;
+    g12::launch_g2_sum(d.A2[1], d.R2[1], d.Y2[1], d.R2[0], d.N, rb.s);
+    g12::launch_b_tile_args(rb.b, rb.k, RoutineBatch::stage(0, w, 1.0), 1, 0, 0, 0, rb.s);
+    if ((rc = rb.download_ref(K, d.Y2[0], raw))) return rc;
+    return rb.download_ref(RK_result, d.A2[0], raw);
 }
 
 // ---------------------------------------------------------------------------

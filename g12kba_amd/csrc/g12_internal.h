@@ -35,7 +35,8 @@ struct TrajParams {
     int nV;
     double U, t1, t2;
     double qstr;       // quench strength
-    double Vprof[G12_MAX_SHELLS];  // V_n = U_eff * Vprof[n-1]
+    double Vprof[G12_MAX_SHELLS];  // V_n = U_eff * Vprof[n-1], or V_n = Vprof[n-1] with v_abs
+    int v_abs;         // routine-level calls: V given as such (also with U = 0)
     double eps[G12_MAX_NB];
 };
 
@@ -127,12 +128,22 @@ struct Bucket {
     int full;                // every listed trajectory has N = NP (no padded tile elements)
 };
 
+// source families of the G2 equation a tile pass adds (see kernels.hip SrcVec)
+constexpr int SRC_PSI = 1;  // Psi: the W and Qp terms
+constexpr int SRC_PI = 2;   // Pi and its partner: the F terms
+
 // batched stage kernels (kernels.hip)
 void launch_b_stage_setup(const BatchDev& b, const StageRef& sr, int precision, hipStream_t s);
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_record(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int fused, int precision, hipStream_t s);
 void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hipStream_t s);
+// FP64 tile pass of a bucket with the given products (hmode: 1 +h.X, 2 -X.h),
+// source families and output mode: the routine-level entries' route through
+// the production epilogues (strided pass: R = [O +] sources; fused pass: the
+// RK update of its stage)
+void launch_b_tile_args(const BatchDev& b, const Bucket& k, const StageRef& r, int fused, int hmode, int sources,
+                        int accumulate, hipStream_t s);
 void launch_b_advance_counter(int* counter, hipStream_t s);
 int device_cu_count();
 int default_waves(int NP);
@@ -151,21 +162,13 @@ int launch_comm2(const double2* T, const double2* hx, const double2* hy, double2
 constexpr int COMM2_DESC_BYTES = 4096;
 int enqueue_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
                   void* desc_area, hipStream_t s);
-void launch_hf(double2* h, const double2* G1, int Ns, int Nb, const StageParams& sp, hipStream_t s);
 void launch_comm1(const double2* G, const double2* h, double2* out, int N, hipStream_t s);
-void launch_d_gather(const double2* T, double2* D, int N, hipStream_t s);
 void launch_pair_diag(const double2* T, double2* out, int N, hipStream_t s);
-void launch_w_tables(double* Wxy, double* Wxx, int Ns, int Nb, const StageParams& sp, hipStream_t s);
-void launch_f_tables(const double2* DP, const double2* DQ, const double* Wxy, const double* Wxx,
-                     double2* F, double2* Ft, int N, hipStream_t s);
-void launch_q_table(const double2* G1, const double* W, double2* Q, double2* Qp, int N, hipStream_t s);
-void launch_collision_from_f(const double2* F, double2* I, int N, hipStream_t s);
-void launch_psi(const double2* G1, const double* W, const double2* Q, const double2* Qp, double2* psi,
-                int N, hipStream_t s);
-void launch_pi(const double2* G1, const double2* F, double2* pi, int N, hipStream_t s);
 void launch_g1_step(const double2* comm, const double2* I, double2* K, double2* RK, double w, int N,
                     hipStream_t s);
-void launch_g2_step(const double2* comm, const double2* psi, const double2* pi, double2* K, double2* RK,
-                    double w, int N, hipStream_t s);
+// out = comm + psi + pi - conj(pi at the particle-swapped transposed element)
+// (natural layout): the G2_step sum (G1_G2_t_step.h:55-86) ahead of the fused
+// epilogue
+void launch_g2_sum(const double2* comm, const double2* psi, const double2* pi, double2* out, int N, hipStream_t s);
 
 }  // namespace g12

@@ -147,50 +147,14 @@ __device__ inline double site_w(int sx, int sz, const StageParams& sp) {  // W_x
 }
 __device__ inline double w_xy(int x, int z, int Ns, const StageParams& sp) { return site_w(x % Ns, z % Ns, sp); }
 
+// ---------------------------------------------------------------------------
 // h = h_HF[G] for one element (X, Y) (h_HFx_quench, HF.h:57-104):
 //   diagonal: eps + quench + Hartree  -iU g_XX - 2i sum_{Z != X} W_xy(X,Z) g_ZZ
 //   -t1 between neighbouring sites of one orbital, -t2 between adjacent
 //   orbitals of one site, onsite Fock +iU g_YX (transposed), long-range Fock
 //   +iV g_XY.
-__device__ inline double2 hf_element(const double2* G, int X, int Y, int Ns, int Nb, const StageParams& sp) {
-    const int N = Ns * Nb;
-    const int sx = X % Ns, ox = X / Ns, sy = Y % Ns, oy = Y / Ns;
-    double2 v = make_double2(0.0, 0.0);
-    if (X == Y) {
-        v.x += sp.eps[ox];
-        if (sx < sp.Nq) v.x += sp.wq;
-        const double2 gxx = G[(int64_t)X * N + X];
-        double2 acc = cscale(gxx, sp.U);  // -i U g_XX
-        double2 lr = make_double2(0.0, 0.0);
-        // Z = site + orb * Ns in ascending order, without a division per term
-        for (int oz = 0, Z = 0; oz < Nb; ++oz)
-            for (int sz = 0; sz < Ns; ++sz, ++Z) {
-                if (Z == X) continue;
-                const double w = site_w(sx, sz, sp);
-                if (w != 0.0) lr = cadd(lr, cscale(G[(int64_t)Z * N + Z], w));
-            }
-        acc = cadd(acc, cscale(lr, 2.0));
-        v = cadd(v, cmul_mi(acc));
-    } else {
-        int ds = sx - sy;
-        ds = ds < 0 ? -ds : ds;
-        if (ox == oy && ds == 1) v.x -= sp.t1;
-        if (sx == sy) {
-            int dorb = ox - oy;
-            dorb = dorb < 0 ? -dorb : dorb;
-            if (dorb == 1) v.x -= sp.t2;
-            v = cadd(v, cmul_i(cscale(G[(int64_t)Y * N + X], sp.U)));
-        } else if (ds <= sp.nV) {
-            v = cadd(v, cmul_i(cscale(G[(int64_t)X * N + Y], sp.V[ds - 1])));
-        }
-    }
-    return v;
-}
-
-// ---------------------------------------------------------------------------
-// h_HF for one element (X, Y) during the stage set-up: the Hartree sums read
-// the diagonal of the stage G1 from LDS, the Fock terms take G1 elements from
-// gv(index).
+// Formed during the stage set-up: the Hartree sums read the diagonal of the
+// stage G1 from LDS, the Fock terms take G1 elements from gv(index).
 template <class GV>
 __device__ inline double2 hf_element_with(GV gv, const double2* diag, int X, int Y, int Ns, int Nb,
                                           const StageParams& sp) {
@@ -237,7 +201,7 @@ __device__ inline StageShared stage_of(const StageRef& r) {
 // together (thread i writes shell i).  Callers synchronise before use.
 __device__ inline void fill_sp(StageParams& sp, const TrajParams& t, const StageShared& ss) {
     const double U = ss.as_phase ? ss.lam * t.U : t.U;  // U_AS = fermi(t) U during switching
-    for (int i = threadIdx.x; i < t.nV; i += blockDim.x) sp.V[i] = U * t.Vprof[i];
+    for (int i = threadIdx.x; i < t.nV; i += blockDim.x) sp.V[i] = t.v_abs ? t.Vprof[i] : U * t.Vprof[i];
     if (threadIdx.x == 0) {
         sp.c = ss.c;
         sp.wk = ss.wk;
@@ -936,7 +900,7 @@ struct BTileArgs {
     int nspin;
     int use_y;        // stage input from the Y buffer (after stage 0, or always when compact; 0: always S)
     int compact;      // R and Y held as complex64 (mixed-precision runs)
-    int sources;      // runs: each pass adds its share of the G2 sources (see SrcVec)
+    int sources;      // source families each pass adds its share of (SRC_PSI | SRC_PI in runs; see SrcVec)
     const double2* h_override;  // routine calls: h to use instead of the trajectory's h
 };
 
@@ -1231,9 +1195,11 @@ struct SrcVec {
     double w0, w1;    // W rows at p2 and q2 (strided only)
 };
 
-// Source vectors of tile t from the buffers of its (trajectory, spin) in v.
+// Source vectors of tile t from the buffers of its (trajectory, spin) in v;
+// `mask` selects the source families (SRC_PSI: the W and Qp terms, SRC_PI:
+// the F terms; runs take both, routine-level calls one).
 template <bool FUSED>
-__device__ __forceinline__ void src_load(const View& v, const TileRef& t, int lane, SrcVec& sv) {
+__device__ __forceinline__ void src_load(const View& v, const TileRef& t, int lane, int mask, SrcVec& sv) {
     const int N = t.N;
     const int t0 = t.tile / N, t1 = t.tile % N;
     const uint32_t n = (uint32_t)N;
@@ -1247,6 +1213,11 @@ __device__ __forceinline__ void src_load(const View& v, const TileRef& t, int la
         sv.w1 = gld1(v.W, (uint32_t)t1 * n + x);
         sv.g = gld(v.G1, (uint32_t)t0 * n + t1);
     }
+    if (!(mask & SRC_PSI)) {
+        sv.q = make_double2(0.0, 0.0);
+        sv.w0 = sv.w1 = 0.0;
+    }
+    if (!(mask & SRC_PI)) sv.f = make_double2(0.0, 0.0);
 }
 
 __device__ __forceinline__ double2 lane_value(double2 v, int src) {
@@ -1324,7 +1295,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     if (a.sources && lead) {
         View v0;
         load_view<FUSED>(v0, a, ss, tr.traj, tr.spin);
-        src_load<FUSED>(v0, tr, lane, sv);
+        src_load<FUSED>(v0, tr, lane, a.sources, sv);
     }
     // one tile; Pc holds its stage input and is refilled with the tile after
     // the ones already in flight
@@ -1531,11 +1502,11 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         }
         if (a.sources && lead && gnext >= 0) {
             if (nt.traj == v.traj && nt.spin == v.spin) {
-                src_load<FUSED>(v, nt, lane, sv);
+                src_load<FUSED>(v, nt, lane, a.sources, sv);
             } else {
                 View vn;
                 load_view<FUSED>(vn, a, ss, nt.traj, nt.spin);
-                src_load<FUSED>(vn, nt, lane, sv);
+                src_load<FUSED>(vn, nt, lane, a.sources, sv);
             }
         }
         gt = gnext;
@@ -1604,7 +1575,7 @@ __global__ __launch_bounds__(WTHR) void k_tile_wide(BTileArgs a) {
         if (a.sources && wave == 0) {  // source vectors, elements x = lane + 64 i
             for (int x = lane; x < N; x += 64) {
                 SrcVec sv;
-                src_load<FUSED>(v, tr, x, sv);
+                src_load<FUSED>(v, tr, x, a.sources, sv);
                 if constexpr (FUSED) {
                     srcv[0][x] = cadd(cmul_i(sv.q), sv.f);
                 } else {
@@ -1938,13 +1909,6 @@ __global__ void k_nat_to_ref(const double2* in, double2* out, int Ns, int Nb) {
 
 // ---------------------------------------------------------------------------
 // Routine-level kernels (reference-routine entry points).
-__global__ void k_hf(double2* h, const double2* G, int Ns, int Nb, StageParams sp) {
-    const int N = Ns * Nb;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= N * N) return;
-    h[e] = cadd(h[e], hf_element(G, e / N, e % N, Ns, Nb, sp));
-}
-
 __global__ void k_comm1(const double2* G, const double2* h, double2* out, int N) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= N * N) return;
@@ -1957,14 +1921,6 @@ __global__ void k_comm1(const double2* G, const double2* h, double2* out, int N)
     out[e] = csub(hg, gh);
 }
 
-__global__ void k_d_gather(const double2* T, double2* D, int N) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t N2 = (int64_t)N * N;
-    if (e >= N2 * N) return;
-    const int X = (int)(e / N2), Y = (int)((e / N) % N), Z = (int)(e % N);
-    D[e] = T[(((int64_t)X * N + Z) * N + Y) * N + Z];
-}
-
 // Pair-diagonal elements M[(X,Y);(X,Y)] = T[X][Y][X][Y] of a G2 tensor in the
 // natural layout (the correlation energy's only input from G2).
 __global__ void k_pair_diag(const double2* T, double2* out, int N) {
@@ -1973,82 +1929,6 @@ __global__ void k_pair_diag(const double2* T, double2* out, int N) {
     if (e >= N2) return;
     const int64_t X = e / N, Y = e % N;
     out[e] = T[((X * N + Y) * N + X) * N + Y];
-}
-
-__global__ void k_w_tables(double* Wxy, double* Wxx, int Ns, int Nb, StageParams sp) {
-    const int N = Ns * Nb;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= N * N) return;
-    const int X = e / N, Y = e % N;
-    const double w = w_xy(X, Y, Ns, sp);
-    Wxy[e] = w;
-    Wxx[e] = X == Y ? 0.0 : w;
-}
-
-// F[X][Y][Z] = -i sum_w ( W_xy[Z][w] DP[X][Y][w] + W_xx[Z][w] DQ[X][Y][w] )
-__global__ void k_f_table(const double2* DP, const double2* DQ, const double* Wxy, const double* Wxx,
-                          double2* F, double2* Ft, int N) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t N2 = (int64_t)N * N;
-    if (e >= N2 * N) return;
-    const int X = (int)(e / N2), Y = (int)((e / N) % N), Z = (int)(e % N);
-    const double2* dp = DP + ((int64_t)X * N + Y) * N;
-    const double2* dq = DQ + ((int64_t)X * N + Y) * N;
-    double2 f = make_double2(0.0, 0.0);
-    for (int w = 0; w < N; ++w) {
-        const double a = Wxy[(int64_t)Z * N + w], b = Wxx[(int64_t)Z * N + w];
-        if (a == 0.0 && b == 0.0) continue;
-        f = cadd(f, cadd(cscale(dp[w], a), cscale(dq[w], b)));
-    }
-    f = cmul_mi(f);
-    F[e] = f;
-    if (Ft) Ft[((int64_t)Z * N + Y) * N + X] = f;
-}
-
-__global__ void k_q_table(const double2* G, const double* W, double2* Q, double2* Qp, int N) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t N2 = (int64_t)N * N;
-    if (e >= N2 * N) return;
-    const int Z = (int)(e / N2), X = (int)((e / N) % N), Y = (int)(e % N);
-    double2 acc = make_double2(0.0, 0.0);
-    for (int w = 0; w < N; ++w) {
-        const double wv = W[(int64_t)Z * N + w];
-        if (wv == 0.0) continue;
-        acc = cadd(acc, cscale(cmul(G[(int64_t)X * N + w], G[(int64_t)w * N + Y]), wv));
-    }
-    Q[e] = acc;
-    if (Qp) Qp[((int64_t)X * N + Y) * N + Z] = acc;
-}
-
-__global__ void k_collision_from_f(const double2* F, double2* I, int N) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= N * N) return;
-    const int X = e / N, Y = e % N;
-    I[e] = F[((int64_t)X * N + Y) * N + X];
-}
-
-// Psi[p1,p2,q1,q2] (natural layout) from the Q tables, as in the fused pass.
-__global__ void k_psi(const double2* G, const double* W, const double2* Q, const double2* Qp, double2* out, int N) {
-    const int64_t N2 = (int64_t)N * N;
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= N2 * N2) return;
-    const int p1 = (int)(e / (N2 * N)), p2 = (int)((e / N2) % N), q1 = (int)((e / N) % N), q2 = (int)(e % N);
-    const double2 g11 = G[(int64_t)p1 * N + q1], g22 = G[(int64_t)p2 * N + q2];
-    const int64_t mn = (int64_t)p2 * N + q2;
-    const double2* Qps = Qp + ((int64_t)p1 * N + q1) * N;
-    double2 psi = cscale(cmul(g11, g22), W[(int64_t)p1 * N + p2] - W[(int64_t)q1 * N + q2]);
-    psi = cadd(psi, cmul_i(cmul(g11, csub(Q[(int64_t)p1 * N2 + mn], Q[(int64_t)q1 * N2 + mn]))));
-    psi = cadd(psi, cmul_i(cmul(g22, csub(Qps[p2], Qps[q2]))));
-    out[e] = psi;
-}
-
-__global__ void k_pi(const double2* G, const double2* F, double2* out, int N) {
-    const int64_t N2 = (int64_t)N * N;
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= N2 * N2) return;
-    const int p1 = (int)(e / (N2 * N)), p2 = (int)((e / N2) % N), q1 = (int)((e / N) % N), q2 = (int)(e % N);
-    const double2* Frow = F + ((int64_t)p1 * N + q1) * N;
-    out[e] = cmul(G[(int64_t)p2 * N + q2], csub(Frow[p2], Frow[q2]));
 }
 
 __global__ void k_g1_step(const double2* comm, const double2* I, double2* K, double2* RK, double w, int N) {
@@ -2060,18 +1940,16 @@ __global__ void k_g1_step(const double2* comm, const double2* I, double2* K, dou
     RK[e] = cadd(RK[e], cscale(k, w));
 }
 
-// natural layout: the particle-swapped transposed partner of (p1,p2,q1,q2)
-// is (q2,q1,p2,p1)
-__global__ void k_g2_step(const double2* comm, const double2* psi, const double2* pi, double2* K, double2* RK,
-                          double w, int N) {
+// G2_step sum (G1_G2_t_step.h:55-86) of the natural layout, whose
+// particle-swapped transposed partner of (p1,p2,q1,q2) is (q2,q1,p2,p1):
+// out = comm + psi + pi - conj(pi[partner])
+__global__ void k_g2_sum(const double2* comm, const double2* psi, const double2* pi, double2* out, int N) {
     const int64_t N2 = (int64_t)N * N;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= N2 * N2) return;
     const int p1 = (int)(e / (N2 * N)), p2 = (int)((e / N2) % N), q1 = (int)((e / N) % N), q2 = (int)(e % N);
     const int64_t t = (((int64_t)q2 * N + q1) * N + p2) * N + p1;
-    const double2 k = cmul_mi(csub(cadd(cadd(comm[e], psi[e]), pi[e]), cconj(pi[t])));
-    K[e] = k;
-    RK[e] = cadd(RK[e], cscale(k, w));
+    out[e] = csub(cadd(cadd(comm[e], psi[e]), pi[e]), cconj(pi[t]));
 }
 
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -2141,9 +2019,27 @@ void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fu
     a.nspin = 2;
     a.use_y = 1;
     a.compact = precision != G12_PREC_FP64;
-    a.sources = 1;
+    a.sources = SRC_PSI | SRC_PI;
     a.h_override = nullptr;
     launch_tile(a, precision, s);
+}
+
+void launch_b_tile_args(const BatchDev& b, const Bucket& k, const StageRef& r, int fused, int hmode, int sources,
+                        int accumulate, hipStream_t s) {
+    BTileArgs a{};
+    a.b = b;
+    a.k = k;
+    a.r = r;
+    a.tiling = fused ? 1 : 0;
+    a.fused = fused;
+    a.hmode = hmode;
+    a.accumulate = accumulate;
+    a.nspin = 2;
+    a.use_y = 1;
+    a.compact = 0;
+    a.sources = sources;
+    a.h_override = nullptr;
+    launch_tile(a, G12_PREC_FP64, s);
 }
 
 void launch_ref_to_nat(const double2* in, double2* out, int Ns, int Nb, hipStream_t s) {
@@ -2229,53 +2125,12 @@ int launch_comm2(const double2* T, const double2* hx, const double2* hy, double2
     return (launched == 0 && e == hipSuccess) ? 0 : G12_E_HIP;
 }
 
-void launch_hf(double2* h, const double2* G1, int Ns, int Nb, const StageParams& sp, hipStream_t s) {
-    const int n2 = Ns * Nb * Ns * Nb;
-    hipLaunchKernelGGL(k_hf, dim3(blocks_for(n2, 256)), dim3(256), 0, s, h, G1, Ns, Nb, sp);
-}
-
 void launch_comm1(const double2* G, const double2* h, double2* out, int N, hipStream_t s) {
     hipLaunchKernelGGL(k_comm1, dim3(blocks_for((int64_t)N * N, 256)), dim3(256), 0, s, G, h, out, N);
 }
 
-void launch_d_gather(const double2* T, double2* D, int N, hipStream_t s) {
-    const int64_t n3 = (int64_t)N * N * N;
-    hipLaunchKernelGGL(k_d_gather, dim3(blocks_for(n3, 256)), dim3(256), 0, s, T, D, N);
-}
-
 void launch_pair_diag(const double2* T, double2* out, int N, hipStream_t s) {
     hipLaunchKernelGGL(k_pair_diag, dim3(blocks_for((int64_t)N * N, 256)), dim3(256), 0, s, T, out, N);
-}
-
-void launch_w_tables(double* Wxy, double* Wxx, int Ns, int Nb, const StageParams& sp, hipStream_t s) {
-    const int n2 = Ns * Nb * Ns * Nb;
-    hipLaunchKernelGGL(k_w_tables, dim3(blocks_for(n2, 256)), dim3(256), 0, s, Wxy, Wxx, Ns, Nb, sp);
-}
-
-void launch_f_tables(const double2* DP, const double2* DQ, const double* Wxy, const double* Wxx, double2* F,
-                     double2* Ft, int N, hipStream_t s) {
-    const int64_t n3 = (int64_t)N * N * N;
-    hipLaunchKernelGGL(k_f_table, dim3(blocks_for(n3, 256)), dim3(256), 0, s, DP, DQ, Wxy, Wxx, F, Ft, N);
-}
-
-void launch_q_table(const double2* G1, const double* W, double2* Q, double2* Qp, int N, hipStream_t s) {
-    const int64_t n3 = (int64_t)N * N * N;
-    hipLaunchKernelGGL(k_q_table, dim3(blocks_for(n3, 256)), dim3(256), 0, s, G1, W, Q, Qp, N);
-}
-
-void launch_collision_from_f(const double2* F, double2* I, int N, hipStream_t s) {
-    hipLaunchKernelGGL(k_collision_from_f, dim3(blocks_for((int64_t)N * N, 256)), dim3(256), 0, s, F, I, N);
-}
-
-void launch_psi(const double2* G1, const double* W, const double2* Q, const double2* Qp, double2* psi, int N,
-                hipStream_t s) {
-    const int64_t n4 = (int64_t)N * N * N * N;
-    hipLaunchKernelGGL(k_psi, dim3(blocks_for(n4, 256)), dim3(256), 0, s, G1, W, Q, Qp, psi, N);
-}
-
-void launch_pi(const double2* G1, const double2* F, double2* pi, int N, hipStream_t s) {
-    const int64_t n4 = (int64_t)N * N * N * N;
-    hipLaunchKernelGGL(k_pi, dim3(blocks_for(n4, 256)), dim3(256), 0, s, G1, F, pi, N);
 }
 
 void launch_g1_step(const double2* comm, const double2* I, double2* K, double2* RK, double w, int N,
@@ -2283,10 +2138,9 @@ void launch_g1_step(const double2* comm, const double2* I, double2* K, double2* 
     hipLaunchKernelGGL(k_g1_step, dim3(blocks_for((int64_t)N * N, 256)), dim3(256), 0, s, comm, I, K, RK, w, N);
 }
 
-void launch_g2_step(const double2* comm, const double2* psi, const double2* pi, double2* K, double2* RK, double w,
-                    int N, hipStream_t s) {
+void launch_g2_sum(const double2* comm, const double2* psi, const double2* pi, double2* out, int N, hipStream_t s) {
     const int64_t n4 = (int64_t)N * N * N * N;
-    hipLaunchKernelGGL(k_g2_step, dim3(blocks_for(n4, 256)), dim3(256), 0, s, comm, psi, pi, K, RK, w, N);
+    hipLaunchKernelGGL(k_g2_sum, dim3(blocks_for(n4, 256)), dim3(256), 0, s, comm, psi, pi, out, N);
 }
 
 }  // namespace g12

@@ -74,3 +74,30 @@ def test_h2_commutator_n128_product_state():
         want += outer(hx @ ar - ar @ hy, br)
         want += outer(ar, hx @ br - br @ hy)
     assert relerr(got, want) < 1e-12
+
+
+def test_routines_n72_match_factorised_oracle():
+    """Psi (both spins), Pi (both call orders) and the collision integral at
+    N = 72 through the routine entries, which run the production table and
+    epilogue code, against the factorised numpy oracle (pinned against the
+    element-wise oracle on the CPU), to 1e-12 relative."""
+    from oracle import fast
+    Ns, Nb, N = 36, 2, 72
+    print("wide routines", N, flush=True)
+    rng = np.random.default_rng(72)
+    G1 = rand_c(rng, N, N)
+    V = rng.uniform(0.1, 1.0, 3)
+    U = 1.1
+    Wxy, Wxx = fast.w_tables(Ns, Nb, U, V)
+    for W, fn in ((Wxx, g.Psi_xxxx), (Wxy, g.Psi_xyxy)):
+        want = fast.psi_term(G1, W, fast.q_table(G1, W))
+        assert relerr(fast.ref_to_nat(fn(G1, Ns, Nb, U, V), Ns, Nb), want) < 1e-12
+        del want
+    P, Q = rand_c(rng, N * N, N * N, scale=1e-2), rand_c(rng, N * N, N * N, scale=1e-2)
+    DP, DQ = fast.d_table(fast.ref_to_nat(P, Ns, Nb)), fast.d_table(fast.ref_to_nat(Q, Ns, Nb))
+    for A, B, DA, DB in ((P, Q, DP, DQ), (Q, P, DQ, DP)):  # both call orders (RK_solver.h:163-164)
+        want = fast.pi_term(G1, fast.f_table(DA, DB, Wxy, Wxx))
+        assert relerr(fast.ref_to_nat(g.Pi_xyxy(G1, A, B, U, Ns, Nb, V), Ns, Nb), want) < 1e-12
+        del want
+    F = fast.f_table(DP, DQ, Wxy, Wxx)  # collision_int(xyxy = P, xxxx = Q)
+    assert relerr(g.collision_int(P, Q, Ns, Nb, U, V), F[np.arange(N), :, np.arange(N)]) < 1e-12
