@@ -1506,9 +1506,10 @@ int g12_collision(double* Ix, const double* G2xyxy, const double* G2xxxx, int Ns
 }  // extern "C"
 
 // Psi of the stage G1 = G1 through the production epilogues: the strided pass
-// writes its Psi terms into R (no products), the fused pass adds its own and
-// takes the stage-0 update A = -i (R + sources); Psi = i A.  Spin 0 carries
-// W_xx (Psi_xxxx), spin 1 W_xy (Psi_xyxy).
+// writes its source terms into R (no products), the fused pass adds its own
+// and takes the stage-0 update A = -i (R + sources); Psi = i A.  G2 = 0, so
+// the partial traces and with them the F tables vanish and the sources are
+// Psi's alone.  Spin 0 carries W_xx (Psi_xxxx), spin 1 W_xy (Psi_xyxy).
 static int psi_common(double* psi, const double* G1, int Ns, int Nb, double U, const double* V, int nV,
                       bool same_spin) {
     if (!psi || !G1 || (nV > 0 && !V)) return G12_E_ARG;
@@ -1519,8 +1520,8 @@ static int psi_common(double* psi, const double* G1, int Ns, int Nb, double U, c
     if ((rc = up(rb.d.S1, G1, sizeof(double2) * rb.d.n2))) return rc;
     const StageRef sr = RoutineBatch::stage(1, 1.0, 0.0);
     g12::launch_b_stage_setup(rb.b, sr, G12_PREC_FP64, rb.s);
-    g12::launch_b_tile_args(rb.b, rb.k, sr, 0, 0, g12::SRC_PSI, 0, rb.s);
-    g12::launch_b_tile_args(rb.b, rb.k, sr, 1, 0, g12::SRC_PSI, 0, rb.s);
+    g12::launch_b_tile_args(rb.b, rb.k, sr, 0, 0, 1, 0, rb.s);
+    g12::launch_b_tile_args(rb.b, rb.k, sr, 1, 0, 1, 0, rb.s);
     const int sp = same_spin ? 0 : 1;
     if ((rc = rb.download_ref(psi, rb.d.A2[sp], rb.d.R2[sp]))) return rc;
     times_i(psi, rb.d.n4);
@@ -1544,8 +1545,9 @@ int g12_pi(double* pi, const double* G1y, const double* G2xyxy, const double* G2
     if (rc) return rc;
     // Pi(P, Q) = g22 (F[p1][q1][p2] - F[p1][q1][q2]) with F built from
     // D^P + D^Q less the diagonal W term on D^Q: the production F table of the
-    // spin whose state is Q.  Q as spin 0, P as spin 1, R = 0, and the fused
-    // pass with the Pi family only: A = -i Pi.
+    // spin whose state is Q.  Q as spin 0, P as spin 1, R = 0, the Qp and W
+    // tables cleared after the set-up (so the sources are the F terms alone),
+    // and the fused pass: A = -i Pi.
     RoutineBatch rb;
     if ((rc = rb.init(Ns, Nb, U, V, nV, true)) || (rc = rb.upload())) return rc;
     if ((rc = up(rb.d.S1, G1y, sizeof(double2) * rb.d.n2)) ||
@@ -1554,7 +1556,12 @@ int g12_pi(double* pi, const double* G1y, const double* G2xyxy, const double* G2
     const StageRef sr = RoutineBatch::stage(1, 1.0, 0.0);
     g12::launch_b_d_from_state(rb.b, rb.k, G12_PREC_FP64, rb.s);
     g12::launch_b_stage_setup(rb.b, sr, G12_PREC_FP64, rb.s);
-    g12::launch_b_tile_args(rb.b, rb.k, sr, 1, 0, g12::SRC_PI, 0, rb.s);
+    DevState& d = rb.d;
+    for (void* z : {(void*)d.Qp[0], (void*)d.Qp[1]})
+        if (hipMemsetAsync(z, 0, sizeof(double2) * d.n3, rb.s) != hipSuccess) return G12_E_HIP;
+    for (void* z : {(void*)d.Wxy, (void*)d.Wxx})
+        if (hipMemsetAsync(z, 0, sizeof(double) * d.n2, rb.s) != hipSuccess) return G12_E_HIP;
+    g12::launch_b_tile_args(rb.b, rb.k, sr, 1, 0, 1, 0, rb.s);
     if ((rc = rb.download_ref(pi, rb.d.A2[0], rb.d.R2[0]))) return rc;
     times_i(pi, rb.d.n4);
     return 0;

@@ -128,10 +128,6 @@ struct Bucket {
     int full;                // every listed trajectory has N = NP (no padded tile elements)
 };
 
-// source families of the G2 equation a tile pass adds (see kernels.hip SrcVec)
-constexpr int SRC_PSI = 1;  // Psi: the W and Qp terms
-constexpr int SRC_PI = 2;   // Pi and its partner: the F terms
-
 // batched stage kernels (kernels.hip)
 void launch_b_stage_setup(const BatchDev& b, const StageRef& sr, int precision, hipStream_t s);
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& sr, hipStream_t s);
@@ -139,9 +135,9 @@ void launch_b_record(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int fused, int precision, hipStream_t s);
 void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hipStream_t s);
 // FP64 tile pass of a bucket with the given products (hmode: 1 +h.X, 2 -X.h),
-// source families and output mode: the routine-level entries' route through
-// the production epilogues (strided pass: R = [O +] sources; fused pass: the
-// RK update of its stage)
+// sources (0/1) and output mode: the routine-level entries' route through the
+// production epilogues (strided pass: R = [O +] sources; fused pass: the RK
+// update of its stage)
 void launch_b_tile_args(const BatchDev& b, const Bucket& k, const StageRef& r, int fused, int hmode, int sources,
                         int accumulate, hipStream_t s);
 void launch_b_advance_counter(int* counter, hipStream_t s);

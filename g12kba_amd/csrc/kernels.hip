@@ -900,7 +900,7 @@ struct BTileArgs {
     int nspin;
     int use_y;        // stage input from the Y buffer (after stage 0, or always when compact; 0: always S)
     int compact;      // R and Y held as complex64 (mixed-precision runs)
-    int sources;      // source families each pass adds its share of (SRC_PSI | SRC_PI in runs; see SrcVec)
+    int sources;      // runs: each pass adds its share of the G2 sources (see SrcVec)
     const double2* h_override;  // routine calls: h to use instead of the trajectory's h
 };
 
@@ -1195,11 +1195,9 @@ struct SrcVec {
     double w0, w1;    // W rows at p2 and q2 (strided only)
 };
 
-// Source vectors of tile t from the buffers of its (trajectory, spin) in v;
-// `mask` selects the source families (SRC_PSI: the W and Qp terms, SRC_PI:
-// the F terms; runs take both, routine-level calls one).
+// Source vectors of tile t from the buffers of its (trajectory, spin) in v.
 template <bool FUSED>
-__device__ __forceinline__ void src_load(const View& v, const TileRef& t, int lane, int mask, SrcVec& sv) {
+__device__ __forceinline__ void src_load(const View& v, const TileRef& t, int lane, SrcVec& sv) {
     const int N = t.N;
     const int t0 = t.tile / N, t1 = t.tile % N;
     const uint32_t n = (uint32_t)N;
@@ -1213,11 +1211,6 @@ __device__ __forceinline__ void src_load(const View& v, const TileRef& t, int la
         sv.w1 = gld1(v.W, (uint32_t)t1 * n + x);
         sv.g = gld(v.G1, (uint32_t)t0 * n + t1);
     }
-    if (!(mask & SRC_PSI)) {
-        sv.q = make_double2(0.0, 0.0);
-        sv.w0 = sv.w1 = 0.0;
-    }
-    if (!(mask & SRC_PI)) sv.f = make_double2(0.0, 0.0);
 }
 
 __device__ __forceinline__ double2 lane_value(double2 v, int src) {
@@ -1295,7 +1288,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     if (a.sources && lead) {
         View v0;
         load_view<FUSED>(v0, a, ss, tr.traj, tr.spin);
-        src_load<FUSED>(v0, tr, lane, a.sources, sv);
+        src_load<FUSED>(v0, tr, lane, sv);
     }
     // one tile; Pc holds its stage input and is refilled with the tile after
     // the ones already in flight
@@ -1502,11 +1495,11 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         }
         if (a.sources && lead && gnext >= 0) {
             if (nt.traj == v.traj && nt.spin == v.spin) {
-                src_load<FUSED>(v, nt, lane, a.sources, sv);
+                src_load<FUSED>(v, nt, lane, sv);
             } else {
                 View vn;
                 load_view<FUSED>(vn, a, ss, nt.traj, nt.spin);
-                src_load<FUSED>(vn, nt, lane, a.sources, sv);
+                src_load<FUSED>(vn, nt, lane, sv);
             }
         }
         gt = gnext;
@@ -1575,7 +1568,7 @@ __global__ __launch_bounds__(WTHR) void k_tile_wide(BTileArgs a) {
         if (a.sources && wave == 0) {  // source vectors, elements x = lane + 64 i
             for (int x = lane; x < N; x += 64) {
                 SrcVec sv;
-                src_load<FUSED>(v, tr, x, a.sources, sv);
+                src_load<FUSED>(v, tr, x, sv);
                 if constexpr (FUSED) {
                     srcv[0][x] = cadd(cmul_i(sv.q), sv.f);
                 } else {
@@ -2019,7 +2012,7 @@ void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fu
     a.nspin = 2;
     a.use_y = 1;
     a.compact = precision != G12_PREC_FP64;
-    a.sources = SRC_PSI | SRC_PI;
+    a.sources = 1;
     a.h_override = nullptr;
     launch_tile(a, precision, s);
 }
