@@ -29,10 +29,9 @@ METRIC = "G2 RK4 steps/s at N=64 (FP64 vs mixed @≤1e-6 G1 err); batched traj/s
 # stages), measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
 # and calibrated with a plain copy kernel (bytes read = 2 x FETCH_SIZE KiB,
 # written = WRITE_SIZE KiB); TRAFFIC_SOURCE names the summary each comes from.
-TRAFFIC_BYTES = {("fp64", "tile_fused"): 3.01e9, ("fp64", "tile_strided"): 1.10e9,
-                 ("fp32_3m", "tile_fused"): 2.50e9, ("fp32_3m", "tile_strided"): 0.557e9,
-                 ("bf16x3", "tile_fused"): 2.34e9, ("bf16x3", "tile_strided"): 0.571e9}
-
+TRAFFIC_BYTES = {("fp64", "tile_fused"): 2.995e9, ("fp64", "tile_strided"): 1.099e9,
+                 ("fp32_3m", "tile_fused"): 2.455e9, ("fp32_3m", "tile_strided"): 0.556e9,
+                 ("bf16x3", "tile_fused"): 2.328e9, ("bf16x3", "tile_strided"): 0.574e9}
 
 TRAFFIC_SOURCE = {"fp64": "profiles/tile_kernels_n64_fp64_hbm_traffic.txt",
                   "fp32_3m": "profiles/tile_kernels_n64_mixed_hbm_traffic.txt",
