@@ -107,6 +107,8 @@ def _load():
         "g12_batch_save_state": [vp, i, ctypes.c_char_p],
         "g12_batch_load_state": [vp, i, ctypes.c_char_p],
         "g12_configure": [vp, i, i],
+        "g12_pair_symmetric": [vp, ctypes.POINTER(i)],
+        "g12_batch_pair_symmetric": [vp, ctypes.POINTER(i)],
         "g12_batch_create": [ctypes.POINTER(Params), i, ctypes.POINTER(vp)],
         "g12_batch_destroy": [vp],
         "g12_batch_size": [vp, ctypes.POINTER(i)],
@@ -165,7 +167,7 @@ EXPORTED = [
     "g12_hf_hamiltonian", "g12_h1_commutator", "g12_h2_commutator", "g12_h2_commutator_noalloc", "g12_collision",
     "g12_psi_xxxx", "g12_psi_xyxy", "g12_pi", "g12_g1_step", "g12_g2_step", "g12_fermi",
     "g12_dipole", "g12_energy", "g12_state_energy", "g12_batch_state_energy", "g12_write_outputs", "g12_save_state", "g12_load_state", "g12_batch_save_state", "g12_batch_load_state", "g12_version",
-    "g12_configure", "g12_batch_create", "g12_batch_destroy", "g12_batch_size",
+    "g12_configure", "g12_pair_symmetric", "g12_batch_pair_symmetric", "g12_batch_create", "g12_batch_destroy", "g12_batch_size",
     "g12_batch_memory_bytes", "g12_batch_set_state", "g12_batch_get_state",
     "g12_batch_adiabatic_switching", "g12_batch_rk4", "g12_batch_rk4_advance", "g12_batch_rk4_record", "g12_batch_rk4_record_collision", "g12_batch_guard",
     "g12_batch_timing_enable", "g12_batch_timing_get", "g12_batch_configure", "g12_batch_tune",
@@ -582,6 +584,13 @@ class Propagator:
         a = -1 if autotune is None else int(bool(autotune))
         _check(lib.g12_configure(self.h, g, a), "g12_configure")
 
+    def pair_symmetric(self):
+        """True when the context runs the pair-symmetric path (one tile-product
+        pass per stage; the state is pair-symmetric and Hermitian)."""
+        on = ctypes.c_int(0)
+        _check(lib.g12_pair_symmetric(self.h, ctypes.byref(on)), "g12_pair_symmetric")
+        return bool(on.value)
+
 
 class Batch:
     """Many independent trajectories advanced together (one set of launches per
@@ -705,6 +714,12 @@ class Batch:
 
     def tune(self):
         _check(lib.g12_batch_tune(self.h), "g12_batch_tune")
+
+    def pair_symmetric(self):
+        """Propagator.pair_symmetric of the batch (every correlated trajectory)."""
+        on = ctypes.c_int(0)
+        _check(lib.g12_batch_pair_symmetric(self.h, ctypes.byref(on)), "g12_batch_pair_symmetric")
+        return bool(on.value)
 
     def save(self, i, path):
         """Binary checkpoint of trajectory i (with the batch's time and step)."""

@@ -267,6 +267,15 @@ struct Run {
     bool graphs = true;
     bool autotune = true;
     bool tuned = false;
+    bool tune_verbose = false;  // G12KBA_TUNE_VERBOSE=1: the autotuner prints every candidate's time
+    // pair-symmetric path (k_sym_update): on while every correlated state is
+    // pair-symmetric and Hermitian with G1 anti-Hermitian (G2 = 0 is) and every
+    // tile size is at most 64; G12KBA_PAIR_SYMMETRY=0 keeps the general path
+    bool sym = false;
+    bool sym_allowed = true;
+    std::vector<char> sym_ok;          // per trajectory: its uploaded state qualifies
+    std::vector<int64_t> sym_prefix;   // update blocks before each correlated trajectory
+    int* d_flag = nullptr;             // device flag of the symmetry check
     hipGraphExec_t gx[2] = {nullptr, nullptr};  // one step: 0 adiabatic switching, 1 production
     void* arena = nullptr;
     int64_t bytes = 0;
@@ -336,6 +345,8 @@ int64_t layout(Run& r, char* base) {
     }
     r.b.tlist = cv.take<int>((int64_t)r.tab_list.size());
     r.b.tprefix = cv.take<int64_t>((int64_t)r.tab_prefix.size());
+    r.b.sprefix = cv.take<int64_t>((int64_t)r.sym_prefix.size());
+    r.d_flag = cv.take<int>(1);
     return cv.off;
 }
 
@@ -348,6 +359,21 @@ void destroy_run(Run& r) {
     if (r.h_table) hipHostFree(r.h_table);
     if (r.h_fail) hipHostFree(r.h_fail);
     if (r.stream) hipStreamDestroy(r.stream);
+}
+
+void drop_graphs(Run& r);
+
+// Chooses the pair-symmetric path when every correlated trajectory's state
+// qualifies and every tile size is at most SYM_MAX_N; a change of path drops
+// the captured steps and the tuned launch shapes.
+void update_sym(Run& r) {
+    bool on = r.sym_allowed && r.any_corr;
+    for (const Bucket& k : r.buckets) on = on && k.NP <= g12::SYM_MAX_N;
+    for (int i = 0; i < r.ntraj && on; ++i) on = !r.tp[i].corr || r.sym_ok[i];
+    if (on == r.sym) return;
+    r.sym = on;
+    drop_graphs(r);
+    r.tuned = false;
 }
 
 int create_run(Run& r, const g12_params* p, int ntraj) {
@@ -436,6 +462,12 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
             r.tab_list.push_back(i);
             r.tab_prefix.push_back(r.tab_prefix.back() + (int64_t)N * g12::table_blocks_per_x(N));
         }
+    // pair-symmetric update: 2 spins x N (N + 1) / 2 slice pairs x bands of 16 rows
+    r.sym_prefix.assign(1, 0);
+    for (int i : r.tab_list) {
+        const int64_t N = r.ds[i].N;
+        r.sym_prefix.push_back(r.sym_prefix.back() + 2 * (N * (N + 1) / 2) * ((N + 15) / 16));
+    }
     const int64_t total = layout(r, nullptr);
     if (hipMalloc(&r.arena, total) != hipSuccess) {
         r.arena = nullptr;
@@ -463,6 +495,8 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
                               hipMemcpyHostToDevice, r.stream));
         HIPCHK(hipMemcpyAsync(const_cast<int64_t*>(r.b.tprefix), r.tab_prefix.data(),
                               sizeof(int64_t) * r.tab_prefix.size(), hipMemcpyHostToDevice, r.stream));
+        HIPCHK(hipMemcpyAsync(const_cast<int64_t*>(r.b.sprefix), r.sym_prefix.data(),
+                              sizeof(int64_t) * r.sym_prefix.size(), hipMemcpyHostToDevice, r.stream));
     }
     HIPCHK(hipStreamSynchronize(r.stream));
     r.b.ds = r.d_ds;
@@ -471,10 +505,17 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
     r.b.maxN = r.maxN;
     r.b.ncorr = (int)r.tab_list.size();
     r.b.tblocks = r.tab_prefix.back();
+    r.b.sunits = r.sym_prefix.back();
     const char* e = std::getenv("G12KBA_GRAPHS");
     if (e && e[0] == '0') r.graphs = false;
     e = std::getenv("G12KBA_AUTOTUNE");
     if (e && e[0] == '0') r.autotune = false;
+    e = std::getenv("G12KBA_TUNE_VERBOSE");
+    if (e && e[0] == '1') r.tune_verbose = true;
+    e = std::getenv("G12KBA_PAIR_SYMMETRY");
+    if (e && e[0] == '0') r.sym_allowed = false;
+    r.sym_ok.assign(ntraj, 1);  // every state starts at G1 = G2 = 0
+    update_sym(r);
     return 0;
 }
 
@@ -497,11 +538,21 @@ void timed(Run& r, int cls, F&& launch) {
 // strided commutator pass with the sources that are vectors over p1 and q1
 // (Psi W and Q terms, Pi^+), G1 right-hand side and update, fused commutator
 // pass with the remaining sources (Psi Qp term, Pi) and the G2 update.
+// Pair-symmetric path: one product pass over the contiguous tiles (half of the
+// commutator and of the sources), then the update pass that adds the
+// pair-exchanged half (k_sym_update); timing classes 0 and 1 are these two.
 void run_stage(Run& r, const StageRef& sr) {
     hipStream_t s = r.stream;
     // G1 set-up and the tables (from the stage G1 and the partial traces D of
     // the stage input; both tile passes add their share of the sources from them)
     timed(r, 2, [&] { g12::launch_b_stage_setup(r.b, sr, r.precision, s); });
+    if (r.sym) {
+        for (const Bucket& k : r.buckets)
+            timed(r, 0, [&] { g12::launch_b_tile(r.b, k, sr, 0, r.precision, s, 1); });
+        timed(r, 2, [&] { g12::launch_b_g1_rhs(r.b, sr, s); });
+        timed(r, 1, [&] { g12::launch_b_sym_update(r.b, sr, r.precision, s); });
+        return;
+    }
     for (const Bucket& k : r.buckets) timed(r, 0, [&] { g12::launch_b_tile(r.b, k, sr, 0, r.precision, s); });
     timed(r, 2, [&] { g12::launch_b_g1_rhs(r.b, sr, s); });
     for (const Bucket& k : r.buckets) timed(r, 1, [&] { g12::launch_b_tile(r.b, k, sr, 1, r.precision, s); });
@@ -555,7 +606,7 @@ void drop_graphs(Run& r) {
 int tune(Run& r) {
     if (r.tuned || !r.autotune) return 0;
     // (waves, workgroups-per-CU cap) of the strided and the fused pass
-    static std::map<std::tuple<int, int, int64_t>, std::array<int, 4>> cache;
+    static std::map<std::tuple<int, int, int64_t, int>, std::array<int, 4>> cache;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -563,7 +614,7 @@ int tune(Run& r) {
     sr.val = stage_block(0, 0.0);
     int rc = 0;
     for (Bucket& k : r.buckets) {
-        const auto key = std::make_tuple(k.NP, r.precision, k.tiles);
+        const auto key = std::make_tuple(k.NP, r.precision, k.tiles, r.sym ? 1 : 0);
         auto it = cache.find(key);
         if (it != cache.end()) {
             k.waves = it->second[0];
@@ -577,16 +628,18 @@ int tune(Run& r) {
         // fewer resident workgroups give each more tiles to overlap loads with
         static const int caps[3] = {0, 2, 1};
         std::array<int, 4> best = {k.waves, k.wg_per_cu, k.waves_fused, k.wg_per_cu_fused};
-        for (int fused = 0; fused < 2 && !rc; ++fused) {
+        // the pair-symmetric path has one tile pass (unfused, contiguous tiles)
+        for (int fused = 0; fused < (r.sym ? 1 : 2) && !rc; ++fused) {
             float best_ms = 1e30f;
             for (int o = 0; o < nopt && !rc; ++o)
                 for (int cap : caps) {
                     Bucket c = k;
                     c.waves = c.waves_fused = opts[o];
                     c.wg_per_cu = c.wg_per_cu_fused = cap;
-                    g12::launch_b_tile(r.b, c, sr, fused, r.precision, r.stream);  // warm-up
+                    g12::launch_b_tile(r.b, c, sr, fused, r.precision, r.stream, r.sym);  // warm-up
                     hipEventRecord(e0, r.stream);
-                    for (int rep = 0; rep < 2; ++rep) g12::launch_b_tile(r.b, c, sr, fused, r.precision, r.stream);
+                    for (int rep = 0; rep < 2; ++rep)
+                        g12::launch_b_tile(r.b, c, sr, fused, r.precision, r.stream, r.sym);
                     hipEventRecord(e1, r.stream);
                     if (hipEventSynchronize(e1) != hipSuccess) {
                         rc = G12_E_HIP;
@@ -594,6 +647,9 @@ int tune(Run& r) {
                     }
                     float ms = 0.f;
                     hipEventElapsedTime(&ms, e0, e1);
+                    if (r.tune_verbose)
+                        std::fprintf(stderr, "g12kba tune: NP=%d %s waves=%d wg_cap=%d %.4f ms\n", k.NP,
+                                     r.sym ? "products" : fused ? "fused" : "strided", opts[o], cap, ms / 2);
                     if (ms < best_ms) {
                         best_ms = ms;
                         best[2 * fused] = opts[o];
@@ -838,9 +894,35 @@ int set_state(Run& r, int i, const double* G1, const double* G2xx, const double*
         if (e == hipSuccess) g12::launch_ref_to_nat(tmp, d.S2[s], d.Ns, d.Nb, r.stream);
     }
     if (e == hipSuccess) e = hipMemsetAsync(d.fail, 0, sizeof(int), r.stream);
+    // does the state qualify for the pair-symmetric path?  G1 anti-Hermitian and
+    // each G2 tensor pair-symmetric, to 1e-12 of its largest element, and in
+    // FP64 runs Hermitian as well (mixed-precision runs build the F tables from
+    // the Hermitian part of D, which is all the exchange of Pi needs)
+    int flag = 0;
+    if (r.tp[i].corr && e == hipSuccess) {
+        e = hipMemsetAsync(r.d_flag, 0, sizeof(int), r.stream);
+        for (int sp = 0; sp < 2 && e == hipSuccess; ++sp) {
+            if (!g2[sp]) continue;
+            double mx = 0.0;
+            for (int64_t k = 0; k < 2 * d.n4; ++k) mx = std::max(mx, std::fabs(g2[sp][k]));
+            g12::launch_pair_check(d.S2[sp], d.N, 1e-12 * mx, r.precision == G12_PREC_FP64, r.d_flag, r.stream);
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(&flag, r.d_flag, sizeof(int), hipMemcpyDeviceToHost, r.stream);
+    }
     const hipError_t es = hipStreamSynchronize(r.stream);
     if (tmp) (void)hipFree(tmp);
     if (e != hipSuccess || es != hipSuccess) return G12_E_HIP;
+    if (r.tp[i].corr) {
+        double mx = 0.0, asym = 0.0;
+        for (int64_t k = 0; k < 2 * d.n2; ++k) mx = std::max(mx, std::fabs(G1[k]));
+        for (int x = 0; x < d.N; ++x)
+            for (int y = 0; y < d.N; ++y) {  // G1 + G1^dagger
+                const int64_t xy = 2 * ((int64_t)x * d.N + y), yx = 2 * ((int64_t)y * d.N + x);
+                asym = std::max(asym, std::max(std::fabs(G1[xy] + G1[yx]), std::fabs(G1[xy + 1] - G1[yx + 1])));
+            }
+        r.sym_ok[i] = (flag == 0 && asym <= 1e-12 * mx) ? 1 : 0;
+        update_sym(r);
+    }
     r.h_fail[i] = 0;
     r.t = 0.0;
     r.steps = 0;
@@ -1043,6 +1125,12 @@ int g12_timing_get(g12_ctx* c, int which, int64_t* launches, double* total_ms) {
     return c ? timing_get(c->r, which, launches, total_ms) : G12_E_ARG;
 }
 
+int g12_pair_symmetric(g12_ctx* c, int* on) {
+    if (!c || !on) return G12_E_ARG;
+    *on = c->r.sym ? 1 : 0;
+    return 0;
+}
+
 int g12_configure(g12_ctx* c, int use_graphs, int autotune) {
     return c ? configure(c->r, use_graphs, autotune) : G12_E_ARG;
 }
@@ -1139,6 +1227,12 @@ int g12_batch_configure(g12_batch* b, int use_graphs, int autotune) {
 }
 
 int g12_batch_tune(g12_batch* b) { return b ? tune(b->r) : G12_E_ARG; }
+
+int g12_batch_pair_symmetric(g12_batch* b, int* on) {
+    if (!b || !on) return G12_E_ARG;
+    *on = b->r.sym ? 1 : 0;
+    return 0;
+}
 
 int g12_batch_buckets(g12_batch* b, int cap, int* NP, int* count, int* waves_strided, int* waves_fused) {
     if (!b) return G12_E_ARG;

@@ -113,6 +113,10 @@ struct BatchDev {
     const int64_t* tprefix;  // device array [ncorr + 1]
     int ncorr;
     int64_t tblocks;
+    // pair-symmetric update (k_sym_update): prefix of its blocks over the
+    // correlated trajectories (2 spins x N (N + 1) / 2 slice pairs x bands of 16 rows)
+    const int64_t* sprefix;  // device array [ncorr + 1]
+    int64_t sunits;
 };
 
 struct Bucket {
@@ -132,7 +136,17 @@ struct Bucket {
 void launch_b_stage_setup(const BatchDev& b, const StageRef& sr, int precision, hipStream_t s);
 void launch_b_g1_rhs(const BatchDev& b, const StageRef& sr, hipStream_t s);
 void launch_b_record(const BatchDev& b, const StageRef& sr, hipStream_t s);
-void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int fused, int precision, hipStream_t s);
+// tile pass of a bucket: strided (fused = 0), fused (1), or with sym = 1 the
+// contiguous product pass of the pair-symmetric path
+void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int fused, int precision, hipStream_t s,
+                   int sym = 0);
+// second pass of the pair-symmetric path: C = R + P(R) and the RK update
+void launch_b_sym_update(const BatchDev& b, const StageRef& sr, int precision, hipStream_t s);
+// flag |= 1 unless the natural-layout tensor T is pair-symmetric (and, with
+// herm, Hermitian) within tol
+void launch_pair_check(const double2* T, int N, double tol, int herm, int* flag, hipStream_t s);
+// largest N of the pair-symmetric path
+constexpr int SYM_MAX_N = 64;
 void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hipStream_t s);
 // FP64 tile pass of a bucket with the given products (hmode: 1 +h.X, 2 -X.h),
 // sources (0/1) and output mode: the routine-level entries' route through the

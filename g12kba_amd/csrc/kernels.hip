@@ -901,6 +901,7 @@ struct BTileArgs {
     int use_y;        // stage input from the Y buffer (after stage 0, or always when compact; 0: always S)
     int compact;      // R and Y held as complex64 (mixed-precision runs)
     int sources;      // runs: each pass adds its share of the G2 sources (see SrcVec)
+    int sym;          // pair-symmetric path: contiguous tiles, half of every source (see k_sym_update)
     const double2* h_override;  // routine calls: h to use instead of the trajectory's h
 };
 
@@ -1196,7 +1197,9 @@ struct SrcVec {
 };
 
 // Source vectors of tile t from the buffers of its (trajectory, spin) in v.
-template <bool FUSED>
+// SYM (pair-symmetric path, contiguous tiles): the fused pass's F row and
+// the W rows and g11 of the strided form.
+template <bool FUSED, bool SYM = false>
 __device__ __forceinline__ void src_load(const View& v, const TileRef& t, int lane, SrcVec& sv) {
     const int N = t.N;
     const int t0 = t.tile / N, t1 = t.tile % N;
@@ -1205,6 +1208,11 @@ __device__ __forceinline__ void src_load(const View& v, const TileRef& t, int la
     sv.q = gld(v.Qp, ((uint32_t)t0 * n + t1) * n + x);
     if constexpr (FUSED) {
         sv.f = gld(v.F, ((uint32_t)t0 * n + t1) * n + x);
+    } else if constexpr (SYM) {
+        sv.f = gld(v.F, ((uint32_t)t0 * n + t1) * n + x);
+        sv.w0 = gld1(v.W, (uint32_t)t0 * n + x);
+        sv.w1 = gld1(v.W, (uint32_t)t1 * n + x);
+        sv.g = gld(v.G1, (uint32_t)t0 * n + t1);
     } else {
         sv.f = gld(v.F, ((uint32_t)t1 * n + t0) * n + x);
         sv.w0 = gld1(v.W, (uint32_t)t0 * n + x);  // W is symmetric
@@ -1219,8 +1227,15 @@ __device__ __forceinline__ double2 lane_value(double2 v, int src) {
 
 // FULL: every tile of the launch has N = NP, so the element bounds checks and
 // the divisions by N drop out at compile time.
-template <int NP, int NW, int PREC, bool FUSED, bool FULL>
+// SYM (unfused only): pass 1 of the pair-symmetric path.  Contiguous tiles
+// X[p2][q2] at fixed (p1, q1) = (t0, t1); the output R = h.X - X.h +
+// g22 (ea[p2] - eb[q2]) with e = i Qp[t0][t1][.] + F[t0][t1][.] and
+// ea = e + (g11/2) W[t0][.], eb = e + (g11/2) W[t1][.]: the fused pass's
+// terms and half of the Psi term g11 g22 (W[p1][p2] - W[q1][q2]); the
+// update kernel adds the pair-exchanged R (k_sym_update).
+template <int NP, int NW, int PREC, bool FUSED, bool FULL, bool SYM = false>
 __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
+    static_assert(!(FUSED && SYM), "the pair-symmetric products run unfused");
     typedef Engine<NP, NW, PREC> E;
     __shared__ typename E::Lds L;
     // source vectors of the current tile, one element per row/column index:
@@ -1231,7 +1246,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     __shared__ double2 srcv[FUSED ? 1 : 3][NP];
     constexpr int NT = TileCfg<NP, NW>::NT, TT = TileCfg<NP, NW>::TT, PW = TileCfg<NP, NW>::PW;
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
-    constexpr int TILING = FUSED ? 1 : -1;
+    constexpr int TILING = (FUSED || SYM) ? 1 : -1;
     constexpr bool ALLB = (TT % NW) == 0;                 // every wave owns PW output blocks
     constexpr bool EXACT = FULL && (NP * NP) % NTHR == 0;  // every prefetch slot holds a tile element
     const StageShared ss = uni_stage(a.r);
@@ -1288,7 +1303,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     if (a.sources && lead) {
         View v0;
         load_view<FUSED>(v0, a, ss, tr.traj, tr.spin);
-        src_load<FUSED>(v0, tr, lane, sv);
+        src_load<FUSED, SYM>(v0, tr, lane, sv);
     }
     // one tile; Pc holds its stage input and is refilled with the tile after
     // the ones already in flight
@@ -1329,6 +1344,10 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         if (SRC_LDS && a.sources && lead && lane < N) {
             if constexpr (FUSED) {
                 srcv[0][lane] = cadd(cmul_i(sv.q), sv.f);
+            } else if constexpr (SYM) {
+                const double2 e = cadd(cmul_i(sv.q), sv.f), hg = cscale(sv.g, 0.5);
+                srcv[0][lane] = cadd(e, cscale(hg, sv.w0));
+                srcv[1][lane] = cadd(e, cscale(hg, sv.w1));
             } else {
                 srcv[0][lane] = cadd(cscale(sv.g, sv.w0), cmul_i(sv.q));
                 srcv[1][lane] = cadd(cscale(sv.g, sv.w1), cmul_i(sv.q));
@@ -1413,7 +1432,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                     double2 bn{}, cn{};
                     if (a.sources) {
                         bn = srcv[1][n];
-                        cn = srcv[2][n];
+                        if constexpr (!SYM) cn = srcv[2][n];
                     }
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
@@ -1421,15 +1440,19 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                         double2 am{}, cm{};
                         if (a.sources) {
                             am = srcv[0][m];
-                            cm = srcv[2][m];
+                            if constexpr (!SYM) cm = srcv[2][m];
                         }
                         if (inside(t, m, n)) {
                             const uint32_t off = (C && BLK_R) ? gc.blk(m, n) : gn.nat(m, n);
                             double2 val = make_double2(ar[u][j], ai[u][j]);
                             if (a.sources) {
                                 const uint32_t gi = (uint32_t)(m * N + n);
-                                const double2 g11 = gld(v.G1, gi), gt = gld(v.G1t, gi);
-                                val = cadd(val, csub(cmul(g11, csub(am, bn)), cmul(gt, csub(cn, cm))));
+                                if constexpr (SYM) {  // g22 (ea[p2] - eb[q2])
+                                    val = cadd(val, cmul(gld(v.G1, gi), csub(am, bn)));
+                                } else {
+                                    const double2 g11 = gld(v.G1, gi), gt = gld(v.G1t, gi);
+                                    val = cadd(val, csub(cmul(g11, csub(am, bn)), cmul(gt, csub(cn, cm))));
+                                }
                             }
                             if (a.accumulate) val = cadd(cld(out, off, C), val);
                             cst(out, off, val, C);
@@ -1495,11 +1518,11 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         }
         if (a.sources && lead && gnext >= 0) {
             if (nt.traj == v.traj && nt.spin == v.spin) {
-                src_load<FUSED>(v, nt, lane, sv);
+                src_load<FUSED, SYM>(v, nt, lane, sv);
             } else {
                 View vn;
                 load_view<FUSED>(vn, a, ss, nt.traj, nt.spin);
-                src_load<FUSED>(vn, nt, lane, sv);
+                src_load<FUSED, SYM>(vn, nt, lane, sv);
             }
         }
         gt = gnext;
@@ -1771,6 +1794,197 @@ __global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pair-symmetric path, second pass.  A physical G2 tensor is symmetric under
+// the exchange of both pairs, T[p1][p2][q1][q2] = T[p2][p1][q2][q1], and
+// Hermitian as an N^2 x N^2 matrix, with G1 anti-Hermitian (every state
+// evolved from G2 = 0 is).  Its right-hand side is then pair-symmetric too:
+//   [H2, M] + Psi + Pi - Pi^+ = X + P(X),   P(X)[p1][p2][q1][q2] = X[p2][p1][q2][q1],
+// with X what the SYM tile pass writes (k_tile): the p1, q1 terms of [H2, M]
+// are the exchanged p2, q2 terms, the Psi term i g11 (Qp[p2][q2][p1] -
+// Qp[p2][q2][q1]) the exchanged g22 term, Pi^+ the exchanged Pi (F is
+// Hermitian in its first two indices when G2 is Hermitian), and the Psi term
+// g11 g22 (W[p1][p2] - W[q1][q2]) is its own exchange, half of it in X.
+// One block per pair of slices {(a, m), (m, a)}, a <= m, and band of 16 rows:
+// slice (a, m) takes its band rows plus the band columns of slice (m, a)
+// transposed, slice (m, a) the reverse (a = m: the band rows only).  Both
+// pieces go through LDS, so every global access runs along rows (complex128)
+// or along the 4 x 4 blocks (complex64).  Then the fused pass's update:
+// K = -i C, A += w K, next stage input Y = S + c' K (the last stage:
+// S += dt (A + w K) / 6 and Y its complex64 copy), and the partial traces D
+// of the next stage input.
+constexpr int SU_BAND = 16, SU_THREADS = 256, SU_MAXN = 64;
+
+// slice pairs (a <= m) in order of m: index m (m + 1) / 2 + a
+__device__ __forceinline__ void su_pair(int64_t pi, int& a, int& m) {
+    int mm = (int)((sqrt(8.0 * (double)pi + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(mm + 1) * (mm + 2) / 2 <= pi) ++mm;
+    while ((int64_t)mm * (mm + 1) / 2 > pi) --mm;
+    m = mm;
+    a = (int)(pi - (int64_t)mm * (mm + 1) / 2);
+}
+
+// Elements of the band rows q1 in [r0, r0 + nr) (ROWS) or the band columns
+// q2 in [r0, r0 + nr) (!ROWS) of one slice, enumerated in memory order of the
+// layout: e -> (q1, q2, offset within the tensor), false for padding.
+template <bool C, bool ROWS>
+struct SuIter {
+    uint32_t base;  // slice start: (p1 N + p2) Nq^2 (blocked) or (p1 N + p2) N^2
+    int N, nq, r0, nr;
+    __device__ int count() const {
+        const int nb = (nr + 3) >> 2;  // block rows / columns of the band
+        if constexpr (C) return ROWS ? nb * 4 * nq : (nq >> 2) * nb * 16;
+        return ROWS ? nr * N : N * SU_BAND;
+    }
+    __device__ bool at(int e, int& q1, int& q2, uint32_t& off) const {
+        if constexpr (C) {
+            const int nb = (nr + 3) >> 2;
+            if constexpr (ROWS) {
+                const int rg = e / (4 * nq), w0 = e - rg * 4 * nq, w = w0 & 15;
+                q1 = r0 + 4 * rg + (w >> 2);
+                q2 = 4 * (w0 >> 4) + (w & 3);
+                off = base + (uint32_t)((r0 >> 2) + rg) * 4 * nq + w0;
+            } else {
+                const int B1 = e / (nb * 16), w0 = e - B1 * nb * 16, w = w0 & 15;
+                q1 = 4 * B1 + (w >> 2);
+                q2 = r0 + 4 * (w0 >> 4) + (w & 3);
+                off = base + (uint32_t)B1 * 4 * nq + (uint32_t)((r0 >> 2) + (w0 >> 4)) * 16 + w;
+            }
+            return ROWS ? (q1 < r0 + nr && q2 < N) : (q1 < N && q2 < r0 + nr);
+        } else {
+            if constexpr (ROWS) {
+                q1 = r0 + e / N;
+                q2 = e - (q1 - r0) * N;
+            } else {
+                q1 = e / SU_BAND;
+                q2 = r0 + (e - q1 * SU_BAND);
+            }
+            off = base + (uint32_t)q1 * N + q2;
+            return ROWS || q2 - r0 < nr;
+        }
+    }
+};
+
+template <bool C>
+__global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef r) {
+    typedef typename std::conditional<C, fv2, dv2>::type LT;
+    __shared__ LT la[SU_BAND][SU_MAXN + 1];  // slice (a, m): band rows [q1 - r0][q2]
+    __shared__ LT lb[SU_MAXN][SU_BAND + 1];  // slice (m, a): band columns [q1][q2 - r0]
+    const StageShared ss = uni_stage(r);
+    const int64_t u = blockIdx.x;
+    int lo = 0, hi = b.ncorr - 1;
+    while (lo < hi) {  // last correlated trajectory whose units start at or before u
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.sprefix[mid] <= u) lo = mid; else hi = mid - 1;
+    }
+    const int traj = uni(b.tlist[lo]);
+    const DevState& d = b.ds[traj];
+    if (!active(d, b.tp[traj], ss)) return;
+    const int N = uni(b.tp[traj].N);
+    const int nbd = (N + SU_BAND - 1) / SU_BAND;
+    const int64_t per_spin = (int64_t)N * (N + 1) / 2 * nbd;
+    int64_t local = u - uni64(b.sprefix[lo]);
+    const int spin = local >= per_spin ? 1 : 0;
+    local -= spin * per_spin;
+    int a, m;
+    su_pair(local / nbd, a, m);
+    const int r0 = (int)(local % nbd) * SU_BAND, nr = min(SU_BAND, N - r0);
+    const double2* X = uni_ptr((const double2*)d.R2[spin]);
+    double2* S = uni_ptr(d.S2[spin]);
+    double2* A = uni_ptr(d.A2[spin]);
+    double2* Y = uni_ptr(d.Y2[spin]);
+    double2* D = uni_ptr(d.D[spin]);
+    const int nq = (N + 3) & ~3;
+    const uint32_t sl = C ? (uint32_t)(nq * nq) : (uint32_t)(N * N);
+    const SuIter<C, true> rows{(uint32_t)(a * N + m) * sl, N, nq, r0, nr};   // slice (a, m)
+    const SuIter<C, false> cols{(uint32_t)(m * N + a) * sl, N, nq, r0, nr};  // slice (m, a)
+    auto xload = [&](uint32_t off) -> LT {
+        if constexpr (C)
+            return *(const GLOBAL fv2*)((const GLOBAL char*)X + (off << 3));
+        else
+            return *(const GLOBAL dv2*)((const GLOBAL char*)X + (off << 4));
+    };
+    // every band holds at most SU_BAND x SU_MAXN elements: IT per thread
+    constexpr int IT = SU_BAND * SU_MAXN / SU_THREADS;
+    static_assert(IT * SU_THREADS == SU_BAND * SU_MAXN, "whole items per thread");
+    auto fill = [&](const auto& it, auto&& put) {
+        int x1[IT], x2[IT];
+        bool ok[IT];
+        LT v[IT];
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int e = threadIdx.x + i * SU_THREADS;
+            uint32_t o = 0;
+            ok[i] = e < it.count() && it.at(e, x1[i], x2[i], o);
+            if (ok[i]) v[i] = xload(o);
+        }
+#pragma unroll
+        for (int i = 0; i < IT; ++i)
+            if (ok[i]) put(x1[i], x2[i], v[i]);
+    };
+    fill(rows, [&](int q1, int q2, LT v) { la[q1 - r0][q2] = v; });
+    fill(cols, [&](int q1, int q2, LT v) { lb[q1][q2 - r0] = v; });
+    __syncthreads();
+    auto tod = [](LT v) { return make_double2((double)v.x, (double)v.y); };
+    // the fused pass's update of the elements (p1, p2, q1, q2) of one slice
+    // with C = R + P(R) from cval(q1, q2); the loads of S and A are issued
+    // for all of a thread's elements first
+    auto update = [&](const auto& it, int p1, int p2, auto&& cval) {
+        int x1[IT], x2[IT];
+        uint32_t o[IT], yo[IT];
+        bool ok[IT];
+        double2 sv[IT], av[IT];
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int e = threadIdx.x + i * SU_THREADS;
+            ok[i] = e < it.count() && it.at(e, x1[i], x2[i], yo[i]);
+            o[i] = ok[i] ? (((uint32_t)p1 * N + p2) * N + x1[i]) * N + x2[i] : 0u;  // S, A (natural)
+            sv[i] = av[i] = make_double2(0.0, 0.0);
+            if (ok[i]) {
+                sv[i] = gld(S, o[i]);
+                if (!ss.first) av[i] = gld(A, o[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            if (!ok[i]) continue;
+            const uint32_t yoff = C ? yo[i] : o[i];
+            const double2 K = cmul_mi(cval(x1[i], x2[i]));
+            const double2 wK = cscale(K, ss.wk);
+            double2 next;
+            if (!ss.last) {
+                gst(A, o[i], ss.first ? wK : cadd(av[i], wK));
+                next = cadd(sv[i], cscale(K, ss.cn));
+                cst(Y, yoff, next, C);
+            } else {
+                const double2 acc = cadd(av[i], wK);
+                next = cadd(sv[i], cdiv6(cscale(acc, ss.dt)));
+                gst(S, o[i], next);
+                if constexpr (C) cst(Y, yoff, next, true);  // next step's stage-0 input
+            }
+            if (x2[i] == p2) gst(D, ((uint32_t)p1 * N + x1[i]) * N + p2, next);  // D[p1][q1][p2]
+        }
+    };
+    update(rows, a, m, [&](int q1, int q2) { return cadd(tod(la[q1 - r0][q2]), tod(lb[q2][q1 - r0])); });
+    if (a != m)
+        update(cols, m, a, [&](int q1, int q2) { return cadd(tod(lb[q1][q2 - r0]), tod(la[q2 - r0][q1])); });
+}
+
+// flag |= 1 unless T is pair-symmetric (and, with herm, Hermitian) within tol,
+// elementwise
+__global__ void k_pair_check(const double2* T, int N, double tol, int herm, int* flag) {
+    const int64_t n = (int64_t)N * N, n4 = n * n;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n4) return;
+    const int q2 = (int)(e % N), q1 = (int)((e / N) % N), p2 = (int)((e / n) % N), p1 = (int)(e / (n * N));
+    const double2 v = gload(T, e);
+    const double2 vp = gload(T, (((int64_t)p2 * N + p1) * N + q2) * N + q1);
+    const double2 vh = gload(T, (((int64_t)q1 * N + q2) * N + p1) * N + p2);
+    const double dp = fmax(fabs(v.x - vp.x), fabs(v.y - vp.y));
+    const double dh = herm ? fmax(fabs(v.x - vh.x), fabs(v.y + vh.y)) : 0.0;
+    if (!(dp <= tol && dh <= tol)) atomicOr(flag, 1);
+}
+
 int device_cus() {
     static int cus = 0;
     if (!cus) {
@@ -1801,6 +2015,9 @@ void launch_tile_cfg(const BTileArgs& a, hipStream_t s) {
     constexpr bool FULL_FUSED = FULL && PREC == G12_PREC_FP32_3M;
     if (a.fused)
         hipLaunchKernelGGL((k_tile<NP, NW, PREC, true, FULL_FUSED>), dim3((unsigned)grid),
+                           dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
+    else if (a.sym)
+        hipLaunchKernelGGL((k_tile<NP, NW, PREC, false, FULL, true>), dim3((unsigned)grid),
                            dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
     else
         hipLaunchKernelGGL((k_tile<NP, NW, PREC, false, FULL>), dim3((unsigned)grid), dim3(TileCfg<NP, NW>::NTHR), 0,
@@ -1996,7 +2213,21 @@ void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hi
     hipLaunchKernelGGL(k_d_from_state, dim3((unsigned)(k.tiles / 2)), dim3(64), 0, s, a);
 }
 
-void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fused, int precision, hipStream_t s) {
+void launch_b_sym_update(const BatchDev& b, const StageRef& r, int precision, hipStream_t s) {
+    if (b.sunits <= 0) return;
+    if (precision != G12_PREC_FP64)
+        hipLaunchKernelGGL(k_sym_update<true>, dim3((unsigned)b.sunits), dim3(SU_THREADS), 0, s, b, r);
+    else
+        hipLaunchKernelGGL(k_sym_update<false>, dim3((unsigned)b.sunits), dim3(SU_THREADS), 0, s, b, r);
+}
+
+void launch_pair_check(const double2* T, int N, double tol, int herm, int* flag, hipStream_t s) {
+    const int64_t n4 = (int64_t)N * N * N * N;
+    hipLaunchKernelGGL(k_pair_check, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, s, T, N, tol, herm, flag);
+}
+
+void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fused, int precision, hipStream_t s,
+                   int sym) {
     BTileArgs a{};
     a.b = b;
     a.k = k;
@@ -2005,7 +2236,8 @@ void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fu
         a.k.wg_per_cu = k.wg_per_cu_fused;
     }
     a.r = r;
-    a.tiling = fused ? 1 : 0;
+    a.sym = (!fused && sym) ? 1 : 0;
+    a.tiling = (fused || a.sym) ? 1 : 0;
     a.fused = fused;
     a.hmode = 3;
     a.accumulate = 0;

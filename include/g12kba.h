@@ -202,6 +202,17 @@ int g12_timing_get(g12_ctx* ctx, int which, int64_t* launches, double* total_ms)
  * an option unchanged.  No reference counterpart (launch configuration). */
 int g12_configure(g12_ctx* ctx, int use_graphs, int autotune);
 
+/* *on = 1 when the context runs the pair-symmetric path: its correlated state
+ * is symmetric under the exchange of both pairs with G1 anti-Hermitian, in
+ * FP64 runs also Hermitian as an N^2 x N^2 matrix (every state evolved from
+ * G2 = 0 is; checked to 1e-12 of the largest element whenever a state is set
+ * or loaded), and N <= 64.  The G2 right-hand side is then the contiguous-tile half of the
+ * commutator and sources plus its pair exchange: one tile-product pass instead
+ * of two.  Otherwise (or with G12KBA_PAIR_SYMMETRY=0 in the environment at
+ * creation) the general two-pass path runs.  No reference counterpart: the
+ * reference evaluates all four commutator terms (commutators.h:53-99). */
+int g12_pair_symmetric(g12_ctx* ctx, int* on);
+
 /* Waits for all work on the library's GPU (device-wide synchronisation). */
 int g12_synchronize(void);
 
@@ -248,6 +259,7 @@ int g12_batch_timing_enable(g12_batch* b, int on);
 int g12_batch_timing_get(g12_batch* b, int which, int64_t* launches, double* total_ms);
 int g12_batch_configure(g12_batch* b, int use_graphs, int autotune);
 int g12_batch_tune(g12_batch* b);
+int g12_batch_pair_symmetric(g12_batch* b, int* on);
 int g12_batch_buckets(g12_batch* b, int cap, int* NP, int* count, int* waves_strided, int* waves_fused);
 /* g12_state_energy of trajectory i. */
 int g12_batch_state_energy(g12_batch* b, int i, double* E);

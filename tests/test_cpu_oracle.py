@@ -232,3 +232,69 @@ def test_planted_mistakes_are_caught():
     # a sign flip of the onsite Fock term is detected
     assert rel(O.hf(z["h0"], z["G1"], Ns, Nb, -float(z["U"]), float(z["t1"]), float(z["t2"]), float(z["w"]),
                     z["eps"], z["V"], int(z["nq"])), z["hf"]) > 1e-3
+
+
+def _physical_run_state(Ns, Nb, U, V, steps=20, dt=0.05, seed=1):
+    """G1 (anti-Hermitian, a filled random band) and both G2 tensors after
+    `steps` Euler steps of the oracle's derivative from G2 = 0: a state with
+    the symmetries every run keeps (G2 Hermitian, pair-symmetric)."""
+    N = Ns * Nb
+    rng = np.random.default_rng(seed)
+    H = rand_c(rng, N, N)
+    H = H + H.conj().T
+    _, v = np.linalg.eigh(H)
+    G1 = 1j * (v[:, : N // 2] @ v[:, : N // 2].conj().T)
+    Txx = np.zeros((N,) * 4, complex)
+    Txy = np.zeros_like(Txx)
+    for _ in range(steps):
+        K1, Kxx, Kxy, _h = fast.derivative(G1, Txx, Txy, Ns, Nb, U, 1.0, 0.4, 0.3, None, V, 1)
+        G1, Txx, Txy = G1 + dt * K1, Txx + dt * Kxx, Txy + dt * Kxy
+    return G1, Txx, Txy
+
+
+def _pair_symmetric_rhs(G1, T, F, W, h, half=0.5):
+    """The pair-symmetric path's right-hand side of one G2 tensor: X + P(X)
+    with X the contiguous-tile (p2, q2) commutator terms plus
+    g22 (ea[p2] - eb[q2]), e = i Qp[p1][q1][.] + F[p1][q1][.],
+    ea = e + half g11 W[p1][.], eb = e + half g11 W[q1][.]."""
+    Qp = fast.q_table(G1, W).transpose(1, 2, 0)
+    X = np.einsum("bk,akcd->abcd", h, T) - np.einsum("abck,kd->abcd", T, h)
+    e = 1j * Qp + F
+    ea = e + half * G1[:, :, None] * W[:, None, :]
+    eb = e + half * G1[:, :, None] * W.T[None, :, :]
+    # X[p1,p2,q1,q2] += g[p2,q2] (ea[p1,q1,p2] - eb[p1,q1,q2])
+    X = X + G1[None, :, None, :] * (ea.transpose(0, 2, 1)[:, :, :, None] - eb[:, None, :, :])
+    return X + X.transpose(1, 0, 3, 2)
+
+
+@pytest.mark.parametrize("Ns,Nb,U,V", [(3, 2, 1.5, [0.75, 0.5]), (4, 1, 2.0, []), (2, 3, 1.0, [0.5])])
+def test_pair_symmetric_decomposition(Ns, Nb, U, V):
+    """The mathematics of the GPU's pair-symmetric path: for a state with the
+    run symmetries, the full G2 derivative (commutators.h:53-99, psi.h,
+    pi.h, G1_G2_t_step.h:55-86 as the oracle states them) equals
+    -i (X + P(X)) built from the contiguous-tile half alone.  Dropping the
+    half weight of the W term, or a state without the symmetry, fails it."""
+    G1, Txx, Txy = _physical_run_state(Ns, Nb, U, V)
+    for T in (Txx, Txy):
+        s = np.abs(T).max()
+        assert s > 1e-4
+        assert np.abs(T - T.transpose(1, 0, 3, 2)).max() < 1e-14 * s * 100
+    K1, Kxx, Kxy, h = fast.derivative(G1, Txx, Txy, Ns, Nb, U, 1.0, 0.4, 0.3, None, V, 1)
+    Wxy, Wxx = fast.w_tables(Ns, Nb, U, V)
+    Dxx, Dxy = fast.d_table(Txx), fast.d_table(Txy)
+    Fxx = fast.f_table(Dxy, Dxx, Wxy, Wxx)
+    Fxy = fast.f_table(Dxx, Dxy, Wxy, Wxx)
+    for T, F, W, K in ((Txx, Fxx, Wxx, Kxx), (Txy, Fxy, Wxy, Kxy)):
+        assert np.abs(-1j * _pair_symmetric_rhs(G1, T, F, W, h) - K).max() < 1e-13 * np.abs(K).max()
+        # planted mistake: the whole W term in X counts it twice (W_xx = 0 on
+        # one site per orbital without long-range shells)
+        if np.abs(W).max() > 0:
+            assert np.abs(-1j * _pair_symmetric_rhs(G1, T, F, W, h, half=1.0) - K).max() > 1e-3 * np.abs(K).max()
+    # a state without the exchange symmetry: the decomposition does not hold
+    rng = np.random.default_rng(9)
+    N = Ns * Nb
+    A = rand_c(rng, N * N, N * N, scale=1e-2)
+    Tn = (A + A.conj().T).reshape(N, N, N, N)
+    _, Kn, _, _ = fast.derivative(G1, Tn, Txy, Ns, Nb, U, 1.0, 0.4, 0.3, None, V, 1)
+    Fn = fast.f_table(fast.d_table(Txy), fast.d_table(Tn), Wxy, Wxx)
+    assert np.abs(-1j * _pair_symmetric_rhs(G1, Tn, Fn, Wxx, h) - Kn).max() > 1e-3 * np.abs(Kn).max()
