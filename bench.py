@@ -7,13 +7,15 @@ interactions V_n = U/(1+n), U = 1.5, t1 = 1, t2 = 0.5, full quench of strength
 (2^24 complex entries each) hold a small seeded Hermitian, pair-exchange
 symmetric perturbation.
 A "step" is one RK4 step (4 stages) of the production protocol; inputs are
-resident in HBM before the timed region.
+resident in HBM before the timed region.  The state is pair-symmetric and
+Hermitian, so the library runs its pair-symmetric path (one tile-product pass
+and one pair-exchange update pass per stage); `config.path` records the path.
 
 Prints one JSON line (bench contract).  `value` times the steps as a run
 executes them (one captured hipGraph per step); `roofline` covers the dominant
-kernel (the two-particle commutator tile passes), timed with HIP events on the
-library's stream over a second run of as many steps; `cpu_baseline` times the
-project's faithful-complexity C++ restatement (OpenMP) on a bounded sample.
+kernel, timed with HIP events on the library's stream over a second run of as
+many steps, and `passes` both G2 passes; `cpu_baseline` times the project's
+faithful-complexity C++ restatement (OpenMP) on a bounded sample.
 """
 import argparse
 import json
@@ -25,17 +27,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "G2 RK4 steps/s at N=64 (FP64 vs mixed @≤1e-6 G1 err); batched traj/s"
-# HBM bytes per launch of each tile pass (read + written, averaged over the four
+# HBM bytes per launch of each G2 pass (read + written, averaged over the four
 # stages), measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
-# and calibrated with a plain copy kernel (bytes read = 2 x FETCH_SIZE KiB,
-# written = WRITE_SIZE KiB); TRAFFIC_SOURCE names the summary each comes from.
+# and calibrated with a plain copy kernel; TRAFFIC_SOURCE names the summary
+# each comes from.  Passes without a measurement report null.
 TRAFFIC_BYTES = {("fp64", "tile_fused"): 2.995e9, ("fp64", "tile_strided"): 1.099e9,
                  ("fp32_3m", "tile_fused"): 2.455e9, ("fp32_3m", "tile_strided"): 0.556e9,
                  ("bf16x3", "tile_fused"): 2.328e9, ("bf16x3", "tile_strided"): 0.574e9}
 
-TRAFFIC_SOURCE = {"fp64": "profiles/tile_kernels_n64_fp64_hbm_traffic.txt",
-                  "fp32_3m": "profiles/tile_kernels_n64_mixed_hbm_traffic.txt",
-                  "bf16x3": "profiles/tile_kernels_n64_mixed_hbm_traffic.txt"}
+TRAFFIC_SOURCE = {("fp64", False): "profiles/tile_kernels_n64_fp64_hbm_traffic.txt",
+                  ("fp32_3m", False): "profiles/tile_kernels_n64_mixed_hbm_traffic.txt",
+                  ("bf16x3", False): "profiles/tile_kernels_n64_mixed_hbm_traffic.txt"}
+HBM_PEAK_TBS = 8.0
 
 
 def workload_params(g, precision="fp64", time_steps=5000):
@@ -127,6 +130,7 @@ def run_leg(g, precision, G1, Gxx, Gxy, steps, warmup):
     if rc != 0:
         raise SystemExit(f"density guard tripped (rc={rc}, {precision})")
     G1_end = prop.get_G1()
+    sym = prop.pair_symmetric()
     prop.timing(True)
     rc = prop.advance(steps)
     k = [prop.timing_get(i) for i in range(3)]
@@ -134,7 +138,7 @@ def run_leg(g, precision, G1, Gxx, Gxy, steps, warmup):
     prop.close()
     if rc != 0:
         raise SystemExit(f"density guard tripped (rc={rc}, {precision})")
-    return 1e3 * (t1 - t0) / steps, G1_end, k
+    return 1e3 * (t1 - t0) / steps, G1_end, (k, sym)
 
 
 ENSEMBLE_DRAWS = os.path.join(ROOT, "tests", "golden", "ensemble_config4.json")
@@ -330,24 +334,67 @@ def observables_leg(g, G1, Gxx, Gxy, reps=5):
     return out
 
 
-def roofline(N, precision, k, steps):
-    # dominant kernel: the tile pass with the larger summed time; one launch of
-    # either pass applies two of the four one-particle terms of [H2,G2] to both
-    # G2 tensors: 2 tensors x N^2 tiles x 2 N^3 complex MACs x 8 flop = 32 N^5
-    # (8 flop per complex MAC whatever the product scheme: 4M, 3M or BF16 split).
-    dom = max((k[0], "tile_strided"), (k[1], "tile_fused"), key=lambda x: x[0][1])
-    (n_launch, ms_total), name = dom
-    avg_ms = ms_total / max(n_launch, 1)
+def update_bytes_per_element(precision):
+    """Algorithmic HBM bytes per G2 element of one pair-exchange update launch,
+    averaged over the four stages.  It reads the product pass's result X and
+    the state S (16 B each in FP64; the mixed modes read X (8 B) and, at stages
+    0-2, the complex64 copy of S (8 B)), reads and writes the RK accumulator A
+    (16 B; written only at stage 0, read only at stage 3) and writes the next
+    stage input Y (16 B / 8 B; at stage 3 the new state S, and in the mixed
+    modes its complex64 copy):
+      FP64:  64, 80, 80, 64 B at stages 0-3 -> 72 B;
+      mixed: 40, 56, 56, 64 B               -> 54 B."""
+    return 72.0 if precision == "fp64" else 54.0
+
+
+def roofline(N, precision, kk, steps):
+    """Rooflines of the two G2 passes from their evented launch times, and the
+    dominant one (larger summed time).
+
+    Pair-symmetric path: the product pass applies the p2 and q2 terms of
+    [H2, G2] to both tensors, 2 tensors x N^2 tiles x 2 N^3 complex MACs x
+    8 flop = 32 N^5 per launch (8 flop per complex MAC whatever the product
+    scheme: 3M or BF16 split); the p1 and q1 terms are its pair exchange, which
+    the update pass adds while it streams the RK update (HBM-bound:
+    update_bytes_per_element x 2 N^4 bytes per launch).  General path: the
+    strided and the fused pass each apply two of the four terms, 32 N^5 flop.
+    """
+    k, sym = kk
     flops = 32.0 * N ** 5
-    achieved = flops / (avg_ms * 1e-3) / 1e12
     peak, peak_name = PEAK[precision]
-    return {"bound": "mfma", "kernel": name, "achieved": round(achieved, 3), "peak": peak, "peak_type": peak_name,
-            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-            "traffic": TRAFFIC_BYTES.get((precision, name)), "traffic_source": TRAFFIC_SOURCE.get(precision),
-            "algorithmic_flop_per_launch": flops, "avg_launch_ms": round(avg_ms, 4),
-            "breakdown_ms_per_step": {"tile_strided": round(k[0][1] / steps, 3),
-                                      "tile_fused": round(k[1][1] / steps, 3),
-                                      "other": round(k[2][1] / steps, 3)}}
+
+    def mfma(name, n_launch, ms_total):
+        avg_ms = ms_total / max(n_launch, 1)
+        achieved = flops / (avg_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "kernel": name, "achieved": round(achieved, 3), "peak": peak,
+                "peak_type": peak_name, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                "traffic": TRAFFIC_BYTES.get((precision, name)),
+                "traffic_source": TRAFFIC_SOURCE.get((precision, sym)),
+                "algorithmic_flop_per_launch": flops, "avg_launch_ms": round(avg_ms, 4)}
+
+    def hbm(name, n_launch, ms_total):
+        avg_ms = ms_total / max(n_launch, 1)
+        nbytes = update_bytes_per_element(precision) * 2 * N ** 4
+        achieved = nbytes / (avg_ms * 1e-3) / 1e12
+        return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 3), "peak": HBM_PEAK_TBS,
+                "unit": "TB/s", "frac": round(achieved / HBM_PEAK_TBS, 4),
+                "traffic": TRAFFIC_BYTES.get((precision, name)),
+                "traffic_source": TRAFFIC_SOURCE.get((precision, sym)),
+                "algorithmic_bytes_per_launch": nbytes, "avg_launch_ms": round(avg_ms, 4)}
+
+    if sym:
+        passes = {"tile_products": mfma("tile_products", *k[0]), "pair_update": hbm("pair_update", *k[1])}
+        names = ("tile_products", "pair_update")
+    else:
+        passes = {"tile_strided": mfma("tile_strided", *k[0]), "tile_fused": mfma("tile_fused", *k[1])}
+        names = ("tile_strided", "tile_fused")
+    dom = names[0] if k[0][1] >= k[1][1] else names[1]
+    out = dict(passes[dom])
+    out["path"] = "pair-symmetric" if sym else "general"
+    out["passes"] = passes
+    out["breakdown_ms_per_step"] = {names[0]: round(k[0][1] / steps, 3), names[1]: round(k[1][1] / steps, 3),
+                                    "other": round(k[2][1] / steps, 3)}
+    return out
 
 
 T_START = time.time()
@@ -434,7 +481,8 @@ def main():
            "data": "synthetic (non-interacting G1(0), seeded Hermitian pair-symmetric G2 perturbation)",
            "config": {"workload": "Ns=32 Nb=2 (N=64) EHM V=U/(1+n) U=1.5 t1=1 t2=0.5 full quench 1.0 on 8 sites "
                                   "dt=0.01", "N": N, "precision": a.precision,
-                      "algorithmic_flop_per_step": 256.0 * N ** 5},
+                      "algorithmic_flop_per_step": 256.0 * N ** 5,
+                      "path": "pair-symmetric" if k_head[1] else "general"},
            "roofline": roofline(N, a.precision, k_head, a.steps), "mixed": mixed, "ensemble": ensemble, "extrapolator": extrap,
            "observables": observables, "cpu_baseline": cpu}
     progress("done")

@@ -321,6 +321,7 @@ int64_t layout(Run& r, char* base) {
         for (int s = 0; s < 2; ++s) {
             d.S2[s] = cv.take<double2>(d.n4);  // the state is kept even when frozen
             d.Y2[s] = cv.take<double2>(n4c);
+            d.Sc[s] = r.precision != G12_PREC_FP64 ? cv.take<double2>(n4c) : nullptr;
             d.A2[s] = cv.take<double2>(n4);
             d.R2[s] = cv.take<double2>(n4c);
             d.D[s] = cv.take<double2>(n3);
@@ -600,7 +601,7 @@ void drop_graphs(Run& r) {
 // the run's own tile lists.  Both passes are run as stage 0 of a step (the
 // strided pass writes only R2, the fused pass only Y2, A2 and D; the next
 // stage 0 overwrites R2 and A2 before reading them, and every run starts, after
-// tuning, by rebuilding D and the compact stage-0 input Y2 from S), so tuning
+// tuning, by rebuilding D and the compact stage-0 input Sc from S), so tuning
 // at a step boundary leaves the state unchanged.  Results are cached per
 // process by (tile size, precision, tile count).
 int tune(Run& r) {

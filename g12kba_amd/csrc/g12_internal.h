@@ -74,6 +74,9 @@ struct DevState {
     // state, next stage input, RK accumulator, strided-pass result; in
     // mixed-precision runs Y2 and R2 hold complex64 elements
     double2 *S2[2], *Y2[2], *A2[2], *R2[2];
+    // mixed-precision runs: the complex64 copy of S (blocked like Y2), the
+    // stage-0 input, written with every new state; null in FP64 runs
+    double2* Sc[2];
     double2 *D[2];                     // D[X][Y][Z] = T[X][Z][Y][Z] of the stage input
     double2 *F[2];                     // F[X][Y][Z]
     double2 *Qp[2];                    // Qp[X][Y][Z] = Q[Z][X][Y]

@@ -998,17 +998,19 @@ struct View {
     double2* R;         // unfused output / strided-pass result (compact in mixed runs)
     double2* D;         // fused: partial traces of the next stage input
     double2 *A, *Y;     // fused: RK accumulator, next stage input (compact in mixed runs)
+    double2* Sc;        // fused, mixed runs: complex64 copy of the new state (last stage)
     const double2* G1;  // the stage G1 g (Y1)
     const double2* G1t; // its conjugate transpose: conj g[Y][X] at [X][Y]
     const double2 *Qp, *F;  // source tables of the spin
     const double* W;        // W_xx (spin 0) or W_xy (spin 1)
 };
 
-// Mixed-precision runs keep the stage input compact at every stage: Y holds
-// the complex64 copy of S at stage 0 (written by the previous step's last
-// fused pass, or at the start of a run).
+// Mixed-precision runs keep the stage input compact at every stage: at stage 0
+// it is Sc, the complex64 copy of S (written with every new state by the last
+// stage's update, or at the start of a run).
 __device__ inline const double2* stage_input(const DevState& d, const BTileArgs& a, const StageShared& ss, int spin) {
-    return (a.use_y && (a.compact || !ss.first)) ? d.Y2[spin] : d.S2[spin];
+    if (a.use_y && a.compact) return ss.first ? d.Sc[spin] : d.Y2[spin];
+    return (a.use_y && !ss.first) ? d.Y2[spin] : d.S2[spin];
 }
 // the stage input is compact when it is the Y buffer of a mixed-precision run
 __device__ inline bool input_compact(const BTileArgs& a, const StageShared&) {
@@ -1032,6 +1034,7 @@ __device__ inline void load_view(View& v, const BTileArgs& a, const StageShared&
     if constexpr (FUSED) {
         v.A = uni_ptr(d.A2[spin]);
         v.Y = uni_ptr(d.Y2[spin]);
+        v.Sc = uni_ptr(d.Sc[spin]);
         v.D = uni_ptr(d.D[spin]);
     }
 }
@@ -1503,7 +1506,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                             const double2 A = cadd(apre[u][j], wK);
                             next = cadd(s, cdiv6(cscale(A, ss.dt)));
                             gst(v.S, off, next);
-                            if constexpr (C) cst(v.Y, offc, next, true);  // next step's stage-0 input
+                            if constexpr (C) cst(v.Sc, offc, next, true);  // next step's stage-0 input
                         }
                         // partial trace of the next stage input, FP64:
                         // D[X=p1][Y=q1][Z=p2] for p2 = q2
@@ -1789,7 +1792,7 @@ __global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
             for (int p2 = 0; p2 < N; ++p2)
                 for (int q2 = threadIdx.x; q2 < N; q2 += blockDim.x) {
                     const int64_t e = (int64_t)X * N2 * N + (int64_t)p2 * N2 + (int64_t)Y * N + q2;
-                    cst(d.Y2[s], gc.at<BLK_Y>(p2, q2), gload(d.S2[s], e), true);
+                    cst(d.Sc[s], gc.at<BLK_Y>(p2, q2), gload(d.S2[s], e), true);
                 }
     }
 }
@@ -1893,6 +1896,7 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     double2* S = uni_ptr(d.S2[spin]);
     double2* A = uni_ptr(d.A2[spin]);
     double2* Y = uni_ptr(d.Y2[spin]);
+    double2* Sc = uni_ptr(d.Sc[spin]);  // mixed runs: complex64 copy of S
     double2* D = uni_ptr(d.D[spin]);
     const int nq = (N + 3) & ~3;
     const uint32_t sl = C ? (uint32_t)(nq * nq) : (uint32_t)(N * N);
@@ -1941,7 +1945,9 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
             o[i] = ok[i] ? (((uint32_t)p1 * N + p2) * N + x1[i]) * N + x2[i] : 0u;  // S, A (natural)
             sv[i] = av[i] = make_double2(0.0, 0.0);
             if (ok[i]) {
-                sv[i] = gld(S, o[i]);
+                // mixed runs form the complex64 stage inputs from the complex64
+                // copy of S; the new state (last stage) comes from S itself
+                sv[i] = (C && !ss.last) ? cld(Sc, yo[i], true) : gld(S, o[i]);
                 if (!ss.first) av[i] = gld(A, o[i]);
             }
         }
@@ -1960,7 +1966,7 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
                 const double2 acc = cadd(av[i], wK);
                 next = cadd(sv[i], cdiv6(cscale(acc, ss.dt)));
                 gst(S, o[i], next);
-                if constexpr (C) cst(Y, yoff, next, true);  // next step's stage-0 input
+                if constexpr (C) cst(Sc, yoff, next, true);  // next step's stage-0 input
             }
             if (x2[i] == p2) gst(D, ((uint32_t)p1 * N + x1[i]) * N + p2, next);  // D[p1][q1][p2]
         }
