@@ -336,15 +336,16 @@ def observables_leg(g, G1, Gxx, Gxy, reps=5):
 
 def update_bytes_per_element(precision):
     """Algorithmic HBM bytes per G2 element of one pair-exchange update launch,
-    averaged over the four stages.  It reads the product pass's result X and
-    the state S (16 B each in FP64; the mixed modes read X (8 B) and, at stages
-    0-2, the complex64 copy of S (8 B)), reads and writes the RK accumulator A
-    (16 B; written only at stage 0, read only at stage 3) and writes the next
-    stage input Y (16 B / 8 B; at stage 3 the new state S, and in the mixed
-    modes its complex64 copy):
-      FP64:  64, 80, 80, 64 B at stages 0-3 -> 72 B;
-      mixed: 40, 56, 56, 64 B               -> 54 B."""
-    return 72.0 if precision == "fp64" else 54.0
+    averaged over the four stages.  Every stage reads the product pass's
+    result X (16 B in FP64, 8 B in the mixed modes) and the state S (16 B; the
+    mixed modes read its complex64 copy, 8 B, at stages 0 and 1).  Stages 0-2
+    write the next stage input Y (16 B / 8 B); the RK accumulator A (16 B) is
+    written at stage 0, read and written at stage 1, read at stage 2, which
+    folds it into S (S written); stage 3 writes the new state S (and in the
+    mixed modes its complex64 copy, 8 B):
+      FP64:  64, 80, 80, 48 B at stages 0-3 -> 68 B;
+      mixed: 40, 56, 64, 48 B               -> 52 B."""
+    return 68.0 if precision == "fp64" else 52.0
 
 
 def roofline(N, precision, kk, steps):
@@ -352,15 +353,17 @@ def roofline(N, precision, kk, steps):
     dominant one (larger summed time).
 
     Pair-symmetric path: the product pass applies the p2 and q2 terms of
-    [H2, G2] to both tensors, 2 tensors x N^2 tiles x 2 N^3 complex MACs x
-    8 flop = 32 N^5 per launch (8 flop per complex MAC whatever the product
-    scheme: 3M or BF16 split); the p1 and q1 terms are its pair exchange, which
-    the update pass adds while it streams the RK update (HBM-bound:
-    update_bytes_per_element x 2 N^4 bytes per launch).  General path: the
-    strided and the fused pass each apply two of the four terms, 32 N^5 flop.
+    [H2, G2] to the tiles t0 <= t1 of both tensors (the others are their
+    negated conjugate transposes), 2 tensors x N (N + 1) / 2 tiles x 2 N^3
+    complex MACs x 8 flop = 16 N^4 (N + 1) per launch (8 flop per complex MAC
+    whatever the product scheme: 3M or BF16 split); the p1 and q1 terms are its
+    pair exchange, which the update pass adds while it streams the RK update
+    (HBM-bound: update_bytes_per_element x 2 N^4 bytes per launch).  General
+    path: the strided and the fused pass each apply two of the four terms to
+    every tile, 32 N^5 flop.
     """
     k, sym = kk
-    flops = 32.0 * N ** 5
+    flops = 16.0 * N ** 4 * (N + 1) if sym else 32.0 * N ** 5
     peak, peak_name = PEAK[precision]
 
     def mfma(name, n_launch, ms_total):

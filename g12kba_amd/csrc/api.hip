@@ -175,6 +175,7 @@ StageShared stage_block(int k, double dt_full) {
     ss.cn = (k == 0 || k == 1) ? dt_full / 2 : dt_full;  // input of stage k + 1 (unused at k = 3)
     ss.first = k == 0;
     ss.last = k == 3;
+    ss.fold = k == 2;
     ss.lam = 1.0;
     return ss;
 }
@@ -259,6 +260,7 @@ struct Run {
     std::vector<Bucket> buckets;
     std::vector<std::vector<int>> lists;
     std::vector<std::vector<int64_t>> prefixes;
+    std::vector<std::vector<int64_t>> hprefixes;  // pair-symmetric product pass: tiles t0 <= t1
     std::vector<int> tab_list;         // correlated trajectories
     std::vector<int64_t> tab_prefix;   // prefix of their table blocks
     BatchDev b{};
@@ -343,6 +345,7 @@ int64_t layout(Run& r, char* base) {
     for (size_t k = 0; k < r.buckets.size(); ++k) {
         r.buckets[k].list = cv.take<int>(r.buckets[k].n);
         r.buckets[k].prefix = cv.take<int64_t>(r.buckets[k].n + 1);
+        r.buckets[k].hprefix = cv.take<int64_t>(r.buckets[k].n + 1);
     }
     r.b.tlist = cv.take<int>((int64_t)r.tab_list.size());
     r.b.tprefix = cv.take<int64_t>((int64_t)r.tab_prefix.size());
@@ -443,17 +446,23 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
             if (r.tp[i].corr && (r.tp[i].N + 15) / 16 * 16 == NP) l.push_back(i);
         if (l.empty()) continue;
         std::stable_sort(l.begin(), l.end(), [&](int a, int b) { return r.tp[a].N > r.tp[b].N; });
-        std::vector<int64_t> pre(l.size() + 1, 0);
-        for (size_t j = 0; j < l.size(); ++j) pre[j + 1] = pre[j] + 2 * r.ds[l[j]].n2;
+        std::vector<int64_t> pre(l.size() + 1, 0), hpre(l.size() + 1, 0);
+        for (size_t j = 0; j < l.size(); ++j) {
+            const int64_t N = r.ds[l[j]].N;
+            pre[j + 1] = pre[j] + 2 * r.ds[l[j]].n2;
+            hpre[j + 1] = hpre[j] + 2 * 16 * g12::hgroups((int)N);
+        }
         Bucket k{};
         k.NP = NP;
         k.n = (int)l.size();
         k.tiles = pre.back();
+        k.htiles = hpre.back();
         k.full = r.tp[l.back()].N == NP ? 1 : 0;  // l is sorted by N, largest first
         k.waves = k.waves_fused = g12::default_waves(NP);
         r.buckets.push_back(k);
         r.lists.push_back(l);
         r.prefixes.push_back(pre);
+        r.hprefixes.push_back(hpre);
     }
     // table launches: N x table_blocks_per_x(N) blocks per correlated trajectory
     r.tab_prefix.assign(1, 0);
@@ -490,6 +499,8 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
                               hipMemcpyHostToDevice, r.stream));
         HIPCHK(hipMemcpyAsync(const_cast<int64_t*>(r.buckets[k].prefix), r.prefixes[k].data(),
                               sizeof(int64_t) * r.prefixes[k].size(), hipMemcpyHostToDevice, r.stream));
+        HIPCHK(hipMemcpyAsync(const_cast<int64_t*>(r.buckets[k].hprefix), r.hprefixes[k].data(),
+                              sizeof(int64_t) * r.hprefixes[k].size(), hipMemcpyHostToDevice, r.stream));
     }
     if (!r.tab_list.empty()) {
         HIPCHK(hipMemcpyAsync(const_cast<int*>(r.b.tlist), r.tab_list.data(), sizeof(int) * r.tab_list.size(),
@@ -896,9 +907,8 @@ int set_state(Run& r, int i, const double* G1, const double* G2xx, const double*
     }
     if (e == hipSuccess) e = hipMemsetAsync(d.fail, 0, sizeof(int), r.stream);
     // does the state qualify for the pair-symmetric path?  G1 anti-Hermitian and
-    // each G2 tensor pair-symmetric, to 1e-12 of its largest element, and in
-    // FP64 runs Hermitian as well (mixed-precision runs build the F tables from
-    // the Hermitian part of D, which is all the exchange of Pi needs)
+    // each G2 tensor pair-symmetric and Hermitian, to 1e-12 of its largest
+    // element (the path then keeps both symmetries exactly)
     int flag = 0;
     if (r.tp[i].corr && e == hipSuccess) {
         e = hipMemsetAsync(r.d_flag, 0, sizeof(int), r.stream);
@@ -906,7 +916,7 @@ int set_state(Run& r, int i, const double* G1, const double* G2xx, const double*
             if (!g2[sp]) continue;
             double mx = 0.0;
             for (int64_t k = 0; k < 2 * d.n4; ++k) mx = std::max(mx, std::fabs(g2[sp][k]));
-            g12::launch_pair_check(d.S2[sp], d.N, 1e-12 * mx, r.precision == G12_PREC_FP64, r.d_flag, r.stream);
+            g12::launch_pair_check(d.S2[sp], d.N, 1e-12 * mx, 1, r.d_flag, r.stream);
         }
         if (e == hipSuccess) e = hipMemcpyAsync(&flag, r.d_flag, sizeof(int), hipMemcpyDeviceToHost, r.stream);
     }

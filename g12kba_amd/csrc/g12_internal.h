@@ -51,6 +51,7 @@ struct StageShared {
     int as_phase;      // 1 during adiabatic switching
     int step;          // adiabatic-switching step index (trajectory stops after as_steps)
     int slot;          // trajectory-record slot of this step (production)
+    int fold;          // stage 2: the pair-symmetric update folds A into S (k_sym_update)
 };
 
 // Where a kernel finds its StageShared: by value, or (graph replay) at
@@ -133,6 +134,11 @@ struct Bucket {
     int wg_per_cu;           // workgroups per CU cap, strided pass (0: as many as LDS and waves allow)
     int wg_per_cu_fused;     // the same for the fused pass
     int full;                // every listed trajectory has N = NP (no padded tile elements)
+    // the pair-symmetric product pass's tile list: tiles (t0, t1), t0 <= t1,
+    // in groups of 4 x 4 (kernels.hip ref_of), 2 x 16 hgroups(N) slots per
+    // listed trajectory
+    const int64_t* hprefix;  // device array [n + 1]
+    int64_t htiles;
 };
 
 // batched stage kernels (kernels.hip)
@@ -150,6 +156,11 @@ void launch_b_sym_update(const BatchDev& b, const StageRef& sr, int precision, h
 void launch_pair_check(const double2* T, int N, double tol, int herm, int* flag, hipStream_t s);
 // largest N of the pair-symmetric path
 constexpr int SYM_MAX_N = 64;
+// group pairs (A <= B) of 4 x 4 tiles of the pair-symmetric product pass
+__host__ __device__ inline int hgroups(int N) {
+    const int ng = (N + 3) / 4;
+    return ng * (ng + 1) / 2;
+}
 void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hipStream_t s);
 // FP64 tile pass of a bucket with the given products (hmode: 1 +h.X, 2 -X.h),
 // sources (0/1) and output mode: the routine-level entries' route through the

@@ -663,6 +663,11 @@ template <int NP, int NW>
 struct Engine<NP, NW, G12_PREC_FP64> {
     typedef TileLds<NP> Lds;
     static constexpr int P = Lds::P;
+    // output staging of the pair-symmetric pass: the X planes (pitch OP),
+    // free once the products have read them
+    typedef double Out;
+    static constexpr int OP = P;
+    __device__ static Out* out_plane(Lds& L, int c) { return c ? L.xi : L.xr; }
     __device__ static void put_h(Lds& L, int m, int n, double2 v) {
         L.hr[m * P + n] = v.x;
         L.hi[m * P + n] = v.y;
@@ -696,6 +701,9 @@ template <int NP, int NW>
 struct Engine<NP, NW, G12_PREC_FP32_3M> {
     typedef TileLds32<NP> Lds;
     static constexpr int P = Lds::P;
+    typedef float Out;  // output staging: the X planes
+    static constexpr int OP = P;
+    __device__ static Out* out_plane(Lds& L, int c) { return c ? L.xi : L.xr; }
     static constexpr int NT = TileCfg<NP, NW>::NT, TT = TileCfg<NP, NW>::TT, PW = TileCfg<NP, NW>::PW;
     __device__ static void put_h(Lds& L, int m, int n, double2 v) {
         L.hr[m * P + n] = (float)v.x;
@@ -792,6 +800,12 @@ template <int NP, int NW>
 struct Engine<NP, NW, G12_PREC_BF16X3> {
     typedef TileLdsBf<NP> Lds;
     static constexpr int PB = Lds::PB, KP = Lds::KP;
+    // output staging: two FP32 planes over the X planes (8-15); their zero
+    // padding has to be restored afterwards when N < NP
+    typedef float Out;
+    static constexpr int OP = NP + 1;
+    static_assert(2 * NP * OP * sizeof(float) <= 8 * NP * PB * sizeof(__bf16), "staging fits the X planes");
+    __device__ static Out* out_plane(Lds& L, int c) { return reinterpret_cast<float*>(&L.v[8][0]) + c * NP * OP; }
     static constexpr int NT = TileCfg<NP, NW>::NT, TT = TileCfg<NP, NW>::TT, PW = TileCfg<NP, NW>::PW;
     __device__ static void split(double v, __bf16& hi, __bf16& lo) {
         hi = (__bf16)(float)v;
@@ -947,6 +961,7 @@ __device__ inline StageShared uni_stage(const StageRef& r) {
     u.as_phase = uni(v.as_phase);
     u.step = uni(v.step);
     u.slot = uni(v.slot);
+    u.fold = uni(v.fold);
     return u;
 }
 
@@ -973,15 +988,39 @@ __device__ inline void seek(const BTileArgs& a, const StageShared& ss, Cursor& c
     c.on = uni(active(a.b.ds[c.traj], a.b.tp[c.traj], ss) ? 1 : 0) != 0;
 }
 
-// Tile of a trajectory: spin (0 xxxx, 1 xyxy) and index within the spin.
+// Tile of a trajectory: spin (0 xxxx, 1 xyxy) and index within the spin;
+// ok = false for the unused slots of the pair-symmetric list.
 struct TileRef {
     int traj, N, spin, tile;
+    bool ok;
 };
 
+// HALF: the pair-symmetric pass's list of tiles (t0, t1) with t0 <= t1, in
+// groups of 4 x 4: slot 16 g + 4 i + j of a spin is t0 = 4 A + i, t1 = 4 B + j
+// for the group pair g = B (B + 1) / 2 + A (A <= B); slots with t0 > t1 or
+// past N are unused.  The 16 tiles of a group write whole 128-byte lines of
+// the blocked complex64 result, directly (4 consecutive t1 at each t0) and
+// mirrored (4 consecutive t0 at each t1).  16 hgroups(N) slots per spin.
+template <bool HALF = false>
 __device__ inline TileRef ref_of(const Cursor& c, int64_t gt) {
     TileRef t;
     t.traj = c.traj;
     t.N = c.N;
+    t.ok = true;
+    if constexpr (HALF) {
+        const int H = 16 * hgroups(c.N);
+        int local = (int)(gt - c.begin);  // < 2 H
+        t.spin = local >= H ? 1 : 0;
+        local -= t.spin * H;
+        const int g = local >> 4, w = local & 15;
+        int B = (int)((sqrt(8.0 * (double)g + 1.0) - 1.0) * 0.5);
+        while ((B + 1) * (B + 2) / 2 <= g) ++B;
+        while (B * (B + 1) / 2 > g) --B;
+        const int t0 = 4 * (g - B * (B + 1) / 2) + (w >> 2), t1 = 4 * B + (w & 3);
+        t.ok = t0 <= t1 && t1 < c.N;
+        t.tile = t.ok ? t0 * c.N + t1 : 0;
+        return t;
+    }
     const int local = (int)(gt - c.begin);  // < 2 N^2
     t.spin = local >= c.N * c.N ? 1 : 0;
     t.tile = local - t.spin * c.N * c.N;
@@ -1230,12 +1269,19 @@ __device__ __forceinline__ double2 lane_value(double2 v, int src) {
 
 // FULL: every tile of the launch has N = NP, so the element bounds checks and
 // the divisions by N drop out at compile time.
-// SYM (unfused only): pass 1 of the pair-symmetric path.  Contiguous tiles
-// X[p2][q2] at fixed (p1, q1) = (t0, t1); the output R = h.X - X.h +
-// g22 (ea[p2] - eb[q2]) with e = i Qp[t0][t1][.] + F[t0][t1][.] and
-// ea = e + (g11/2) W[t0][.], eb = e + (g11/2) W[t1][.]: the fused pass's
-// terms and half of the Psi term g11 g22 (W[p1][p2] - W[q1][q2]); the
-// update kernel adds the pair-exchanged R (k_sym_update).
+// SYM (unfused only): the product pass of the pair-symmetric path.
+// Contiguous tiles X[p2][q2] at fixed (p1, q1) = (t0, t1); the output
+// R = h.X - X.h + g22 (ea[p2] - eb[q2]) with e = i Qp[t0][t1][.] +
+// F[t0][t1][.] and ea = e + (g11/2) W[t0][.], eb = e + (g11/2) W[t1][.]: the
+// fused pass's terms and half of the Psi term g11 g22 (W[p1][p2] -
+// W[q1][q2]); the update kernel adds the pair-exchanged R (k_sym_update).
+// The state is Hermitian as well, so R is anti-Hermitian as an N^2 x N^2
+// matrix: the tile (t1, t0) of R is -(tile (t0, t1))^dagger, because
+// (h X - X h)^dagger = X^dagger h - h X^dagger with the input tile
+// (t1, t0) = (tile (t0, t1))^dagger, Qp[t1][t0][.] = conj Qp[t0][t1][.] and
+// F[t1][t0][.] = -conj F[t0][t1][.] for G1 anti-Hermitian and D Hermitian.
+// Only the tiles t0 <= t1 are computed; the result goes through LDS and
+// leaves as rows of both tiles (t0 = t1: (R - R^dagger) / 2).
 template <int NP, int NW, int PREC, bool FUSED, bool FULL, bool SYM = false>
 __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     static_assert(!(FUSED && SYM), "the pair-symmetric products run unfused");
@@ -1262,7 +1308,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
             const int64_t g = sched.tile(it, blockIdx.x, gridDim.x);
             if (g < 0) return -1;
             seek(a, ss, c, g);
-            if (c.on) return g;
+            if (c.on && (!SYM || ref_of<SYM>(c, g).ok)) return g;
             ++it;
         }
     };
@@ -1270,7 +1316,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     int it = 0;
     int64_t gt = next_tile(it);
     if (gt < 0) return;
-    TileRef tr = ref_of(c, gt);
+    TileRef tr = ref_of<SYM>(c, gt);
     // Stage-input prefetch: a tile's X loads into registers while earlier
     // tiles are multiplied.  The mixed-precision engines hold raw complex64
     // elements (their stage input is compact), so two tiles fit in the
@@ -1292,7 +1338,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         ++it;
         g1 = next_tile(it);
         if (g1 >= 0) {
-            t1r = ref_of(c, g1);
+            t1r = ref_of<SYM>(c, g1);
             prefetch(t1r, P1);
         }
     }
@@ -1362,7 +1408,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         const int64_t gnew = next_tile(it);
         TileRef rnew = tr;
         if (gnew >= 0) {  // its loads overlap the MFMA loop(s) before its turn
-            rnew = ref_of(c, gnew);
+            rnew = ref_of<SYM>(c, gnew);
             prefetch(rnew, Pc);
         }
         int64_t gnext = gnew;  // the tile after tr
@@ -1421,7 +1467,65 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
             }
         });
 
-        if constexpr (!FUSED) {
+        if constexpr (SYM) {
+            // stage R (products + sources) in LDS once every wave has read X
+            typedef typename E::Out OT;
+            OT* ore = E::out_plane(L, 0);
+            OT* oim = E::out_plane(L, 1);
+            constexpr int OP = E::OP;
+            lds_sync();
+#pragma unroll
+            for (int u = 0; u < PW; ++u) {
+                const int t = wave + NW * u;
+                if (!ALLB && t >= TT) continue;
+                const int n = (t % NT) * 16 + (lane & 15);
+                const double2 bn = a.sources ? srcv[1][n] : double2{};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int m = (t / NT) * 16 + E::row_of(lane, j);
+                    if (!inside(t, m, n)) continue;
+                    double2 val = make_double2(ar[u][j], ai[u][j]);
+                    if (a.sources)  // g22 (ea[p2] - eb[q2])
+                        val = cadd(val, cmul(gld(v.G1, (uint32_t)(m * N + n)), csub(srcv[0][m], bn)));
+                    ore[m * OP + n] = (OT)val.x;
+                    oim[m * OP + n] = (OT)val.y;
+                }
+            }
+            lds_sync();
+            // rows of tile (t0, t1) and of its mirror (t1, t0) = -(.)^dagger
+            auto write_out = [&](auto compact) {
+                constexpr bool C = decltype(compact)::value;
+                const TileGeom ga = tile_geom(1, N, t0 * N + t1, C && BLK_R);
+                const TileGeom gb = tile_geom(1, N, t1 * N + t0, C && BLK_R);
+#pragma unroll
+                for (int i = 0; i < PF; ++i) {
+                    const int e = threadIdx.x + NTHR * i;
+                    if (!(EXACT || e < N * N)) continue;
+                    const int m = e / N, n = e % N;
+                    const double vr = ore[m * OP + n], vi = oim[m * OP + n];
+                    const double wr = ore[n * OP + m], wi = oim[n * OP + m];
+                    if (t0 == t1) {
+                        cst(v.R, ga.at<C && BLK_R>(m, n), make_double2((vr - wr) * 0.5, (vi + wi) * 0.5), C);
+                    } else {
+                        cst(v.R, ga.at<C && BLK_R>(m, n), make_double2(vr, vi), C);
+                        cst(v.R, gb.at<C && BLK_R>(m, n), make_double2(-wr, wi), C);
+                    }
+                }
+            };
+            if (a.compact)
+                write_out(std::true_type{});
+            else
+                write_out(std::false_type{});
+            if constexpr (PREC == G12_PREC_BF16X3) {
+                // the staging covered the zero padding of the bf16 X planes
+                // (k runs to the MFMA depth KP >= NP)
+                if (N < TileLdsBf<NP>::KP) {
+                    lds_sync();
+                    unsigned* w = reinterpret_cast<unsigned*>(ore);
+                    for (int i = threadIdx.x; i < 2 * NP * OP; i += NTHR) w[i] = 0u;
+                }
+            }
+        } else if constexpr (!FUSED) {
             double2* out = v.R;
             // strided-pass sources: g11 (va[m] - vb[n]) - conj g(q1,p1) (vc[n] - vc[m]),
             // va, vb, vc in srcv[0..2]
@@ -1902,78 +2006,84 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     const uint32_t sl = C ? (uint32_t)(nq * nq) : (uint32_t)(N * N);
     const SuIter<C, true> rows{(uint32_t)(a * N + m) * sl, N, nq, r0, nr};   // slice (a, m)
     const SuIter<C, false> cols{(uint32_t)(m * N + a) * sl, N, nq, r0, nr};  // slice (m, a)
-    auto xload = [&](uint32_t off) -> LT {
-        if constexpr (C)
-            return *(const GLOBAL fv2*)((const GLOBAL char*)X + (off << 3));
-        else
-            return *(const GLOBAL dv2*)((const GLOBAL char*)X + (off << 4));
-    };
+    const bool two = a != m;  // slice (m, a) is another slice
     // every band holds at most SU_BAND x SU_MAXN elements: IT per thread
     constexpr int IT = SU_BAND * SU_MAXN / SU_THREADS;
     static_assert(IT * SU_THREADS == SU_BAND * SU_MAXN, "whole items per thread");
-    auto fill = [&](const auto& it, auto&& put) {
-        int x1[IT], x2[IT];
-        bool ok[IT];
-        LT v[IT];
+    // RK protocol of this pass (its stages see S and A only at element level):
+    //   stage 0: A = w K;      stage 1: A += w K;   Y = S + c' K
+    //   stage 2: Y = S + c' K, then S += dt (A + w K) / 6 (A folded into S)
+    //   stage 3: S += dt w K / 6 (the new state; mixed runs: and its copy Sc)
+    // Mixed runs take S at stages 0 and 1 from its complex64 copy Sc (the
+    // stage inputs are complex64 anyway); stages 2 and 3 update S in FP64.
+    const bool s_c64 = C && !ss.fold && !ss.last;
+    struct Item {
+        int x1, x2;
+        uint32_t o, yo;  // natural offset (S, A); layout offset (X, Y, Sc)
+        bool ok;
+        LT xv;           // this element of X
+        double2 sv, av;  // S (or Sc), A
+    };
+    Item ri[IT], ci[IT];
+    // with_state = false: X only (the band columns of a slice paired with itself)
+    auto gather = [&](const auto& it, int p1, int p2, Item (&w)[IT], bool with_state) {
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
             const int e = threadIdx.x + i * SU_THREADS;
-            uint32_t o = 0;
-            ok[i] = e < it.count() && it.at(e, x1[i], x2[i], o);
-            if (ok[i]) v[i] = xload(o);
+            w[i].ok = e < it.count() && it.at(e, w[i].x1, w[i].x2, w[i].yo);
+            w[i].o = w[i].ok ? (((uint32_t)p1 * N + p2) * N + w[i].x1) * N + w[i].x2 : 0u;
+            if constexpr (!C) w[i].yo = w[i].o;
+            w[i].sv = w[i].av = make_double2(0.0, 0.0);
+            if (w[i].ok) {
+                if constexpr (C)
+                    w[i].xv = *(const GLOBAL fv2*)((const GLOBAL char*)X + (w[i].yo << 3));
+                else
+                    w[i].xv = *(const GLOBAL dv2*)((const GLOBAL char*)X + (w[i].yo << 4));
+                if (with_state) {
+                    w[i].sv = s_c64 ? cld(Sc, w[i].yo, true) : gld(S, w[i].o);
+                    if (!ss.first && !ss.last) w[i].av = gld(A, w[i].o);
+                }
+            }
         }
-#pragma unroll
-        for (int i = 0; i < IT; ++i)
-            if (ok[i]) put(x1[i], x2[i], v[i]);
     };
-    fill(rows, [&](int q1, int q2, LT v) { la[q1 - r0][q2] = v; });
-    fill(cols, [&](int q1, int q2, LT v) { lb[q1][q2 - r0] = v; });
+    // all loads of the block (X and the RK state of both slices) in flight at once
+    gather(rows, a, m, ri, true);
+    gather(cols, m, a, ci, two);  // for a = m the band columns of the slice itself: X only
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        if (ri[i].ok) la[ri[i].x1 - r0][ri[i].x2] = ri[i].xv;
+        if (ci[i].ok) lb[ci[i].x1][ci[i].x2 - r0] = ci[i].xv;
+    }
     __syncthreads();
     auto tod = [](LT v) { return make_double2((double)v.x, (double)v.y); };
-    // the fused pass's update of the elements (p1, p2, q1, q2) of one slice
-    // with C = R + P(R) from cval(q1, q2); the loads of S and A are issued
-    // for all of a thread's elements first
-    auto update = [&](const auto& it, int p1, int p2, auto&& cval) {
-        int x1[IT], x2[IT];
-        uint32_t o[IT], yo[IT];
-        bool ok[IT];
-        double2 sv[IT], av[IT];
+    auto update = [&](const Item (&w)[IT], int p2, bool rows_side) {
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
-            const int e = threadIdx.x + i * SU_THREADS;
-            ok[i] = e < it.count() && it.at(e, x1[i], x2[i], yo[i]);
-            o[i] = ok[i] ? (((uint32_t)p1 * N + p2) * N + x1[i]) * N + x2[i] : 0u;  // S, A (natural)
-            sv[i] = av[i] = make_double2(0.0, 0.0);
-            if (ok[i]) {
-                // mixed runs form the complex64 stage inputs from the complex64
-                // copy of S; the new state (last stage) comes from S itself
-                sv[i] = (C && !ss.last) ? cld(Sc, yo[i], true) : gld(S, o[i]);
-                if (!ss.first) av[i] = gld(A, o[i]);
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            if (!ok[i]) continue;
-            const uint32_t yoff = C ? yo[i] : o[i];
-            const double2 K = cmul_mi(cval(x1[i], x2[i]));
+            if (!w[i].ok) continue;
+            const int x1 = w[i].x1, x2 = w[i].x2;
+            // C = R + P(R): this element of R plus the exchanged one from LDS
+            const double2 c = rows_side ? cadd(tod(la[x1 - r0][x2]), tod(lb[x2][x1 - r0]))
+                                        : cadd(tod(lb[x1][x2 - r0]), tod(la[x2 - r0][x1]));
+            const double2 K = cmul_mi(c);
             const double2 wK = cscale(K, ss.wk);
             double2 next;
-            if (!ss.last) {
-                gst(A, o[i], ss.first ? wK : cadd(av[i], wK));
-                next = cadd(sv[i], cscale(K, ss.cn));
-                cst(Y, yoff, next, C);
+            if (ss.last) {
+                next = cadd(w[i].sv, cdiv6(cscale(wK, ss.dt)));
+                gst(S, w[i].o, next);
+                if constexpr (C) cst(Sc, w[i].yo, next, true);  // next step's stage-0 input
             } else {
-                const double2 acc = cadd(av[i], wK);
-                next = cadd(sv[i], cdiv6(cscale(acc, ss.dt)));
-                gst(S, o[i], next);
-                if constexpr (C) cst(Sc, yoff, next, true);  // next step's stage-0 input
+                next = cadd(w[i].sv, cscale(K, ss.cn));
+                cst(Y, w[i].yo, next, C);
+                if (ss.fold)
+                    gst(S, w[i].o, cadd(w[i].sv, cdiv6(cscale(cadd(w[i].av, wK), ss.dt))));
+                else
+                    gst(A, w[i].o, ss.first ? wK : cadd(w[i].av, wK));
             }
-            if (x2[i] == p2) gst(D, ((uint32_t)p1 * N + x1[i]) * N + p2, next);  // D[p1][q1][p2]
+            if (x2 == p2) gst(D, (((uint32_t)(rows_side ? a : m)) * N + x1) * N + p2, next);  // D[p1][q1][p2]
         }
     };
-    update(rows, a, m, [&](int q1, int q2) { return cadd(tod(la[q1 - r0][q2]), tod(lb[q2][q1 - r0])); });
-    if (a != m)
-        update(cols, m, a, [&](int q1, int q2) { return cadd(tod(lb[q1][q2 - r0]), tod(la[q2 - r0][q1])); });
+    update(ri, m, true);
+    if (two) update(ci, a, false);
 }
 
 // flag |= 1 unless T is pair-symmetric (and, with herm, Hermitian) within tol,
@@ -2022,8 +2132,8 @@ void launch_tile_cfg(const BTileArgs& a, hipStream_t s) {
     if (a.fused)
         hipLaunchKernelGGL((k_tile<NP, NW, PREC, true, FULL_FUSED>), dim3((unsigned)grid),
                            dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
-    else if (a.sym)
-        hipLaunchKernelGGL((k_tile<NP, NW, PREC, false, FULL, true>), dim3((unsigned)grid),
+    else if (a.sym)  // BF16x3's full-tile form of this pass spills registers
+        hipLaunchKernelGGL((k_tile<NP, NW, PREC, false, FULL && PREC != G12_PREC_BF16X3, true>), dim3((unsigned)grid),
                            dim3(TileCfg<NP, NW>::NTHR), 0, s, a);
     else
         hipLaunchKernelGGL((k_tile<NP, NW, PREC, false, FULL>), dim3((unsigned)grid), dim3(TileCfg<NP, NW>::NTHR), 0,
@@ -2243,6 +2353,10 @@ void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& r, int fu
     }
     a.r = r;
     a.sym = (!fused && sym) ? 1 : 0;
+    if (a.sym) {  // the half tile list (t0 <= t1)
+        a.k.prefix = k.hprefix;
+        a.k.tiles = k.htiles;
+    }
     a.tiling = (fused || a.sym) ? 1 : 0;
     a.fused = fused;
     a.hmode = 3;

@@ -11,7 +11,8 @@ created, keeps the general path, so both run on the same input here:
   precision (FP64 within 1e-12 relative; the mixed modes within their
   own error bounds of each other);
 * an uneven batch (N = 6 ... 60, every tile bucket, N not a multiple of 4 or
-  16), adiabatic switching from G2 = 0 plus production steps, FP64 and FP32-3M;
+  16), adiabatic switching from G2 = 0 plus production steps, in every
+  precision;
 * path selection: non-symmetric, non-Hermitian or G1-not-anti-Hermitian states
   take the general path.
 The oracle comparisons of the same path are test_gpu_runs.py (config 1 and 2
@@ -118,7 +119,7 @@ def run_batch(params, general):
     return sym, rc, trajs, states
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32_3m"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32_3m", "bf16x3"])
 def test_paths_agree_uneven_batch(prec):
     params = uneven_params(prec)
     sym_a, rc_a, tr_a, st_a = run_batch(params, general=False)
