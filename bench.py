@@ -33,11 +33,17 @@ METRIC = "G2 RK4 steps/s at N=64 (FP64 vs mixed @≤1e-6 G1 err); batched traj/s
 # each comes from.  Passes without a measurement report null.
 TRAFFIC_BYTES = {("fp64", "tile_fused"): 2.995e9, ("fp64", "tile_strided"): 1.099e9,
                  ("fp32_3m", "tile_fused"): 2.455e9, ("fp32_3m", "tile_strided"): 0.556e9,
-                 ("bf16x3", "tile_fused"): 2.328e9, ("bf16x3", "tile_strided"): 0.574e9}
+                 ("bf16x3", "tile_fused"): 2.328e9, ("bf16x3", "tile_strided"): 0.574e9,
+                 ("fp64", "tile_products"): 0.823e9, ("fp64", "pair_update"): 2.309e9,
+                 ("fp32_3m", "tile_products"): 0.478e9, ("fp32_3m", "pair_update"): 1.769e9,
+                 ("bf16x3", "tile_products"): 0.501e9, ("bf16x3", "pair_update"): 1.769e9}
 
 TRAFFIC_SOURCE = {("fp64", False): "profiles/tile_kernels_n64_fp64_hbm_traffic.txt",
                   ("fp32_3m", False): "profiles/tile_kernels_n64_mixed_hbm_traffic.txt",
-                  ("bf16x3", False): "profiles/tile_kernels_n64_mixed_hbm_traffic.txt"}
+                  ("bf16x3", False): "profiles/tile_kernels_n64_mixed_hbm_traffic.txt",
+                  ("fp64", True): "profiles/pair_symmetric_n64_hbm_traffic.txt",
+                  ("fp32_3m", True): "profiles/pair_symmetric_n64_hbm_traffic.txt",
+                  ("bf16x3", True): "profiles/pair_symmetric_n64_hbm_traffic.txt"}
 HBM_PEAK_TBS = 8.0
 
 

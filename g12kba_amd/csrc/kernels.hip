@@ -1920,7 +1920,10 @@ __global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
 // K = -i C, A += w K, next stage input Y = S + c' K (the last stage:
 // S += dt (A + w K) / 6 and Y its complex64 copy), and the partial traces D
 // of the next stage input.
-constexpr int SU_BAND = 16, SU_THREADS = 256, SU_MAXN = 64;
+// 1024 threads per block: one element of each side per thread at N = 64
+// (256 and 512 threads, and an XCD-contiguous block order, measured within
+// 2-3% of it)
+constexpr int SU_BAND = 16, SU_THREADS = 1024, SU_MAXN = 64;
 
 // slice pairs (a <= m) in order of m: index m (m + 1) / 2 + a
 __device__ __forceinline__ void su_pair(int64_t pi, int& a, int& m) {
@@ -1974,6 +1977,7 @@ struct SuIter {
 
 template <bool C>
 __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef r) {
+    constexpr int TH = SU_THREADS;
     typedef typename std::conditional<C, fv2, dv2>::type LT;
     __shared__ LT la[SU_BAND][SU_MAXN + 1];  // slice (a, m): band rows [q1 - r0][q2]
     __shared__ LT lb[SU_MAXN][SU_BAND + 1];  // slice (m, a): band columns [q1][q2 - r0]
@@ -2008,8 +2012,8 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     const SuIter<C, false> cols{(uint32_t)(m * N + a) * sl, N, nq, r0, nr};  // slice (m, a)
     const bool two = a != m;  // slice (m, a) is another slice
     // every band holds at most SU_BAND x SU_MAXN elements: IT per thread
-    constexpr int IT = SU_BAND * SU_MAXN / SU_THREADS;
-    static_assert(IT * SU_THREADS == SU_BAND * SU_MAXN, "whole items per thread");
+    constexpr int IT = SU_BAND * SU_MAXN / TH;
+    static_assert(IT * TH == SU_BAND * SU_MAXN, "whole items per thread");
     // RK protocol of this pass (its stages see S and A only at element level):
     //   stage 0: A = w K;      stage 1: A += w K;   Y = S + c' K
     //   stage 2: Y = S + c' K, then S += dt (A + w K) / 6 (A folded into S)
@@ -2029,7 +2033,7 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     auto gather = [&](const auto& it, int p1, int p2, Item (&w)[IT], bool with_state) {
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
-            const int e = threadIdx.x + i * SU_THREADS;
+            const int e = threadIdx.x + i * TH;
             w[i].ok = e < it.count() && it.at(e, w[i].x1, w[i].x2, w[i].yo);
             w[i].o = w[i].ok ? (((uint32_t)p1 * N + p2) * N + w[i].x1) * N + w[i].x2 : 0u;
             if constexpr (!C) w[i].yo = w[i].o;

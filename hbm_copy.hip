@@ -1,9 +1,12 @@
 // hbm_copy — device copies of a known size: the achievable HBM bandwidth of
 // the box (the denominator bench.py's roofline is read against) and the
 // calibration of the FETCH_SIZE / WRITE_SIZE counters against bytes moved.
-//   hbm_copy [MiB=1024] [reps=5] [variant=all|plain|wide]
-// plain: one 16-byte element per thread per grid-stride iteration;
-// wide:  four independent 16-byte loads in flight per thread, non-temporal.
+//   hbm_copy [MiB=1024] [reps=5] [variant=all|plain|wide|chunk]
+// plain: one 16-byte element per thread per grid-stride iteration (4096
+//        blocks; the counter calibration);
+// wide:  four independent 16-byte loads in flight per thread, non-temporal;
+// chunk: one 16-byte element per thread, one block per 4 KB chunk, no loop
+//        (the fastest form measured: the achievable copy rate).
 // Built by __graft_entry__.build() into _build/hbm_copy.
 #include <hip/hip_runtime.h>
 
@@ -17,6 +20,11 @@ __global__ void k_copy(const double2* __restrict__ src, double2* __restrict__ ds
 }
 
 typedef double dv2 __attribute__((ext_vector_type(2)));
+
+__global__ void k_copy_chunk(const double2* __restrict__ src, double2* __restrict__ dst, long n) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
 
 __global__ void k_copy_wide(const dv2* __restrict__ src, dv2* __restrict__ dst, long n) {
     const long stride = (long)gridDim.x * blockDim.x;
@@ -46,15 +54,17 @@ int main(int argc, char** argv) {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess) return 1;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 1;
-    for (int v = 0; v < 2; ++v) {
-        const char* name = v == 0 ? "plain" : "wide";
+    for (int v = 0; v < 3; ++v) {
+        const char* name = v == 0 ? "plain" : v == 1 ? "wide" : "chunk";
         if (std::strcmp(variant, "all") != 0 && std::strcmp(variant, name) != 0) continue;
         for (int r = 0; r < reps; ++r) {
             if (hipEventRecord(e0) != hipSuccess) return 1;
             if (v == 0)
                 hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, 0, a, b, n);
-            else
+            else if (v == 1)
                 hipLaunchKernelGGL(k_copy_wide, dim3(cus * 8), dim3(256), 0, 0, (const dv2*)a, (dv2*)b, n);
+            else
+                hipLaunchKernelGGL(k_copy_chunk, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, a, b, n);
             if (hipEventRecord(e1) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return 1;
             float ms = 0;
             if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return 1;
