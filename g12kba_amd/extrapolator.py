@@ -490,7 +490,30 @@ def _main():
     tr.add_argument("--learning-curve", default="", help="comma-separated training-set sizes (trajectories) "
                                                          "to train and evaluate on the same held-out set")
     tr.add_argument("--log", default=None, help="JSON summary file")
+    cp = sub.add_parser("compare", help="closure runs on the config-5 horizon with the network and with J held "
+                                        "constant, at two chunk lengths, judged by RK4")
+    cp.add_argument("--model", default=DEFAULT_MODEL)
+    cp.add_argument("--prefix", type=int, default=4000)
+    cp.add_argument("--chunks", default="8,2", help="comma-separated chunk lengths in blocks of 16 steps")
     a = ap.parse_args()
+    if a.cmd == "compare":
+        p = config5_params()
+        judge, t_judge = rk4_run(p)
+        print(json.dumps({"judge_rk4_seconds": round(t_judge, 2), "steps": len(judge)}), flush=True)
+        with Extrapolator() as m:
+            m.load(a.model)
+            for cb in [int(x) for x in a.chunks.split(",") if x]:
+                for mode in ("network", "hold"):
+                    r = closure_run(p, m, prefix=a.prefix, chunk_blocks=cb, collision=mode)
+                    post = r["by_net"][a.prefix:]
+                    dens = np.abs(np.diagonal(r["traj"] - judge, axis1=1, axis2=2).imag).max(axis=1)
+                    print(json.dumps({"collision": mode, "chunk_steps": cb * m.horizon,
+                                      "post_prefix_coverage": round(float(post.mean()), 4),
+                                      "accepted_chunks": r["accepted_chunks"], "rejected_chunks": r["rejected_chunks"],
+                                      "rk4_steps": r["rk_steps"], "max_density_err_vs_rk4": float(dens.max()),
+                                      "largest_energy_drift_accepted": r["checks"]["energy"],
+                                      "seconds": round(r["seconds"], 2)}), flush=True)
+        return
     if a.cmd == "train":
         say = lambda s: print(s, flush=True)  # noqa: E731
         S, C, D, held = build_training_set(a.trajectories, steps=a.steps, per_traj=a.per_traj, holdout=a.holdout,
