@@ -1468,6 +1468,38 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         });
 
         if constexpr (SYM) {
+          // t0 != t1: the tile and its mirror -(.)^dagger leave straight from
+          // the MFMA layout (complex128: 256-byte rows of the tile, 64-byte
+          // pieces of the mirror's rows; blocked complex64: the four j of a
+          // lane complete each 128-byte block); t0 = t1, and BF16x3 (whose
+          // pass ran 23% slower with direct stores), through LDS
+          if (PREC != G12_PREC_BF16X3 && t0 != t1) {
+            const bool cb = a.compact && BLK_R;
+            const TileGeom ga = tile_geom(1, N, t0 * N + t1, cb);
+            const TileGeom gb = tile_geom(1, N, t1 * N + t0, cb);
+#pragma unroll
+            for (int u = 0; u < PW; ++u) {
+                const int t = wave + NW * u;
+                if (!ALLB && t >= TT) continue;
+                const int n = (t % NT) * 16 + (lane & 15);
+                const double2 bn = a.sources ? srcv[1][n] : double2{};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int m = (t / NT) * 16 + E::row_of(lane, j);
+                    if (!inside(t, m, n)) continue;
+                    double2 val = make_double2(ar[u][j], ai[u][j]);
+                    if (a.sources)  // g22 (ea[p2] - eb[q2])
+                        val = cadd(val, cmul(gld(v.G1, (uint32_t)(m * N + n)), csub(srcv[0][m], bn)));
+                    if (a.compact) {
+                        cst(v.R, BLK_R ? ga.blk(m, n) : ga.nat(m, n), val, true);
+                        cst(v.R, BLK_R ? gb.blk(n, m) : gb.nat(n, m), make_double2(-val.x, val.y), true);
+                    } else {
+                        gst(v.R, ga.nat(m, n), val);
+                        gst(v.R, gb.nat(n, m), make_double2(-val.x, val.y));
+                    }
+                }
+            }
+          } else {
             // stage R (products + sources) in LDS once every wave has read X
             typedef typename E::Out OT;
             OT* ore = E::out_plane(L, 0);
@@ -1525,6 +1557,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                     for (int i = threadIdx.x; i < 2 * NP * OP; i += NTHR) w[i] = 0u;
                 }
             }
+          }
         } else if constexpr (!FUSED) {
             double2* out = v.R;
             // strided-pass sources: g11 (va[m] - vb[n]) - conj g(q1,p1) (vc[n] - vc[m]),
