@@ -472,12 +472,10 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
             r.tab_list.push_back(i);
             r.tab_prefix.push_back(r.tab_prefix.back() + (int64_t)N * g12::table_blocks_per_x(N));
         }
-    // pair-symmetric update: 2 spins x N (N + 1) / 2 slice pairs x bands of 16 rows
+    // pair-symmetric update: blocks of (slice pair, band of 16 rows) units
     r.sym_prefix.assign(1, 0);
-    for (int i : r.tab_list) {
-        const int64_t N = r.ds[i].N;
-        r.sym_prefix.push_back(r.sym_prefix.back() + 2 * (N * (N + 1) / 2) * ((N + 15) / 16));
-    }
+    for (int i : r.tab_list)
+        r.sym_prefix.push_back(r.sym_prefix.back() + g12::sym_update_blocks(r.ds[i].N, r.precision));
     const int64_t total = layout(r, nullptr);
     if (hipMalloc(&r.arena, total) != hipSuccess) {
         r.arena = nullptr;

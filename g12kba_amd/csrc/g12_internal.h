@@ -118,7 +118,7 @@ struct BatchDev {
     int ncorr;
     int64_t tblocks;
     // pair-symmetric update (k_sym_update): prefix of its blocks over the
-    // correlated trajectories (2 spins x N (N + 1) / 2 slice pairs x bands of 16 rows)
+    // correlated trajectories (sym_update_blocks each)
     const int64_t* sprefix;  // device array [ncorr + 1]
     int64_t sunits;
 };
@@ -151,6 +151,8 @@ void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int f
                    int sym = 0);
 // second pass of the pair-symmetric path: C = R + P(R) and the RK update
 void launch_b_sym_update(const BatchDev& b, const StageRef& sr, int precision, hipStream_t s);
+// blocks of that pass for one trajectory of size N
+int64_t sym_update_blocks(int N, int precision);
 // flag |= 1 unless the natural-layout tensor T is pair-symmetric (and, with
 // herm, Hermitian) within tol
 void launch_pair_check(const double2* T, int N, double tol, int herm, int* flag, hipStream_t s);

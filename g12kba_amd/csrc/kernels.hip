@@ -1957,6 +1957,7 @@ __global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
 // (256 and 512 threads, and an XCD-contiguous block order, measured within
 // 2-3% of it)
 constexpr int SU_BAND = 16, SU_THREADS = 1024, SU_MAXN = 64;
+static_assert(SU_BAND * SU_MAXN == SU_THREADS, "one slot per thread and side at N = 64");
 
 // slice pairs (a <= m) in order of m: index m (m + 1) / 2 + a
 __device__ __forceinline__ void su_pair(int64_t pi, int& a, int& m) {
@@ -2008,28 +2009,54 @@ struct SuIter {
     }
 };
 
+// Slots per unit and side: the element count of a full band (the band rows of
+// one slice, or its band columns in the natural layout's 16-wide form).
+template <bool C>
+__host__ __device__ inline int su_count(int N) {
+    const int br = N < SU_BAND ? N : SU_BAND, nq = (N + 3) & ~3;
+    return C ? ((br + 3) >> 2) * 4 * nq : N * SU_BAND;
+}
+// Units per block: small N pack several (slice pair, band) units into one
+// block, so that its threads are not idle.
+template <bool C>
+__host__ __device__ inline int su_per_block(int N) {
+    const int u = SU_THREADS / su_count<C>(N);
+    return u < 1 ? 1 : u;
+}
+// LDS elements per side: the largest units-per-block x band size over N <= 64
+constexpr int SU_LDS = 1280;
+
 template <bool C>
 __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef r) {
-    constexpr int TH = SU_THREADS;
     typedef typename std::conditional<C, fv2, dv2>::type LT;
-    __shared__ LT la[SU_BAND][SU_MAXN + 1];  // slice (a, m): band rows [q1 - r0][q2]
-    __shared__ LT lb[SU_MAXN][SU_BAND + 1];  // slice (m, a): band columns [q1][q2 - r0]
+    // per unit: slice (a, m) band rows at [q1 - r0][q2] (pitch N + 1), slice
+    // (m, a) band columns at [q1][q2 - r0] (pitch BR + 1)
+    __shared__ LT la[SU_LDS];
+    __shared__ LT lb[SU_LDS];
     const StageShared ss = uni_stage(r);
-    const int64_t u = blockIdx.x;
+    const int64_t blk = blockIdx.x;
     int lo = 0, hi = b.ncorr - 1;
-    while (lo < hi) {  // last correlated trajectory whose units start at or before u
+    while (lo < hi) {  // last correlated trajectory whose blocks start at or before this one
         const int mid = (lo + hi + 1) >> 1;
-        if (b.sprefix[mid] <= u) lo = mid; else hi = mid - 1;
+        if (b.sprefix[mid] <= blk) lo = mid; else hi = mid - 1;
     }
     const int traj = uni(b.tlist[lo]);
     const DevState& d = b.ds[traj];
     if (!active(d, b.tp[traj], ss)) return;
     const int N = uni(b.tp[traj].N);
     const int nbd = (N + SU_BAND - 1) / SU_BAND;
+    const int BR = N < SU_BAND ? N : SU_BAND;
     const int64_t per_spin = (int64_t)N * (N + 1) / 2 * nbd;
-    int64_t local = u - uni64(b.sprefix[lo]);
-    const int spin = local >= per_spin ? 1 : 0;
-    local -= spin * per_spin;
+    const int cnt = su_count<C>(N), upb = su_per_block<C>(N);
+    // the block's spin (blocks never mix spins) and this thread's unit and slot
+    const int64_t bps = (per_spin + upb - 1) / upb;  // blocks per spin
+    int64_t lb0 = blk - uni64(b.sprefix[lo]);
+    const int spin = lb0 >= bps ? 1 : 0;
+    lb0 -= spin * bps;
+    const int ui = threadIdx.x / cnt, e = threadIdx.x - ui * cnt;
+    const int64_t unit = lb0 * upb + ui;
+    const bool live = ui < upb && unit < per_spin;
+    const int64_t local = live ? unit : 0;
     int a, m;
     su_pair(local / nbd, a, m);
     const int r0 = (int)(local % nbd) * SU_BAND, nr = min(SU_BAND, N - r0);
@@ -2044,9 +2071,8 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     const SuIter<C, true> rows{(uint32_t)(a * N + m) * sl, N, nq, r0, nr};   // slice (a, m)
     const SuIter<C, false> cols{(uint32_t)(m * N + a) * sl, N, nq, r0, nr};  // slice (m, a)
     const bool two = a != m;  // slice (m, a) is another slice
-    // every band holds at most SU_BAND x SU_MAXN elements: IT per thread
-    constexpr int IT = SU_BAND * SU_MAXN / TH;
-    static_assert(IT * TH == SU_BAND * SU_MAXN, "whole items per thread");
+    LT* lau = la + ui * BR * (N + 1);
+    LT* lbu = lb + ui * N * (BR + 1);
     // RK protocol of this pass (its stages see S and A only at element level):
     //   stage 0: A = w K;      stage 1: A += w K;   Y = S + c' K
     //   stage 2: Y = S + c' K, then S += dt (A + w K) / 6 (A folded into S)
@@ -2061,66 +2087,54 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
         LT xv;           // this element of X
         double2 sv, av;  // S (or Sc), A
     };
-    Item ri[IT], ci[IT];
     // with_state = false: X only (the band columns of a slice paired with itself)
-    auto gather = [&](const auto& it, int p1, int p2, Item (&w)[IT], bool with_state) {
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            const int e = threadIdx.x + i * TH;
-            w[i].ok = e < it.count() && it.at(e, w[i].x1, w[i].x2, w[i].yo);
-            w[i].o = w[i].ok ? (((uint32_t)p1 * N + p2) * N + w[i].x1) * N + w[i].x2 : 0u;
-            if constexpr (!C) w[i].yo = w[i].o;
-            w[i].sv = w[i].av = make_double2(0.0, 0.0);
-            if (w[i].ok) {
-                if constexpr (C)
-                    w[i].xv = *(const GLOBAL fv2*)((const GLOBAL char*)X + (w[i].yo << 3));
-                else
-                    w[i].xv = *(const GLOBAL dv2*)((const GLOBAL char*)X + (w[i].yo << 4));
-                if (with_state) {
-                    w[i].sv = s_c64 ? cld(Sc, w[i].yo, true) : gld(S, w[i].o);
-                    if (!ss.first && !ss.last) w[i].av = gld(A, w[i].o);
-                }
+    auto gather = [&](const auto& it, int p1, int p2, Item& w, bool with_state) {
+        w.ok = live && e < it.count() && it.at(e, w.x1, w.x2, w.yo);
+        w.o = w.ok ? (((uint32_t)p1 * N + p2) * N + w.x1) * N + w.x2 : 0u;
+        if constexpr (!C) w.yo = w.o;
+        w.sv = w.av = make_double2(0.0, 0.0);
+        if (w.ok) {
+            if constexpr (C)
+                w.xv = *(const GLOBAL fv2*)((const GLOBAL char*)X + (w.yo << 3));
+            else
+                w.xv = *(const GLOBAL dv2*)((const GLOBAL char*)X + (w.yo << 4));
+            if (with_state) {
+                w.sv = s_c64 ? cld(Sc, w.yo, true) : gld(S, w.o);
+                if (!ss.first && !ss.last) w.av = gld(A, w.o);
             }
         }
     };
     // all loads of the block (X and the RK state of both slices) in flight at once
+    Item ri, ci;
     gather(rows, a, m, ri, true);
     gather(cols, m, a, ci, two);  // for a = m the band columns of the slice itself: X only
-#pragma unroll
-    for (int i = 0; i < IT; ++i) {
-        if (ri[i].ok) la[ri[i].x1 - r0][ri[i].x2] = ri[i].xv;
-        if (ci[i].ok) lb[ci[i].x1][ci[i].x2 - r0] = ci[i].xv;
-    }
+    if (ri.ok) lau[(ri.x1 - r0) * (N + 1) + ri.x2] = ri.xv;
+    if (ci.ok) lbu[ci.x1 * (BR + 1) + (ci.x2 - r0)] = ci.xv;
     __syncthreads();
     auto tod = [](LT v) { return make_double2((double)v.x, (double)v.y); };
-    auto update = [&](const Item (&w)[IT], int p2, bool rows_side) {
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            if (!w[i].ok) continue;
-            const int x1 = w[i].x1, x2 = w[i].x2;
-            // C = R + P(R): this element of R plus the exchanged one from LDS
-            const double2 c = rows_side ? cadd(tod(la[x1 - r0][x2]), tod(lb[x2][x1 - r0]))
-                                        : cadd(tod(lb[x1][x2 - r0]), tod(la[x2 - r0][x1]));
-            const double2 K = cmul_mi(c);
-            const double2 wK = cscale(K, ss.wk);
-            double2 next;
-            if (ss.last) {
-                next = cadd(w[i].sv, cdiv6(cscale(wK, ss.dt)));
-                gst(S, w[i].o, next);
-                if constexpr (C) cst(Sc, w[i].yo, next, true);  // next step's stage-0 input
-            } else {
-                next = cadd(w[i].sv, cscale(K, ss.cn));
-                cst(Y, w[i].yo, next, C);
-                if (ss.fold)
-                    gst(S, w[i].o, cadd(w[i].sv, cdiv6(cscale(cadd(w[i].av, wK), ss.dt))));
-                else
-                    gst(A, w[i].o, ss.first ? wK : cadd(w[i].av, wK));
-            }
-            if (x2 == p2) gst(D, (((uint32_t)(rows_side ? a : m)) * N + x1) * N + p2, next);  // D[p1][q1][p2]
+    auto update = [&](const Item& w, int p1, int p2, double2 c) {
+        const double2 K = cmul_mi(c);
+        const double2 wK = cscale(K, ss.wk);
+        double2 next;
+        if (ss.last) {
+            next = cadd(w.sv, cdiv6(cscale(wK, ss.dt)));
+            gst(S, w.o, next);
+            if constexpr (C) cst(Sc, w.yo, next, true);  // next step's stage-0 input
+        } else {
+            next = cadd(w.sv, cscale(K, ss.cn));
+            cst(Y, w.yo, next, C);
+            if (ss.fold)
+                gst(S, w.o, cadd(w.sv, cdiv6(cscale(cadd(w.av, wK), ss.dt))));
+            else
+                gst(A, w.o, ss.first ? wK : cadd(w.av, wK));
         }
+        if (w.x2 == p2) gst(D, (((uint32_t)p1) * N + w.x1) * N + p2, next);  // D[p1][q1][p2]
     };
-    update(ri, m, true);
-    if (two) update(ci, a, false);
+    // C = R + P(R): this element of R plus the exchanged one from LDS
+    if (ri.ok)
+        update(ri, a, m, cadd(tod(lau[(ri.x1 - r0) * (N + 1) + ri.x2]), tod(lbu[ri.x2 * (BR + 1) + (ri.x1 - r0)])));
+    if (two && ci.ok)
+        update(ci, m, a, cadd(tod(lbu[ci.x1 * (BR + 1) + (ci.x2 - r0)]), tod(lau[(ci.x2 - r0) * (N + 1) + ci.x1])));
 }
 
 // flag |= 1 unless T is pair-symmetric (and, with herm, Hermitian) within tol,
@@ -2364,6 +2378,12 @@ void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hi
     a.k = k;
     a.compact = precision != G12_PREC_FP64;
     hipLaunchKernelGGL(k_d_from_state, dim3((unsigned)(k.tiles / 2)), dim3(64), 0, s, a);
+}
+
+int64_t sym_update_blocks(int N, int precision) {
+    const int64_t per_spin = ((int64_t)N * (N + 1) / 2) * ((N + SU_BAND - 1) / SU_BAND);
+    const int upb = precision != G12_PREC_FP64 ? su_per_block<true>(N) : su_per_block<false>(N);
+    return 2 * ((per_spin + upb - 1) / upb);  // a block never holds units of both spins
 }
 
 void launch_b_sym_update(const BatchDev& b, const StageRef& r, int precision, hipStream_t s) {
