@@ -2079,7 +2079,12 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     //   stage 3: S += dt w K / 6 (the new state; mixed runs: and its copy Sc)
     // Mixed runs take S at stages 0 and 1 from its complex64 copy Sc (the
     // stage inputs are complex64 anyway); stages 2 and 3 update S in FP64.
+    // FP64 runs write no A at stage 0: stage 1 recovers its w0 K1 (w0 = 1) from
+    // its own complex128 input, K1 = (Y - S) / c (Y = S + c K1 was written by
+    // stage 0; the recovered term is exact to the rounding of Y, a relative
+    // 1e-16 of S).
     const bool s_c64 = C && !ss.fold && !ss.last;
+    const bool k1_from_y = !C && !ss.first && !ss.fold && !ss.last;  // FP64, stage 1
     struct Item {
         int x1, x2;
         uint32_t o, yo;  // natural offset (S, A); layout offset (X, Y, Sc)
@@ -2100,7 +2105,7 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
                 w.xv = *(const GLOBAL dv2*)((const GLOBAL char*)X + (w.yo << 4));
             if (with_state) {
                 w.sv = s_c64 ? cld(Sc, w.yo, true) : gld(S, w.o);
-                if (!ss.first && !ss.last) w.av = gld(A, w.o);
+                if (!ss.first && !ss.last) w.av = gld(k1_from_y ? Y : A, w.o);
             }
         }
     };
@@ -2115,6 +2120,8 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     auto update = [&](const Item& w, int p1, int p2, double2 c) {
         const double2 K = cmul_mi(c);
         const double2 wK = cscale(K, ss.wk);
+        const double2 av =
+            k1_from_y ? make_double2((w.av.x - w.sv.x) / ss.c, (w.av.y - w.sv.y) / ss.c) : w.av;
         double2 next;
         if (ss.last) {
             next = cadd(w.sv, cdiv6(cscale(wK, ss.dt)));
@@ -2124,9 +2131,11 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
             next = cadd(w.sv, cscale(K, ss.cn));
             cst(Y, w.yo, next, C);
             if (ss.fold)
-                gst(S, w.o, cadd(w.sv, cdiv6(cscale(cadd(w.av, wK), ss.dt))));
-            else
-                gst(A, w.o, ss.first ? wK : cadd(w.av, wK));
+                gst(S, w.o, cadd(w.sv, cdiv6(cscale(cadd(av, wK), ss.dt))));
+            else if (!ss.first)
+                gst(A, w.o, cadd(av, wK));
+            else if constexpr (C)
+                gst(A, w.o, wK);
         }
         if (w.x2 == p2) gst(D, (((uint32_t)p1) * N + w.x1) * N + p2, next);  // D[p1][q1][p2]
     };
