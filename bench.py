@@ -345,14 +345,15 @@ def update_bytes_per_element(precision):
     averaged over the four stages.  Every stage reads the product pass's
     result X (16 B in FP64, 8 B in the mixed modes) and the state S (16 B; the
     mixed modes read its complex64 copy, 8 B, at stages 0 and 1).  Stages 0-2
-    write the next stage input Y (16 B / 8 B).  The RK accumulator A (16 B) is
-    written at stage 1 and read at stage 2, which folds it into S (S written);
-    the mixed modes also write it at stage 0 and read it at stage 1, where FP64
-    reads its own stage input Y instead (the stage-0 term is (Y - S) / c).
-    Stage 3 writes the new state S (and in the mixed modes its complex64 copy):
-      FP64:  48, 80, 80, 48 B at stages 0-3 -> 64 B;
+    write the next stage input Y (16 B / 8 B).  Stage 2 folds the RK terms into
+    S (S written).  The mixed modes accumulate them in A (16 B: written at
+    stage 0, read and written at stage 1, read at stage 2); FP64 keeps the
+    stage-2 input apart instead (stage 1 writes it to the A buffer) and stage 2
+    recovers K1 and K2 from the two stored inputs (16 B each).  Stage 3 writes
+    the new state S (and in the mixed modes its complex64 copy):
+      FP64:  48, 48, 96, 48 B at stages 0-3 -> 60 B;
       mixed: 40, 56, 64, 48 B               -> 52 B."""
-    return 64.0 if precision == "fp64" else 52.0
+    return 60.0 if precision == "fp64" else 52.0
 
 
 def roofline(N, precision, kk, steps):

@@ -1047,8 +1047,11 @@ struct View {
 // Mixed-precision runs keep the stage input compact at every stage: at stage 0
 // it is Sc, the complex64 copy of S (written with every new state by the last
 // stage's update, or at the start of a run).
+// FP64 runs on the pair-symmetric path keep the stage-1 and stage-2 inputs
+// apart: the stage-2 input lives in the A buffer (k_sym_update).
 __device__ inline const double2* stage_input(const DevState& d, const BTileArgs& a, const StageShared& ss, int spin) {
     if (a.use_y && a.compact) return ss.first ? d.Sc[spin] : d.Y2[spin];
+    if (a.use_y && a.sym && ss.fold) return d.A2[spin];
     return (a.use_y && !ss.first) ? d.Y2[spin] : d.S2[spin];
 }
 // the stage input is compact when it is the Y buffer of a mixed-precision run
@@ -2079,25 +2082,27 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     //   stage 3: S += dt w K / 6 (the new state; mixed runs: and its copy Sc)
     // Mixed runs take S at stages 0 and 1 from its complex64 copy Sc (the
     // stage inputs are complex64 anyway); stages 2 and 3 update S in FP64.
-    // FP64 runs write no A at stage 0: stage 1 recovers its w0 K1 (w0 = 1) from
-    // its own complex128 input, K1 = (Y - S) / c (Y = S + c K1 was written by
-    // stage 0; the recovered term is exact to the rounding of Y, a relative
-    // 1e-16 of S).
+    // FP64 runs keep no accumulator: stage 0 writes the stage-1 input Y1 to Y,
+    // stage 1 the stage-2 input Y2 to the A buffer, and stage 2 recovers
+    // K1 = (Y1 - S) / c and K2 = (Y2 - S) / c (c = dt/2 for both; exact to the
+    // rounding of Y, a relative 1e-16 of S), folds S += dt (K1 + 2 K2 + 2 K3) / 6
+    // and writes the stage-3 input over Y1.
     const bool s_c64 = C && !ss.fold && !ss.last;
-    const bool k1_from_y = !C && !ss.first && !ss.fold && !ss.last;  // FP64, stage 1
+    const bool fold64 = !C && ss.fold;  // FP64, stage 2
     struct Item {
         int x1, x2;
         uint32_t o, yo;  // natural offset (S, A); layout offset (X, Y, Sc)
         bool ok;
         LT xv;           // this element of X
-        double2 sv, av;  // S (or Sc), A
+        double2 sv, av;  // S (or Sc), A (FP64 stage 2: Y1)
+        double2 a2;      // FP64 stage 2: Y2
     };
     // with_state = false: X only (the band columns of a slice paired with itself)
     auto gather = [&](const auto& it, int p1, int p2, Item& w, bool with_state) {
         w.ok = live && e < it.count() && it.at(e, w.x1, w.x2, w.yo);
         w.o = w.ok ? (((uint32_t)p1 * N + p2) * N + w.x1) * N + w.x2 : 0u;
         if constexpr (!C) w.yo = w.o;
-        w.sv = w.av = make_double2(0.0, 0.0);
+        w.sv = w.av = w.a2 = make_double2(0.0, 0.0);
         if (w.ok) {
             if constexpr (C)
                 w.xv = *(const GLOBAL fv2*)((const GLOBAL char*)X + (w.yo << 3));
@@ -2105,7 +2110,11 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
                 w.xv = *(const GLOBAL dv2*)((const GLOBAL char*)X + (w.yo << 4));
             if (with_state) {
                 w.sv = s_c64 ? cld(Sc, w.yo, true) : gld(S, w.o);
-                if (!ss.first && !ss.last) w.av = gld(k1_from_y ? Y : A, w.o);
+                if (C && !ss.first && !ss.last) w.av = gld(A, w.o);
+                if (fold64) {
+                    w.av = gld(Y, w.o);
+                    w.a2 = gld(A, w.o);
+                }
             }
         }
     };
@@ -2120,8 +2129,12 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     auto update = [&](const Item& w, int p1, int p2, double2 c) {
         const double2 K = cmul_mi(c);
         const double2 wK = cscale(K, ss.wk);
-        const double2 av =
-            k1_from_y ? make_double2((w.av.x - w.sv.x) / ss.c, (w.av.y - w.sv.y) / ss.c) : w.av;
+        double2 av = w.av;  // A, or in FP64 at stage 2 K1 + 2 K2
+        if (fold64) {
+            const double2 k1 = make_double2((w.av.x - w.sv.x) / ss.c, (w.av.y - w.sv.y) / ss.c);
+            const double2 k2 = make_double2((w.a2.x - w.sv.x) / ss.c, (w.a2.y - w.sv.y) / ss.c);
+            av = cadd(k1, cscale(k2, 2.0));
+        }
         double2 next;
         if (ss.last) {
             next = cadd(w.sv, cdiv6(cscale(wK, ss.dt)));
@@ -2129,13 +2142,14 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
             if constexpr (C) cst(Sc, w.yo, next, true);  // next step's stage-0 input
         } else {
             next = cadd(w.sv, cscale(K, ss.cn));
-            cst(Y, w.yo, next, C);
+            if (!C && !ss.first && !ss.fold)
+                gst(A, w.o, next);  // FP64 stage 1: the stage-2 input into the A buffer
+            else
+                cst(Y, w.yo, next, C);
             if (ss.fold)
                 gst(S, w.o, cadd(w.sv, cdiv6(cscale(cadd(av, wK), ss.dt))));
-            else if (!ss.first)
-                gst(A, w.o, cadd(av, wK));
             else if constexpr (C)
-                gst(A, w.o, wK);
+                gst(A, w.o, ss.first ? wK : cadd(av, wK));
         }
         if (w.x2 == p2) gst(D, (((uint32_t)p1) * N + w.x1) * N + p2, next);  // D[p1][q1][p2]
     };
