@@ -203,14 +203,17 @@ int g12_timing_get(g12_ctx* ctx, int which, int64_t* launches, double* total_ms)
 int g12_configure(g12_ctx* ctx, int use_graphs, int autotune);
 
 /* *on = 1 when the context runs the pair-symmetric path: its correlated state
- * is symmetric under the exchange of both pairs with G1 anti-Hermitian, in
- * FP64 runs also Hermitian as an N^2 x N^2 matrix (every state evolved from
- * G2 = 0 is; checked to 1e-12 of the largest element whenever a state is set
- * or loaded), and N <= 64.  The G2 right-hand side is then the contiguous-tile half of the
- * commutator and sources plus its pair exchange: one tile-product pass instead
- * of two.  Otherwise (or with G12KBA_PAIR_SYMMETRY=0 in the environment at
- * creation) the general two-pass path runs.  No reference counterpart: the
- * reference evaluates all four commutator terms (commutators.h:53-99). */
+ * is symmetric under the exchange of both pairs and Hermitian as an N^2 x N^2
+ * matrix, with G1 anti-Hermitian (every state evolved from G2 = 0 is; checked
+ * to 1e-12 of the largest element whenever a state is set or loaded), and
+ * N <= 64.  The G2 right-hand side is then the contiguous-tile half of the
+ * commutator and sources plus its pair exchange, and its tiles (t1, t0) are
+ * the negated conjugate transposes of the tiles (t0, t1): one tile-product
+ * pass over the tiles t0 <= t1 instead of two passes over all tiles.  The path
+ * keeps both symmetries exactly.  Otherwise (or with G12KBA_PAIR_SYMMETRY=0 in
+ * the environment at creation) the general two-pass path runs.  No reference
+ * counterpart: the reference evaluates all four commutator terms
+ * (commutators.h:53-99). */
 int g12_pair_symmetric(g12_ctx* ctx, int* on);
 
 /* Waits for all work on the library's GPU (device-wide synchronisation). */
