@@ -35,8 +35,8 @@ TRAFFIC_BYTES = {("fp64", "tile_fused"): 2.995e9, ("fp64", "tile_strided"): 1.09
                  ("fp32_3m", "tile_fused"): 2.455e9, ("fp32_3m", "tile_strided"): 0.556e9,
                  ("bf16x3", "tile_fused"): 2.328e9, ("bf16x3", "tile_strided"): 0.574e9,
                  ("fp64", "tile_products"): 0.835e9, ("fp64", "pair_update"): 2.040e9,
-                 ("fp32_3m", "tile_products"): 0.475e9, ("fp32_3m", "pair_update"): 1.768e9,
-                 ("bf16x3", "tile_products"): 0.503e9, ("bf16x3", "pair_update"): 1.768e9}
+                 ("fp32_3m", "tile_products"): 0.476e9, ("fp32_3m", "pair_update"): 1.499e9,
+                 ("bf16x3", "tile_products"): 0.503e9, ("bf16x3", "pair_update"): 1.499e9}
 
 TRAFFIC_SOURCE = {("fp64", False): "profiles/tile_kernels_n64_fp64_hbm_traffic.txt",
                   ("fp32_3m", False): "profiles/tile_kernels_n64_mixed_hbm_traffic.txt",
@@ -346,14 +346,14 @@ def update_bytes_per_element(precision):
     result X (16 B in FP64, 8 B in the mixed modes) and the state S (16 B; the
     mixed modes read its complex64 copy, 8 B, at stages 0 and 1).  Stages 0-2
     write the next stage input Y (16 B / 8 B).  Stage 2 folds the RK terms into
-    S (S written).  The mixed modes accumulate them in A (16 B: written at
-    stage 0, read and written at stage 1, read at stage 2); FP64 keeps the
-    stage-2 input apart instead (stage 1 writes it to the A buffer) and stage 2
-    recovers K1 and K2 from the two stored inputs (16 B each).  Stage 3 writes
-    the new state S (and in the mixed modes its complex64 copy):
+    S (S written).  The mixed modes store K1 and K2 as complex64 (8 B each,
+    written at stages 0 and 1, read at stage 2); FP64 keeps the stage-2 input
+    apart instead (stage 1 writes it to the A buffer) and stage 2 recovers K1
+    and K2 from the two stored inputs (16 B each).  Stage 3 writes the new
+    state S (and in the mixed modes its complex64 copy):
       FP64:  48, 48, 96, 48 B at stages 0-3 -> 60 B;
-      mixed: 40, 56, 64, 48 B               -> 52 B."""
-    return 60.0 if precision == "fp64" else 52.0
+      mixed: 32, 32, 64, 48 B               -> 44 B."""
+    return 60.0 if precision == "fp64" else 44.0
 
 
 def roofline(N, precision, kk, steps):

@@ -324,7 +324,9 @@ int64_t layout(Run& r, char* base) {
             d.S2[s] = cv.take<double2>(d.n4);  // the state is kept even when frozen
             d.Y2[s] = cv.take<double2>(n4c);
             d.Sc[s] = r.precision != G12_PREC_FP64 ? cv.take<double2>(n4c) : nullptr;
-            d.A2[s] = cv.take<double2>(n4);
+            // mixed runs: also two blocked complex64 arrays (the stage terms K1, K2
+            // of the pair-symmetric update), N^2 Nq^2 complex128 slots
+            d.A2[s] = cv.take<double2>(r.precision != G12_PREC_FP64 && corr ? std::max(n4, 2 * n4c) : n4);
             d.R2[s] = cv.take<double2>(n4c);
             d.D[s] = cv.take<double2>(n3);
             d.F[s] = cv.take<double2>(n3);

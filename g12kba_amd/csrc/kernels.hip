@@ -2076,12 +2076,12 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     const bool two = a != m;  // slice (m, a) is another slice
     LT* lau = la + ui * BR * (N + 1);
     LT* lbu = lb + ui * N * (BR + 1);
-    // RK protocol of this pass (its stages see S and A only at element level):
-    //   stage 0: A = w K;      stage 1: A += w K;   Y = S + c' K
-    //   stage 2: Y = S + c' K, then S += dt (A + w K) / 6 (A folded into S)
-    //   stage 3: S += dt w K / 6 (the new state; mixed runs: and its copy Sc)
-    // Mixed runs take S at stages 0 and 1 from its complex64 copy Sc (the
-    // stage inputs are complex64 anyway); stages 2 and 3 update S in FP64.
+    // RK protocol of this pass (its stages see S and A only at element level).
+    // Mixed runs: stages 0 and 1 store their K (complex64, the precision of the
+    // FP32 products it comes from) in the A buffer, as two blocked complex64
+    // arrays K1, K2, and write Y = Sc + c' K (Sc: the complex64 copy of S; the
+    // stage inputs are complex64 anyway); stage 2 writes Y = S + c' K and folds
+    // S += dt (K1 + 2 K2 + 2 K3) / 6 in FP64; stage 3 S += dt K4 / 6, and Sc.
     // FP64 runs keep no accumulator: stage 0 writes the stage-1 input Y1 to Y,
     // stage 1 the stage-2 input Y2 to the A buffer, and stage 2 recovers
     // K1 = (Y1 - S) / c and K2 = (Y2 - S) / c (c = dt/2 for both; exact to the
@@ -2089,6 +2089,7 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     // and writes the stage-3 input over Y1.
     const bool s_c64 = C && !ss.fold && !ss.last;
     const bool fold64 = !C && ss.fold;  // FP64, stage 2
+    const uint32_t k2o = (uint32_t)(N * N) * (uint32_t)(nq * nq);  // mixed: K2 after K1 in the A buffer
     struct Item {
         int x1, x2;
         uint32_t o, yo;  // natural offset (S, A); layout offset (X, Y, Sc)
@@ -2110,7 +2111,10 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
                 w.xv = *(const GLOBAL dv2*)((const GLOBAL char*)X + (w.yo << 4));
             if (with_state) {
                 w.sv = s_c64 ? cld(Sc, w.yo, true) : gld(S, w.o);
-                if (C && !ss.first && !ss.last) w.av = gld(A, w.o);
+                if (C && ss.fold) {
+                    w.av = cld(A, w.yo, true);        // K1
+                    w.a2 = cld(A, k2o + w.yo, true);  // K2
+                }
                 if (fold64) {
                     w.av = gld(Y, w.o);
                     w.a2 = gld(A, w.o);
@@ -2129,11 +2133,13 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     auto update = [&](const Item& w, int p1, int p2, double2 c) {
         const double2 K = cmul_mi(c);
         const double2 wK = cscale(K, ss.wk);
-        double2 av = w.av;  // A, or in FP64 at stage 2 K1 + 2 K2
+        double2 av = w.av;  // at stage 2: K1 + 2 K2
         if (fold64) {
             const double2 k1 = make_double2((w.av.x - w.sv.x) / ss.c, (w.av.y - w.sv.y) / ss.c);
             const double2 k2 = make_double2((w.a2.x - w.sv.x) / ss.c, (w.a2.y - w.sv.y) / ss.c);
             av = cadd(k1, cscale(k2, 2.0));
+        } else if (C && ss.fold) {
+            av = cadd(w.av, cscale(w.a2, 2.0));
         }
         double2 next;
         if (ss.last) {
@@ -2149,7 +2155,7 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
             if (ss.fold)
                 gst(S, w.o, cadd(w.sv, cdiv6(cscale(cadd(av, wK), ss.dt))));
             else if constexpr (C)
-                gst(A, w.o, ss.first ? wK : cadd(av, wK));
+                cst(A, (ss.first ? 0u : k2o) + w.yo, K, true);  // K1 or K2
         }
         if (w.x2 == p2) gst(D, (((uint32_t)p1) * N + w.x1) * N + p2, next);  // D[p1][q1][p2]
     };
