@@ -34,9 +34,9 @@ METRIC = "G2 RK4 steps/s at N=64 (FP64 vs mixed @≤1e-6 G1 err); batched traj/s
 TRAFFIC_BYTES = {("fp64", "tile_fused"): 2.995e9, ("fp64", "tile_strided"): 1.099e9,
                  ("fp32_3m", "tile_fused"): 2.455e9, ("fp32_3m", "tile_strided"): 0.556e9,
                  ("bf16x3", "tile_fused"): 2.328e9, ("bf16x3", "tile_strided"): 0.574e9,
-                 ("fp64", "tile_products"): 0.835e9, ("fp64", "pair_update"): 2.040e9,
-                 ("fp32_3m", "tile_products"): 0.476e9, ("fp32_3m", "pair_update"): 1.499e9,
-                 ("bf16x3", "tile_products"): 0.503e9, ("bf16x3", "pair_update"): 1.499e9}
+                 ("fp64", "tile_products"): 0.610e9, ("fp64", "pair_update"): 1.116e9,
+                 ("fp32_3m", "tile_products"): 0.284e9, ("fp32_3m", "pair_update"): 0.862e9,
+                 ("bf16x3", "tile_products"): 0.283e9, ("bf16x3", "pair_update"): 0.862e9}
 
 TRAFFIC_SOURCE = {("fp64", False): "profiles/tile_kernels_n64_fp64_hbm_traffic.txt",
                   ("fp32_3m", False): "profiles/tile_kernels_n64_mixed_hbm_traffic.txt",
@@ -341,19 +341,32 @@ def observables_leg(g, G1, Gxx, Gxy, reps=5):
 
 
 def update_bytes_per_element(precision):
-    """Algorithmic HBM bytes per G2 element of one pair-exchange update launch,
-    averaged over the four stages.  Every stage reads the product pass's
-    result X (16 B in FP64, 8 B in the mixed modes) and the state S (16 B; the
-    mixed modes read its complex64 copy, 8 B, at stages 0 and 1).  Stages 0-2
-    write the next stage input Y (16 B / 8 B).  Stage 2 folds the RK terms into
-    S (S written).  The mixed modes store K1 and K2 as complex64 (8 B each,
-    written at stages 0 and 1, read at stage 2); FP64 keeps the stage-2 input
-    apart instead (stage 1 writes it to the A buffer) and stage 2 recovers K1
-    and K2 from the two stored inputs (16 B each).  Stage 3 writes the new
-    state S (and in the mixed modes its complex64 copy):
-      FP64:  48, 48, 96, 48 B at stages 0-3 -> 60 B;
-      mixed: 32, 32, 64, 48 B               -> 44 B."""
-    return 60.0 if precision == "fp64" else 44.0
+    """Algorithmic HBM bytes (read, written) per stored G2 element of one
+    pair-exchange update launch, averaged over the four stages.  Every stage
+    reads the product pass's result X (16 B in FP64, 8 B in the mixed modes)
+    and the state S (16 B; the mixed modes read its complex64 copy, 8 B, at
+    stages 0 and 1).  Stages 0-2 write the next stage input Y (16 B / 8 B).
+    Stage 2 folds the RK terms into S (S written).  The mixed modes store K1
+    and K2 as complex64 (8 B each, written at stages 0 and 1, read at stage 2);
+    FP64 keeps the stage-2 input apart instead (stage 1 writes it to the A
+    buffer) and stage 2 recovers K1 and K2 from the two stored inputs (16 B
+    each).  Stage 3 writes the new state S (and in the mixed modes its
+    complex64 copy):
+      FP64:  48, 48, 96, 48 B at stages 0-3 -> 40 B read + 20 B written;
+      mixed: 32, 32, 64, 48 B               -> 24 B read + 20 B written."""
+    return (40.0, 20.0) if precision == "fp64" else (24.0, 20.0)
+
+
+def update_bytes_per_launch(N, precision):
+    """Algorithmic bytes of one update launch over both tensors in the
+    Hermitian half layout: N (N + 1) / 2 stored tiles of N^2 elements per
+    tensor, of which the N (N - 1) / 2 lower elements of each of the N diagonal
+    tiles are written as the Hermitian images of the upper ones (written
+    only)."""
+    rd, wr = update_bytes_per_element(precision)
+    stored = N * (N + 1) // 2 * N * N
+    images = N * (N * (N - 1) // 2)
+    return 2.0 * ((stored - images) * (rd + wr) + images * wr)
 
 
 def roofline(N, precision, kk, steps):
@@ -366,7 +379,7 @@ def roofline(N, precision, kk, steps):
     complex MACs x 8 flop = 16 N^4 (N + 1) per launch (8 flop per complex MAC
     whatever the product scheme: 3M or BF16 split); the p1 and q1 terms are its
     pair exchange, which the update pass adds while it streams the RK update
-    (HBM-bound: update_bytes_per_element x 2 N^4 bytes per launch).  General
+    (HBM-bound: update_bytes_per_launch bytes per launch).  General
     path: the strided and the fused pass each apply two of the four terms to
     every tile, 32 N^5 flop.
     """
@@ -385,7 +398,7 @@ def roofline(N, precision, kk, steps):
 
     def hbm(name, n_launch, ms_total):
         avg_ms = ms_total / max(n_launch, 1)
-        nbytes = update_bytes_per_element(precision) * 2 * N ** 4
+        nbytes = update_bytes_per_launch(N, precision)
         achieved = nbytes / (avg_ms * 1e-3) / 1e12
         return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 3), "peak": HBM_PEAK_TBS,
                 "unit": "TB/s", "frac": round(achieved / HBM_PEAK_TBS, 4),

@@ -277,6 +277,13 @@ struct Run {
     bool sym_allowed = true;
     std::vector<char> sym_ok;          // per trajectory: its uploaded state qualifies
     std::vector<int64_t> sym_prefix;   // update blocks before each correlated trajectory
+    // unit tables of the update pass (su_table), one per correlated N <= 64
+    std::vector<int> su_N;
+    std::vector<std::vector<uint64_t>> su_units;
+    std::vector<std::vector<uint32_t>> su_firsts;
+    std::vector<int64_t> su_nblk;
+    std::vector<uint64_t*> su_dunit;
+    std::vector<uint32_t*> su_dfirst;
     int* d_flag = nullptr;             // device flag of the symmetry check
     hipGraphExec_t gx[2] = {nullptr, nullptr};  // one step: 0 adiabatic switching, 1 production
     void* arena = nullptr;
@@ -332,6 +339,9 @@ int64_t layout(Run& r, char* base) {
             d.F[s] = cv.take<double2>(n3);
             d.Qp[s] = cv.take<double2>(n3);
         }
+        d.su_unit = nullptr;
+        d.su_first = nullptr;
+        d.su_nblk = 0;
         d.traj_slots = r.window;
         d.traj = cv.take<double2>(d.n2 * r.window);
         d.jtraj = corr ? cv.take<double2>(d.n2 * r.window) : nullptr;
@@ -352,6 +362,18 @@ int64_t layout(Run& r, char* base) {
     r.b.tlist = cv.take<int>((int64_t)r.tab_list.size());
     r.b.tprefix = cv.take<int64_t>((int64_t)r.tab_prefix.size());
     r.b.sprefix = cv.take<int64_t>((int64_t)r.sym_prefix.size());
+    r.su_dunit.assign(r.su_N.size(), nullptr);
+    r.su_dfirst.assign(r.su_N.size(), nullptr);
+    for (size_t k = 0; k < r.su_N.size(); ++k) {
+        r.su_dunit[k] = cv.take<uint64_t>((int64_t)r.su_units[k].size());
+        r.su_dfirst[k] = cv.take<uint32_t>((int64_t)r.su_firsts[k].size());
+        for (int i = 0; i < r.ntraj; ++i)
+            if (r.tp[i].corr && r.ds[i].N == r.su_N[k]) {
+                r.ds[i].su_unit = r.su_dunit[k];
+                r.ds[i].su_first = r.su_dfirst[k];
+                r.ds[i].su_nblk = (int)r.su_nblk[k];
+            }
+    }
     r.d_flag = cv.take<int>(1);
     return cv.off;
 }
@@ -474,10 +496,23 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
             r.tab_list.push_back(i);
             r.tab_prefix.push_back(r.tab_prefix.back() + (int64_t)N * g12::table_blocks_per_x(N));
         }
-    // pair-symmetric update: blocks of (slice pair, band of 16 rows) units
+    // pair-symmetric update: one unit table per correlated N (N <= 64), the
+    // same for both spins of every trajectory of that N
+    for (int i : r.tab_list) {
+        const int N = r.ds[i].N;
+        if (N > g12::SYM_MAX_N || std::find(r.su_N.begin(), r.su_N.end(), N) != r.su_N.end()) continue;
+        r.su_N.push_back(N);
+        r.su_units.emplace_back();
+        r.su_firsts.emplace_back();
+        r.su_nblk.push_back(g12::su_table(N, r.su_units.back(), r.su_firsts.back()));
+    }
     r.sym_prefix.assign(1, 0);
-    for (int i : r.tab_list)
-        r.sym_prefix.push_back(r.sym_prefix.back() + g12::sym_update_blocks(r.ds[i].N, r.precision));
+    for (int i : r.tab_list) {
+        const int N = r.ds[i].N;
+        const auto it = std::find(r.su_N.begin(), r.su_N.end(), N);
+        const int64_t nb = it == r.su_N.end() ? 0 : r.su_nblk[it - r.su_N.begin()];
+        r.sym_prefix.push_back(r.sym_prefix.back() + 2 * nb);  // a block never holds units of both spins
+    }
     const int64_t total = layout(r, nullptr);
     if (hipMalloc(&r.arena, total) != hipSuccess) {
         r.arena = nullptr;
@@ -509,6 +544,12 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
                               sizeof(int64_t) * r.tab_prefix.size(), hipMemcpyHostToDevice, r.stream));
         HIPCHK(hipMemcpyAsync(const_cast<int64_t*>(r.b.sprefix), r.sym_prefix.data(),
                               sizeof(int64_t) * r.sym_prefix.size(), hipMemcpyHostToDevice, r.stream));
+    }
+    for (size_t k = 0; k < r.su_N.size(); ++k) {
+        HIPCHK(hipMemcpyAsync(r.su_dunit[k], r.su_units[k].data(), sizeof(uint64_t) * r.su_units[k].size(),
+                              hipMemcpyHostToDevice, r.stream));
+        HIPCHK(hipMemcpyAsync(r.su_dfirst[k], r.su_firsts[k].data(), sizeof(uint32_t) * r.su_firsts[k].size(),
+                              hipMemcpyHostToDevice, r.stream));
     }
     HIPCHK(hipStreamSynchronize(r.stream));
     r.b.ds = r.d_ds;
@@ -696,7 +737,7 @@ int run_windows(Run& r, int phase, int64_t nsteps, Fill&& fill, After&& after) {
     // partial traces of the first stage's input (S), and its complex64 copy
     // in mixed-precision runs; every fused pass then writes both for the
     // next stage input
-    for (const Bucket& k : r.buckets) g12::launch_b_d_from_state(r.b, k, r.precision, r.stream);
+    for (const Bucket& k : r.buckets) g12::launch_b_d_from_state(r.b, k, r.precision, r.stream, r.sym ? 1 : 0);
     for (int64_t w0 = 0; w0 < nsteps; w0 += r.window) {
         const int n = (int)std::min<int64_t>(r.window, nsteps - w0);
         for (int i = 0; i < n; ++i) fill(w0 + i, i, r.h_table + 4 * i);
@@ -886,9 +927,24 @@ int run_advance(Run& r, int64_t nsteps) {
     return 0;
 }
 
+// Moves trajectory j's G2 state between the natural layout (general path) and
+// the Hermitian half layout (pair-symmetric path), through its A buffer (at
+// least N^4 complex128 elements; a step-internal buffer, free between steps).
+void convert_state(Run& r, int j, bool to_half) {
+    const DevState& d = r.ds[j];
+    for (int s = 0; s < 2; ++s) {
+        (void)hipMemcpyAsync(d.A2[s], d.S2[s], sizeof(double2) * d.n4, hipMemcpyDeviceToDevice, r.stream);
+        if (to_half)
+            g12::launch_nat_to_half(d.A2[s], d.S2[s], d.N, r.stream);
+        else
+            g12::launch_half_to_nat(d.A2[s], d.S2[s], d.N, r.stream);
+    }
+}
+
 int set_state(Run& r, int i, const double* G1, const double* G2xx, const double* G2xy) {
     if (i < 0 || i >= r.ntraj || !G1) return G12_E_ARG;
     const DevState& d = r.ds[i];
+    const bool was_sym = r.sym;  // the layout of the other trajectories' states
     HIPCHK(hipMemcpyAsync(d.S1, G1, sizeof(double2) * d.n2, hipMemcpyHostToDevice, r.stream));
     const double* g2[2] = {G2xx, G2xy};
     double2* tmp = nullptr;
@@ -933,6 +989,14 @@ int set_state(Run& r, int i, const double* G1, const double* G2xx, const double*
             }
         r.sym_ok[i] = (flag == 0 && asym <= 1e-12 * mx) ? 1 : 0;
         update_sym(r);
+        // trajectory i's state is natural now, the others in the previous
+        // path's layout: bring every correlated state to the current one
+        for (int j = 0; j < r.ntraj; ++j) {
+            if (!r.tp[j].corr) continue;
+            const bool half_now = j != i && was_sym;
+            if (half_now != r.sym) convert_state(r, j, r.sym);
+        }
+        HIPCHK(hipStreamSynchronize(r.stream));
     }
     r.h_fail[i] = 0;
     r.t = 0.0;
@@ -945,15 +1009,21 @@ int get_state(Run& r, int i, double* G1, double* G2xx, double* G2xy) {
     const DevState& d = r.ds[i];
     if (G1) HIPCHK(hipMemcpyAsync(G1, d.S1, sizeof(double2) * d.n2, hipMemcpyDeviceToHost, r.stream));
     double* g2[2] = {G2xx, G2xy};
+    const bool half = r.sym && r.tp[i].corr;  // the state is in the Hermitian half layout
     double2* tmp = nullptr;
-    if ((G2xx || G2xy) && hipMalloc(&tmp, sizeof(double2) * d.n4) != hipSuccess) {
+    if ((G2xx || G2xy) && hipMalloc(&tmp, sizeof(double2) * d.n4 * (half ? 2 : 1)) != hipSuccess) {
         (void)hipStreamSynchronize(r.stream);
         return G12_E_NOMEM;
     }
     hipError_t e = hipSuccess;  // first failure; the stream is drained before tmp is freed
     for (int s = 0; s < 2 && e == hipSuccess; ++s) {
         if (!g2[s]) continue;
-        g12::launch_nat_to_ref(d.S2[s], tmp, d.Ns, d.Nb, r.stream);
+        const double2* nat = d.S2[s];
+        if (half) {
+            g12::launch_half_to_nat(d.S2[s], tmp + d.n4, d.N, r.stream);
+            nat = tmp + d.n4;
+        }
+        g12::launch_nat_to_ref(nat, tmp, d.Ns, d.Nb, r.stream);
         e = hipMemcpyAsync(g2[s], tmp, sizeof(double2) * d.n4, hipMemcpyDeviceToHost, r.stream);
     }
     const hipError_t es = hipStreamSynchronize(r.stream);
@@ -1490,7 +1560,7 @@ int state_energy(Run& r, int i, double* E) {
     if (corr && hipMalloc(&tmp, sizeof(double2) * 2 * d.n2) != hipSuccess) return G12_E_NOMEM;
     hipError_t e = hipSuccess;  // first failure; the stream is drained before tmp and the host buffers go
     if (corr) {
-        for (int s = 0; s < 2; ++s) g12::launch_pair_diag(d.S2[s], tmp + s * d.n2, d.N, r.stream);
+        for (int s = 0; s < 2; ++s) g12::launch_pair_diag(d.S2[s], tmp + s * d.n2, d.N, r.stream, r.sym ? 1 : 0);
         e = hipMemcpyAsync(pd.data(), tmp, sizeof(double2) * 2 * d.n2, hipMemcpyDeviceToHost, r.stream);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(g1.data(), d.S1, sizeof(double2) * d.n2, hipMemcpyDeviceToHost, r.stream);

@@ -3,8 +3,10 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 #include "../../include/g12kba.h"
+#include "sym_units.h"
 
 namespace g12 {
 
@@ -78,6 +80,12 @@ struct DevState {
     // mixed-precision runs: the complex64 copy of S (blocked like Y2), the
     // stage-0 input, written with every new state; null in FP64 runs
     double2* Sc[2];
+    // pair-symmetric update (k_sym_update): the unit table of this N
+    // (su_table, sym_units.h), shared by every trajectory of that N; su_nblk
+    // blocks per spin
+    const uint64_t* su_unit;
+    const uint32_t* su_first;
+    int su_nblk;
     double2 *D[2];                     // D[X][Y][Z] = T[X][Z][Y][Z] of the stage input
     double2 *F[2];                     // F[X][Y][Z]
     double2 *Qp[2];                    // Qp[X][Y][Z] = Q[Z][X][Y]
@@ -151,19 +159,23 @@ void launch_b_tile(const BatchDev& b, const Bucket& k, const StageRef& sr, int f
                    int sym = 0);
 // second pass of the pair-symmetric path: C = R + P(R) and the RK update
 void launch_b_sym_update(const BatchDev& b, const StageRef& sr, int precision, hipStream_t s);
-// blocks of that pass for one trajectory of size N
-int64_t sym_update_blocks(int N, int precision);
 // flag |= 1 unless the natural-layout tensor T is pair-symmetric (and, with
 // herm, Hermitian) within tol
 void launch_pair_check(const double2* T, int N, double tol, int herm, int* flag, hipStream_t s);
 // largest N of the pair-symmetric path
 constexpr int SYM_MAX_N = 64;
+// natural <-> Hermitian half layout of one N^4 tensor (out of place; the half
+// form's diagonal tiles are made exactly Hermitian from their upper triangles)
+void launch_nat_to_half(const double2* nat, double2* half, int N, hipStream_t s);
+void launch_half_to_nat(const double2* half, double2* nat, int N, hipStream_t s);
 // group pairs (A <= B) of 4 x 4 tiles of the pair-symmetric product pass
 __host__ __device__ inline int hgroups(int N) {
     const int ng = (N + 3) / 4;
     return ng * (ng + 1) / 2;
 }
-void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hipStream_t s);
+// partial traces D of the state and, in mixed-precision runs, its complex64
+// copy Sc; half = 1: the state is in the Hermitian half layout
+void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hipStream_t s, int half = 0);
 // FP64 tile pass of a bucket with the given products (hmode: 1 +h.X, 2 -X.h),
 // sources (0/1) and output mode: the routine-level entries' route through the
 // production epilogues (strided pass: R = [O +] sources; fused pass: the RK
@@ -189,7 +201,7 @@ constexpr int COMM2_DESC_BYTES = 4096;
 int enqueue_comm2(const double2* T, const double2* hx, const double2* hy, double2* out, int N, int precision,
                   void* desc_area, hipStream_t s);
 void launch_comm1(const double2* G, const double2* h, double2* out, int N, hipStream_t s);
-void launch_pair_diag(const double2* T, double2* out, int N, hipStream_t s);
+void launch_pair_diag(const double2* T, double2* out, int N, hipStream_t s, int half = 0);
 void launch_g1_step(const double2* comm, const double2* I, double2* K, double2* RK, double w, int N,
                     hipStream_t s);
 // out = comm + psi + pi - conj(pi at the particle-swapped transposed element)

@@ -19,9 +19,11 @@
 // contiguous pass then finishes K and applies the RK4 update for its tile.
 #include "g12_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <type_traits>
+#include <vector>
 
 namespace g12 {
 namespace {
@@ -998,9 +1000,8 @@ struct TileRef {
 // HALF: the pair-symmetric pass's list of tiles (t0, t1) with t0 <= t1, in
 // groups of 4 x 4: slot 16 g + 4 i + j of a spin is t0 = 4 A + i, t1 = 4 B + j
 // for the group pair g = B (B + 1) / 2 + A (A <= B); slots with t0 > t1 or
-// past N are unused.  The 16 tiles of a group write whole 128-byte lines of
-// the blocked complex64 result, directly (4 consecutive t1 at each t0) and
-// mirrored (4 consecutive t0 at each t1).  16 hgroups(N) slots per spin.
+// past N are unused.  Neighbouring workgroups then share the rows of the G1
+// planes and source tables their tiles read.  16 hgroups(N) slots per spin.
 template <bool HALF = false>
 __device__ inline TileRef ref_of(const Cursor& c, int64_t gt) {
     TileRef t;
@@ -1149,11 +1150,24 @@ __device__ __forceinline__ TileGeom tile_geom(int tiling, int N, int tile, bool 
     return g;
 }
 
+// Contiguous tile (t0, t1), t0 <= t1, in the Hermitian half layout of the
+// pair-symmetric path (hblock): rows p2 at a pitch of N, columns q2
+// contiguous, for complex128 and complex64 buffers alike (blk = nat).
+__device__ __forceinline__ TileGeom half_geom(int N, int tile) {
+    TileGeom g;
+    g.t0 = tile / N;
+    g.t1 = tile % N;
+    g.base = hblock(g.t0, g.t1) * (uint32_t)(N * N);
+    g.sm = (uint32_t)N;
+    g.B = 1;
+    g.A = 4;
+    return g;
+}
 
 // Stage input of one tile into registers (TILING: 0 strided, 1 contiguous,
-// -1 given at run time).  Plain loads: nothing waits for them before the
-// MFMA loop.
-template <int NP, int NW, int TILING, bool FULL>
+// -1 given at run time; HALF: the Hermitian half layout).  Plain loads:
+// nothing waits for them before the MFMA loop.
+template <int NP, int NW, int TILING, bool FULL, bool HALF = false>
 __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef& t, const double2* X, bool xc,
                                               double2 (&pre)[TileCfg<NP, NW>::PF]) {
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
@@ -1163,7 +1177,7 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
     const int tiling = TILING < 0 ? a.tiling : TILING;
     // one branch on the storage form, outside the unrolled loads
     if (xc) {
-        const TileGeom g = tile_geom(tiling, N, t.tile, BLK_Y);
+        const TileGeom g = HALF ? half_geom(N, t.tile) : tile_geom(tiling, N, t.tile, BLK_Y);
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
@@ -1172,7 +1186,7 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
             pre[i] = y;
         }
     } else {
-        const TileGeom g = tile_geom(tiling, N, t.tile, false);
+        const TileGeom g = HALF ? half_geom(N, t.tile) : tile_geom(tiling, N, t.tile, false);
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int e = threadIdx.x + NTHR * i;
@@ -1187,14 +1201,14 @@ __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef&
 // products take FP32 or BF16 operands): a compact stage input is loaded as it
 // is stored, with nothing waiting for it; a complex128 input (routine-level
 // calls only) is rounded on arrival.
-template <int NP, int NW, int TILING, bool FULL>
+template <int NP, int NW, int TILING, bool FULL, bool HALF = false>
 __device__ __forceinline__ void tile_prefetch(const BTileArgs& a, const TileRef& t, const double2* X, bool xc,
                                               fv2 (&pre)[TileCfg<NP, NW>::PF]) {
     constexpr int NTHR = TileCfg<NP, NW>::NTHR, PF = TileCfg<NP, NW>::PF;
     // every slot holds a tile element only when the threads divide NP^2
     constexpr bool EXACT = FULL && (NP * NP) % NTHR == 0;
     const int N = FULL ? NP : t.N;
-    const TileGeom g = tile_geom(TILING < 0 ? a.tiling : TILING, N, t.tile, xc && BLK_Y);
+    const TileGeom g = HALF ? half_geom(N, t.tile) : tile_geom(TILING < 0 ? a.tiling : TILING, N, t.tile, xc && BLK_Y);
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
         const int e = threadIdx.x + NTHR * i;
@@ -1332,7 +1346,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         const double2* X = (t.traj == v.traj && t.spin == v.spin)
                                ? v.X
                                : uni_ptr(stage_input(a.b.ds[t.traj], a, ss, t.spin));
-        tile_prefetch<NP, NW, TILING, FULL>(a, t, X, input_compact(a, ss), P);
+        tile_prefetch<NP, NW, TILING, FULL, SYM>(a, t, X, input_compact(a, ss), P);
     };
     prefetch(tr, P0);
     int64_t g1 = -1;  // DEEP: the tile after tr, in flight in the other buffer
@@ -1471,15 +1485,12 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
         });
 
         if constexpr (SYM) {
-          // t0 != t1: the tile and its mirror -(.)^dagger leave straight from
-          // the MFMA layout (complex128: 256-byte rows of the tile, 64-byte
-          // pieces of the mirror's rows; blocked complex64: the four j of a
-          // lane complete each 128-byte block); t0 = t1, and BF16x3 (whose
-          // pass ran 23% slower with direct stores), through LDS
+          // the tile (t0, t1) of R in the Hermitian half layout (its mirror
+          // (t1, t0) = -(.)^dagger is not stored).  t0 != t1: straight from
+          // the MFMA layout (256-byte complex128 or 128-byte complex64 rows);
+          // t0 = t1, and BF16x3, through LDS
           if (PREC != G12_PREC_BF16X3 && t0 != t1) {
-            const bool cb = a.compact && BLK_R;
-            const TileGeom ga = tile_geom(1, N, t0 * N + t1, cb);
-            const TileGeom gb = tile_geom(1, N, t1 * N + t0, cb);
+            const TileGeom ga = half_geom(N, t0 * N + t1);
 #pragma unroll
             for (int u = 0; u < PW; ++u) {
                 const int t = wave + NW * u;
@@ -1493,13 +1504,7 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                     double2 val = make_double2(ar[u][j], ai[u][j]);
                     if (a.sources)  // g22 (ea[p2] - eb[q2])
                         val = cadd(val, cmul(gld(v.G1, (uint32_t)(m * N + n)), csub(srcv[0][m], bn)));
-                    if (a.compact) {
-                        cst(v.R, BLK_R ? ga.blk(m, n) : ga.nat(m, n), val, true);
-                        cst(v.R, BLK_R ? gb.blk(n, m) : gb.nat(n, m), make_double2(-val.x, val.y), true);
-                    } else {
-                        gst(v.R, ga.nat(m, n), val);
-                        gst(v.R, gb.nat(n, m), make_double2(-val.x, val.y));
-                    }
+                    cst(v.R, ga.nat(m, n), val, a.compact);
                 }
             }
           } else {
@@ -1527,23 +1532,21 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
                 }
             }
             lds_sync();
-            // rows of tile (t0, t1) and of its mirror (t1, t0) = -(.)^dagger
+            // rows of tile (t0, t1); t0 = t1: its anti-Hermitian part (R - R^dagger) / 2
             auto write_out = [&](auto compact) {
                 constexpr bool C = decltype(compact)::value;
-                const TileGeom ga = tile_geom(1, N, t0 * N + t1, C && BLK_R);
-                const TileGeom gb = tile_geom(1, N, t1 * N + t0, C && BLK_R);
+                const TileGeom ga = half_geom(N, t0 * N + t1);
 #pragma unroll
                 for (int i = 0; i < PF; ++i) {
                     const int e = threadIdx.x + NTHR * i;
                     if (!(EXACT || e < N * N)) continue;
                     const int m = e / N, n = e % N;
                     const double vr = ore[m * OP + n], vi = oim[m * OP + n];
-                    const double wr = ore[n * OP + m], wi = oim[n * OP + m];
                     if (t0 == t1) {
-                        cst(v.R, ga.at<C && BLK_R>(m, n), make_double2((vr - wr) * 0.5, (vi + wi) * 0.5), C);
+                        const double wr = ore[n * OP + m], wi = oim[n * OP + m];
+                        cst(v.R, ga.nat(m, n), make_double2((vr - wr) * 0.5, (vi + wi) * 0.5), C);
                     } else {
-                        cst(v.R, ga.at<C && BLK_R>(m, n), make_double2(vr, vi), C);
-                        cst(v.R, gb.at<C && BLK_R>(m, n), make_double2(-wr, wi), C);
+                        cst(v.R, ga.nat(m, n), make_double2(vr, vi), C);
                     }
                 }
             };
@@ -1903,10 +1906,12 @@ __global__ __launch_bounds__(WTHR) void k_tile_wide(BTileArgs a) {
 
 // Start of a run: partial traces of the state (the stage-0 input)
 // D[X][Y][Z] = S[X][Z][Y][Z] for both spins and, in mixed-precision runs, the
-// complex64 copy of S in Y that the tile passes read as the stage-0 input;
+// complex64 copy Sc of S that the tile passes read as the stage-0 input;
 // within a run each fused pass writes both for the next stage input.  One
 // block per slice (p1, q1) = (X, Y), blocks in the bucket's tile order (spin 0
-// tiles).
+// tiles).  a.sym: S (and Sc) in the Hermitian half layout, where
+// S[X][Z][Y][Z] is element (Z, Z) of block (X, Y), or for X > Y the conjugate
+// of that of block (Y, X).
 __global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
     const int64_t b = blockIdx.x;
     int lo = 0, hi = a.k.n - 1;
@@ -1921,6 +1926,20 @@ __global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
     if (tile >= N * N) return;
     const int X = tile / N, Y = tile % N;
     const int64_t N2 = (int64_t)N * N;
+    if (a.sym) {
+        const int lo2 = X < Y ? X : Y, hi2 = X < Y ? Y : X;
+        const int64_t base = (int64_t)hblock(lo2, hi2) * N2;
+        for (int Z = threadIdx.x; Z < N; Z += blockDim.x)
+            for (int s = 0; s < 2; ++s) {
+                const double2 v = gload(d.S2[s], base + (int64_t)Z * N + Z);
+                gstore(d.D[s], ((int64_t)X * N + Y) * N + Z, X <= Y ? v : cconj(v));
+            }
+        if (a.compact && X <= Y)
+            for (int s = 0; s < 2; ++s)
+                for (int64_t e = threadIdx.x; e < N2; e += blockDim.x)
+                    cstore(d.Sc[s], base + e, gload(d.S2[s], base + e), true);
+        return;
+    }
     for (int Z = threadIdx.x; Z < N; Z += blockDim.x) {
         const int64_t src = (int64_t)X * N2 * N + (int64_t)Z * N2 + (int64_t)Y * N + Z;
         const int64_t dst = ((int64_t)X * N + Y) * N + Z;
@@ -1948,94 +1967,20 @@ __global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
 // Qp[p2][q2][q1]) the exchanged g22 term, Pi^+ the exchanged Pi (F is
 // Hermitian in its first two indices when G2 is Hermitian), and the Psi term
 // g11 g22 (W[p1][p2] - W[q1][q2]) is its own exchange, half of it in X.
-// One block per pair of slices {(a, m), (m, a)}, a <= m, and band of 16 rows:
-// slice (a, m) takes its band rows plus the band columns of slice (m, a)
-// transposed, slice (m, a) the reverse (a = m: the band rows only).  Both
-// pieces go through LDS, so every global access runs along rows (complex128)
-// or along the 4 x 4 blocks (complex64).  Then the fused pass's update:
-// K = -i C, A += w K, next stage input Y = S + c' K (the last stage:
-// S += dt (A + w K) / 6 and Y its complex64 copy), and the partial traces D
-// of the next stage input.
-// 1024 threads per block: one element of each side per thread at N = 64
-// (256 and 512 threads, and an XCD-contiguous block order, measured within
-// 2-3% of it)
-constexpr int SU_BAND = 16, SU_THREADS = 1024, SU_MAXN = 64;
-static_assert(SU_BAND * SU_MAXN == SU_THREADS, "one slot per thread and side at N = 64");
-
-// slice pairs (a <= m) in order of m: index m (m + 1) / 2 + a
-__device__ __forceinline__ void su_pair(int64_t pi, int& a, int& m) {
-    int mm = (int)((sqrt(8.0 * (double)pi + 1.0) - 1.0) * 0.5);
-    while ((int64_t)(mm + 1) * (mm + 2) / 2 <= pi) ++mm;
-    while ((int64_t)mm * (mm + 1) / 2 > pi) --mm;
-    m = mm;
-    a = (int)(pi - (int64_t)mm * (mm + 1) / 2);
-}
-
-// Elements of the band rows q1 in [r0, r0 + nr) (ROWS) or the band columns
-// q2 in [r0, r0 + nr) (!ROWS) of one slice, enumerated in memory order of the
-// layout: e -> (q1, q2, offset within the tensor), false for padding.
-template <bool C, bool ROWS>
-struct SuIter {
-    uint32_t base;  // slice start: (p1 N + p2) Nq^2 (blocked) or (p1 N + p2) N^2
-    int N, nq, r0, nr;
-    __device__ int count() const {
-        const int nb = (nr + 3) >> 2;  // block rows / columns of the band
-        if constexpr (C) return ROWS ? nb * 4 * nq : (nq >> 2) * nb * 16;
-        return ROWS ? nr * N : N * SU_BAND;
-    }
-    __device__ bool at(int e, int& q1, int& q2, uint32_t& off) const {
-        if constexpr (C) {
-            const int nb = (nr + 3) >> 2;
-            if constexpr (ROWS) {
-                const int rg = e / (4 * nq), w0 = e - rg * 4 * nq, w = w0 & 15;
-                q1 = r0 + 4 * rg + (w >> 2);
-                q2 = 4 * (w0 >> 4) + (w & 3);
-                off = base + (uint32_t)((r0 >> 2) + rg) * 4 * nq + w0;
-            } else {
-                const int B1 = e / (nb * 16), w0 = e - B1 * nb * 16, w = w0 & 15;
-                q1 = 4 * B1 + (w >> 2);
-                q2 = r0 + 4 * (w0 >> 4) + (w & 3);
-                off = base + (uint32_t)B1 * 4 * nq + (uint32_t)((r0 >> 2) + (w0 >> 4)) * 16 + w;
-            }
-            return ROWS ? (q1 < r0 + nr && q2 < N) : (q1 < N && q2 < r0 + nr);
-        } else {
-            if constexpr (ROWS) {
-                q1 = r0 + e / N;
-                q2 = e - (q1 - r0) * N;
-            } else {
-                q1 = e / SU_BAND;
-                q2 = r0 + (e - q1 * SU_BAND);
-            }
-            off = base + (uint32_t)q1 * N + q2;
-            return ROWS || q2 - r0 < nr;
-        }
-    }
-};
-
-// Slots per unit and side: the element count of a full band (the band rows of
-// one slice, or its band columns in the natural layout's 16-wide form).
-template <bool C>
-__host__ __device__ inline int su_count(int N) {
-    const int br = N < SU_BAND ? N : SU_BAND, nq = (N + 3) & ~3;
-    return C ? ((br + 3) >> 2) * 4 * nq : N * SU_BAND;
-}
-// Units per block: small N pack several (slice pair, band) units into one
-// block, so that its threads are not idle.
-template <bool C>
-__host__ __device__ inline int su_per_block(int N) {
-    const int u = SU_THREADS / su_count<C>(N);
-    return u < 1 ? 1 : u;
-}
-// LDS elements per side: the largest units-per-block x band size over N <= 64
-constexpr int SU_LDS = 1280;
-
+//
+// Every buffer of the pass is in the Hermitian half layout (hblock).  The
+// stored elements split into the direct and the mirrored family, each a
+// blocked transpose of units (sym_units.h); units are packed into blocks of
+// 1024 slots on the host (su_table).  Each thread takes one element of each
+// side of its unit; both sides go through LDS.  Then the RK update of every
+// element with K = -i C, the partial traces D of the next stage input, and the
+// Hermitian images in the diagonal blocks.
 template <bool C>
 __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef r) {
     typedef typename std::conditional<C, fv2, dv2>::type LT;
-    // per unit: slice (a, m) band rows at [q1 - r0][q2] (pitch N + 1), slice
-    // (m, a) band columns at [q1][q2 - r0] (pitch BR + 1)
-    __shared__ LT la[SU_LDS];
-    __shared__ LT lb[SU_LDS];
+    __shared__ LT la[SU_LA];
+    __shared__ LT lb[SU_LB];
+    __shared__ uint64_t ud[SU_MAXU];
     const StageShared ss = uni_stage(r);
     const int64_t blk = blockIdx.x;
     int lo = 0, hi = b.ncorr - 1;
@@ -2047,40 +1992,33 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     const DevState& d = b.ds[traj];
     if (!active(d, b.tp[traj], ss)) return;
     const int N = uni(b.tp[traj].N);
-    const int nbd = (N + SU_BAND - 1) / SU_BAND;
-    const int BR = N < SU_BAND ? N : SU_BAND;
-    const int64_t per_spin = (int64_t)N * (N + 1) / 2 * nbd;
-    const int cnt = su_count<C>(N), upb = su_per_block<C>(N);
-    // the block's spin (blocks never mix spins) and this thread's unit and slot
-    const int64_t bps = (per_spin + upb - 1) / upb;  // blocks per spin
-    int64_t lb0 = blk - uni64(b.sprefix[lo]);
-    const int spin = lb0 >= bps ? 1 : 0;
-    lb0 -= spin * bps;
-    const int ui = threadIdx.x / cnt, e = threadIdx.x - ui * cnt;
-    const int64_t unit = lb0 * upb + ui;
-    const bool live = ui < upb && unit < per_spin;
-    const int64_t local = live ? unit : 0;
-    int a, m;
-    su_pair(local / nbd, a, m);
-    const int r0 = (int)(local % nbd) * SU_BAND, nr = min(SU_BAND, N - r0);
+    const int nblk = uni(d.su_nblk);
+    int64_t bb = blk - uni64(b.sprefix[lo]);
+    const int spin = bb >= nblk ? 1 : 0;  // blocks never mix spins
+    bb -= spin * nblk;
+    const uint32_t* first = uni_ptr(d.su_first);
+    const uint32_t f0 = (uint32_t)uni((int)first[bb]), f1 = (uint32_t)uni((int)first[bb + 1]);
+    const int cnt = (int)(f1 - f0);
+    if ((int)threadIdx.x < cnt) ud[threadIdx.x] = uni_ptr(d.su_unit)[f0 + threadIdx.x];
+    __syncthreads();
+    // this thread's unit: the last one whose slots start at or before it
+    const int t = threadIdx.x;
+    const SuUnit u = su_decode(ud[su_find(ud, cnt, t)], N);
+    const int e = t - u.so;
+    const bool live = e < u.R * u.L;
     const double2* X = uni_ptr((const double2*)d.R2[spin]);
     double2* S = uni_ptr(d.S2[spin]);
     double2* A = uni_ptr(d.A2[spin]);
     double2* Y = uni_ptr(d.Y2[spin]);
     double2* Sc = uni_ptr(d.Sc[spin]);  // mixed runs: complex64 copy of S
     double2* D = uni_ptr(d.D[spin]);
-    const int nq = (N + 3) & ~3;
-    const uint32_t sl = C ? (uint32_t)(nq * nq) : (uint32_t)(N * N);
-    const SuIter<C, true> rows{(uint32_t)(a * N + m) * sl, N, nq, r0, nr};   // slice (a, m)
-    const SuIter<C, false> cols{(uint32_t)(m * N + a) * sl, N, nq, r0, nr};  // slice (m, a)
-    const bool two = a != m;  // slice (m, a) is another slice
-    LT* lau = la + ui * BR * (N + 1);
-    LT* lbu = lb + ui * N * (BR + 1);
+    const bool mir = u.type != 0;
+    const SuElem q = su_elem(u, live ? e : 0, N);
     // RK protocol of this pass (its stages see S and A only at element level).
     // Mixed runs: stages 0 and 1 store their K (complex64, the precision of the
-    // FP32 products it comes from) in the A buffer, as two blocked complex64
-    // arrays K1, K2, and write Y = Sc + c' K (Sc: the complex64 copy of S; the
-    // stage inputs are complex64 anyway); stage 2 writes Y = S + c' K and folds
+    // FP32 products it comes from) in the A buffer, as two complex64 arrays
+    // K1, K2, and write Y = Sc + c' K (Sc: the complex64 copy of S; the stage
+    // inputs are complex64 anyway); stage 2 writes Y = S + c' K and folds
     // S += dt (K1 + 2 K2 + 2 K3) / 6 in FP64; stage 3 S += dt K4 / 6, and Sc.
     // FP64 runs keep no accumulator: stage 0 writes the stage-1 input Y1 to Y,
     // stage 1 the stage-2 input Y2 to the A buffer, and stage 2 recovers
@@ -2089,48 +2027,55 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
     // and writes the stage-3 input over Y1.
     const bool s_c64 = C && !ss.fold && !ss.last;
     const bool fold64 = !C && ss.fold;  // FP64, stage 2
-    const uint32_t k2o = (uint32_t)(N * N) * (uint32_t)(nq * nq);  // mixed: K2 after K1 in the A buffer
+    const uint32_t k2o = hblock(0, N) * (uint32_t)(N * N);  // mixed: K2 after K1 in the A buffer (N (N + 1) / 2 blocks)
     struct Item {
-        int x1, x2;
-        uint32_t o, yo;  // natural offset (S, A); layout offset (X, Y, Sc)
+        uint32_t o;
         bool ok;
         LT xv;           // this element of X
-        double2 sv, av;  // S (or Sc), A (FP64 stage 2: Y1)
-        double2 a2;      // FP64 stage 2: Y2
+        double2 sv, av;  // S (or Sc); mixed stage 2: K1; FP64 stage 2: Y1
+        double2 a2;      // mixed stage 2: K2; FP64 stage 2: Y2
     };
-    // with_state = false: X only (the band columns of a slice paired with itself)
-    auto gather = [&](const auto& it, int p1, int p2, Item& w, bool with_state) {
-        w.ok = live && e < it.count() && it.at(e, w.x1, w.x2, w.yo);
-        w.o = w.ok ? (((uint32_t)p1 * N + p2) * N + w.x1) * N + w.x2 : 0u;
-        if constexpr (!C) w.yo = w.o;
+    auto gather = [&](uint32_t o, Item& w, bool with_state) {
+        w.ok = live;
+        w.o = o;
         w.sv = w.av = w.a2 = make_double2(0.0, 0.0);
         if (w.ok) {
             if constexpr (C)
-                w.xv = *(const GLOBAL fv2*)((const GLOBAL char*)X + (w.yo << 3));
+                w.xv = *(const GLOBAL fv2*)((const GLOBAL char*)X + (o << 3));
             else
-                w.xv = *(const GLOBAL dv2*)((const GLOBAL char*)X + (w.yo << 4));
+                w.xv = *(const GLOBAL dv2*)((const GLOBAL char*)X + (o << 4));
             if (with_state) {
-                w.sv = s_c64 ? cld(Sc, w.yo, true) : gld(S, w.o);
+                w.sv = s_c64 ? cld(Sc, o, true) : gld(S, o);
                 if (C && ss.fold) {
-                    w.av = cld(A, w.yo, true);        // K1
-                    w.a2 = cld(A, k2o + w.yo, true);  // K2
+                    w.av = cld(A, o, true);        // K1
+                    w.a2 = cld(A, k2o + o, true);  // K2
                 }
                 if (fold64) {
-                    w.av = gld(Y, w.o);
-                    w.a2 = gld(A, w.o);
+                    w.av = gld(Y, o);
+                    w.a2 = gld(A, o);
                 }
             }
         }
     };
-    // all loads of the block (X and the RK state of both slices) in flight at once
-    Item ri, ci;
-    gather(rows, a, m, ri, true);
-    gather(cols, m, a, ci, two);  // for a = m the band columns of the slice itself: X only
-    if (ri.ok) lau[(ri.x1 - r0) * (N + 1) + ri.x2] = ri.xv;
-    if (ci.ok) lbu[ci.x1 * (BR + 1) + (ci.x2 - r0)] = ci.xv;
+    // all loads of the block (X and the RK state of both sides) in flight at once
+    Item w1, w2;
+    gather(q.o1, w1, true);
+    gather(q.o2, w2, q.w2);  // x = y: side 2 is read for X only
+    LT* lau = la + u.la;
+    LT* lbu = lb + u.lb;
+    if (live) {
+        lau[q.i1 * (u.L + 1) + q.j1] = w1.xv;
+        lbu[q.j2 * (u.R + 1) + q.i2] = w2.xv;
+    }
     __syncthreads();
+    if (!live) return;
     auto tod = [](LT v) { return make_double2((double)v.x, (double)v.y); };
-    auto update = [&](const Item& w, int p1, int p2, double2 c) {
+    // a complex64 or complex128 store and, when there is one, its Hermitian image
+    auto put = [&](double2* p, bool c64, uint32_t o, uint32_t im, double2 v) {
+        cst(p, o, v, c64);
+        if (im != SU_NONE) cst(p, im, cconj(v), c64);
+    };
+    auto update = [&](const Item& w, uint32_t im, int bp, int bq, int z, double2 c) {
         const double2 K = cmul_mi(c);
         const double2 wK = cscale(K, ss.wk);
         double2 av = w.av;  // at stage 2: K1 + 2 K2
@@ -2144,26 +2089,31 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
         double2 next;
         if (ss.last) {
             next = cadd(w.sv, cdiv6(cscale(wK, ss.dt)));
-            gst(S, w.o, next);
-            if constexpr (C) cst(Sc, w.yo, next, true);  // next step's stage-0 input
+            put(S, false, w.o, im, next);
+            if constexpr (C) put(Sc, true, w.o, im, next);  // next step's stage-0 input
         } else {
             next = cadd(w.sv, cscale(K, ss.cn));
             if (!C && !ss.first && !ss.fold)
-                gst(A, w.o, next);  // FP64 stage 1: the stage-2 input into the A buffer
+                put(A, false, w.o, im, next);  // FP64 stage 1: the stage-2 input into the A buffer
             else
-                cst(Y, w.yo, next, C);
+                put(Y, C, w.o, im, next);
             if (ss.fold)
-                gst(S, w.o, cadd(w.sv, cdiv6(cscale(cadd(av, wK), ss.dt))));
+                put(S, false, w.o, im, cadd(w.sv, cdiv6(cscale(cadd(av, wK), ss.dt))));
             else if constexpr (C)
-                cst(A, (ss.first ? 0u : k2o) + w.yo, K, true);  // K1 or K2
+                put(A, true, (ss.first ? 0u : k2o) + w.o, im == SU_NONE ? SU_NONE : (ss.first ? 0u : k2o) + im, K);  // K1 or K2
         }
-        if (w.x2 == p2) gst(D, (((uint32_t)p1) * N + w.x1) * N + p2, next);  // D[p1][q1][p2]
+        if (z >= 0) {  // D[bp][bq][z] = T[bp][z][bq][z], and its Hermitian mirror
+            gst(D, ((uint32_t)bp * N + bq) * N + z, next);
+            if (bp != bq) gst(D, ((uint32_t)bq * N + bp) * N + z, cconj(next));
+        }
     };
-    // C = R + P(R): this element of R plus the exchanged one from LDS
-    if (ri.ok)
-        update(ri, a, m, cadd(tod(lau[(ri.x1 - r0) * (N + 1) + ri.x2]), tod(lbu[ri.x2 * (BR + 1) + (ri.x1 - r0)])));
-    if (two && ci.ok)
-        update(ci, m, a, cadd(tod(lbu[ci.x1 * (BR + 1) + (ci.x2 - r0)]), tod(lau[(ci.x2 - r0) * (N + 1) + ci.x1])));
+    // C = X + exchanged X: the partner from LDS, as it is (direct) or -conj (mirrored)
+    auto pair = [&](LT v) {
+        const double2 p = tod(v);
+        return mir ? make_double2(-p.x, p.y) : p;
+    };
+    update(w1, q.im1, q.bp1, q.bq1, q.z1, cadd(tod(w1.xv), pair(lbu[q.j1 * (u.R + 1) + q.i1])));
+    if (q.w2) update(w2, q.im2, q.bp2, q.bq2, q.z2, cadd(tod(w2.xv), pair(lau[q.i2 * (u.L + 1) + q.j2])));
 }
 
 // flag |= 1 unless T is pair-symmetric (and, with herm, Hermitian) within tol,
@@ -2328,13 +2278,42 @@ __global__ void k_comm1(const double2* G, const double2* h, double2* out, int N)
 }
 
 // Pair-diagonal elements M[(X,Y);(X,Y)] = T[X][Y][X][Y] of a G2 tensor in the
-// natural layout (the correlation energy's only input from G2).
-__global__ void k_pair_diag(const double2* T, double2* out, int N) {
+// natural layout, or (half) element (Y, Y) of block (X, X) of the Hermitian
+// half layout (the correlation energy's only input from G2).
+__global__ void k_pair_diag(const double2* T, double2* out, int N, int half) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t N2 = (int64_t)N * N;
     if (e >= N2) return;
     const int64_t X = e / N, Y = e % N;
-    out[e] = T[((X * N + Y) * N + X) * N + Y];
+    out[e] = half ? T[(int64_t)hblock((int)X, (int)X) * N2 + Y * N + Y] : T[((X * N + Y) * N + X) * N + Y];
+}
+
+// natural -> Hermitian half layout: one block per tile (p1, q1), p1 <= q1;
+// a diagonal tile takes its upper triangle and the conjugate transpose of it
+__global__ void k_nat_to_half(const double2* nat, double2* half, int N) {
+    const int q1 = blockIdx.x / N, p1 = blockIdx.x % N;
+    if (p1 > q1) return;
+    const int64_t N2 = (int64_t)N * N;
+    const int64_t out0 = (int64_t)hblock(p1, q1) * N2;
+    for (int64_t e = threadIdx.x; e < N2; e += blockDim.x) {
+        const int p2 = (int)(e / N), q2 = (int)(e % N);
+        const bool low = p1 == q1 && p2 > q2;  // take (q2, p2) conjugated
+        const int r = low ? q2 : p2, c = low ? p2 : q2;
+        const double2 v = nat[(((int64_t)p1 * N + r) * N + q1) * N + c];
+        half[out0 + e] = low ? cconj(v) : v;
+    }
+}
+
+// Hermitian half -> natural layout: one block per natural slice (p1, p2)
+__global__ void k_half_to_nat(const double2* half, double2* nat, int N) {
+    const int p1 = blockIdx.x / N, p2 = blockIdx.x % N;
+    const int64_t N2 = (int64_t)N * N;
+    for (int64_t e = threadIdx.x; e < N2; e += blockDim.x) {
+        const int q1 = (int)(e / N), q2 = (int)(e % N);
+        const double2 v = p1 <= q1 ? half[(int64_t)hblock(p1, q1) * N2 + (int64_t)p2 * N + q2]
+                                   : cconj(half[(int64_t)hblock(q1, p1) * N2 + (int64_t)q2 * N + p2]);
+        nat[(int64_t)blockIdx.x * N2 + e] = v;
+    }
 }
 
 __global__ void k_g1_step(const double2* comm, const double2* I, double2* K, double2* RK, double w, int N) {
@@ -2400,19 +2379,14 @@ int tile_wave_options(int NP, int* opts) {
     return 3;
 }
 
-void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hipStream_t s) {
+void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hipStream_t s, int half) {
     if (k.tiles <= 0) return;
     BTileArgs a{};
     a.b = b;
     a.k = k;
     a.compact = precision != G12_PREC_FP64;
+    a.sym = half;
     hipLaunchKernelGGL(k_d_from_state, dim3((unsigned)(k.tiles / 2)), dim3(64), 0, s, a);
-}
-
-int64_t sym_update_blocks(int N, int precision) {
-    const int64_t per_spin = ((int64_t)N * (N + 1) / 2) * ((N + SU_BAND - 1) / SU_BAND);
-    const int upb = precision != G12_PREC_FP64 ? su_per_block<true>(N) : su_per_block<false>(N);
-    return 2 * ((per_spin + upb - 1) / upb);  // a block never holds units of both spins
 }
 
 void launch_b_sym_update(const BatchDev& b, const StageRef& r, int precision, hipStream_t s) {
@@ -2560,8 +2534,16 @@ void launch_comm1(const double2* G, const double2* h, double2* out, int N, hipSt
     hipLaunchKernelGGL(k_comm1, dim3(blocks_for((int64_t)N * N, 256)), dim3(256), 0, s, G, h, out, N);
 }
 
-void launch_pair_diag(const double2* T, double2* out, int N, hipStream_t s) {
-    hipLaunchKernelGGL(k_pair_diag, dim3(blocks_for((int64_t)N * N, 256)), dim3(256), 0, s, T, out, N);
+void launch_pair_diag(const double2* T, double2* out, int N, hipStream_t s, int half) {
+    hipLaunchKernelGGL(k_pair_diag, dim3(blocks_for((int64_t)N * N, 256)), dim3(256), 0, s, T, out, N, half);
+}
+
+void launch_nat_to_half(const double2* nat, double2* half, int N, hipStream_t s) {
+    hipLaunchKernelGGL(k_nat_to_half, dim3((unsigned)(N * N)), dim3(256), 0, s, nat, half, N);
+}
+
+void launch_half_to_nat(const double2* half, double2* nat, int N, hipStream_t s) {
+    hipLaunchKernelGGL(k_half_to_nat, dim3((unsigned)(N * N)), dim3(256), 0, s, half, nat, N);
 }
 
 void launch_g1_step(const double2* comm, const double2* I, double2* K, double2* RK, double w, int N,
