@@ -1502,8 +1502,8 @@ __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
           // the tile (t0, t1) of R in the Hermitian half layout (its mirror
           // (t1, t0) = -(.)^dagger is not stored).  t0 != t1: straight from
           // the MFMA layout (256-byte complex128 or 128-byte complex64 rows);
-          // t0 = t1, and BF16x3, through LDS
-          if (PREC != G12_PREC_BF16X3 && t0 != t1) {
+          // t0 = t1 through LDS, which forms (R - R^dagger) / 2
+          if (t0 != t1) {
             const TileGeom ga = half_geom(N, t0 * N + t1);
             // two forms, with and without non-temporal stores, chosen per tile
             // (a run-time choice per store would be merged and lose the hint)
