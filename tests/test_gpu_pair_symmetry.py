@@ -14,7 +14,11 @@ created, keeps the general path, so both run on the same input here:
   16), adiabatic switching from G2 = 0 plus production steps, in every
   precision;
 * path selection: non-symmetric, non-Hermitian or G1-not-anti-Hermitian states
-  take the general path.
+  take the general path;
+* layout conversion: a change of path converts every trajectory's state
+  between the Hermitian half layout and the natural one; states read back
+  exactly, and the steps after each change match a run that stayed on the
+  path.
 The oracle comparisons of the same path are test_gpu_runs.py (config 1 and 2
 full runs, the N = 64 workload) and test_gpu_ensemble.py.
 """
@@ -166,3 +170,65 @@ def test_path_selection():
     hf = g.make_params(Ns=Ns, Nb=Nb, t1=1.0, t2=0.4, U=1.2, dt=0.02, time_steps=2, HF=True)
     with g.Propagator(hf) as prop:
         assert not prop.pair_symmetric()  # no G2 to propagate
+
+
+def test_layout_follows_path_changes():
+    """A run keeps its G2 states in the Hermitian half layout on the
+    pair-symmetric path and in the natural layout on the general one; a
+    g12_set_state that changes the path converts every other trajectory's
+    state.  Every state reads back exactly through both changes, and the steps
+    after each change agree with a run that stayed on that path."""
+    geoms = [(6, 1), (10, 2), (7, 3)]  # N = 6, 20, 21
+    params = [g.make_params(Ns=Ns, Nb=Nb, t1=1.0, t2=0.4, U=1.1 + 0.2 * k, dt=0.02, time_steps=3, EHM=True,
+                            V_law="coulomb") for k, (Ns, Nb) in enumerate(geoms)]
+    rng = np.random.default_rng(77)
+    phys = [(g.make_G0(q), physical_g2(rng, Ns, Nb), physical_g2(rng, Ns, Nb)) for q, (Ns, Nb) in zip(params, geoms)]
+    N1 = geoms[1][0] * geoms[1][1]
+    A = rand_c(rng, N1 * N1, N1 * N1, scale=1e-3)
+    herm_only = A + A.conj().T  # Hermitian, not pair-symmetric: the general path
+
+    def same(b, i, st):
+        G1, Gxx, Gxy = b.get_state(i)
+        return np.array_equal(G1, st[0]) and np.array_equal(Gxx, st[1]) and np.array_equal(Gxy, st[2])
+
+    def run(states, general):
+        with general_path(general):
+            b = g.Batch(params)
+            try:
+                for i, st in enumerate(states):
+                    b.set_state(i, *st)
+                sym = b.pair_symmetric()
+                rc, _ = b.rk4()
+                return sym, rc, [b.get_state(i) for i in range(len(params))]
+            finally:
+                b.close()
+
+    mixed = [phys[0], (phys[1][0], herm_only, phys[1][2]), phys[2]]
+    b = g.Batch(params)
+    try:
+        for i, st in enumerate(phys):
+            b.set_state(i, *st)
+        assert b.pair_symmetric()
+        assert all(same(b, i, phys[i]) for i in range(3))  # half layout, read back exactly
+        b.set_state(1, *mixed[1])  # the run turns general: trajectories 0 and 2 convert to natural
+        assert not b.pair_symmetric()
+        assert all(same(b, i, mixed[i]) for i in range(3))
+        b.set_state(1, *phys[1])   # and back: every state converts to the half layout
+        assert b.pair_symmetric()
+        assert all(same(b, i, phys[i]) for i in range(3))
+        b.set_state(1, *mixed[1])
+        rc, _ = b.rk4()
+        after_general = [b.get_state(i) for i in range(3)]
+    finally:
+        b.close()
+    sym_ref, rc_ref, ref = run(mixed, general=True)
+    assert rc == rc_ref == 0 and not sym_ref
+    for a, r in zip(after_general, ref):
+        for x, y in zip(a, r):
+            assert relerr(x, y) < 1e-13
+    sym_a, rc_a, st_a = run(phys, general=False)
+    sym_b, rc_b, st_b = run(phys, general=True)
+    assert sym_a and not sym_b and rc_a == rc_b == 0
+    for a, r in zip(st_a, st_b):
+        for x, y in zip(a, r):
+            assert relerr(x, y) < 1e-12
