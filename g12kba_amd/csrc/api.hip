@@ -559,6 +559,8 @@ int create_run(Run& r, const g12_params* p, int ntraj) {
     r.b.ncorr = (int)r.tab_list.size();
     r.b.tblocks = r.tab_prefix.back();
     r.b.sunits = r.sym_prefix.back();
+    r.b.sym_w8 = 1;
+    for (int i : r.tab_list) r.b.sym_w8 = r.b.sym_w8 && r.ds[i].N % 16 == 0;
     const char* e = std::getenv("G12KBA_GRAPHS");
     if (e && e[0] == '0') r.graphs = false;
     e = std::getenv("G12KBA_AUTOTUNE");

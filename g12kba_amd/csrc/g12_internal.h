@@ -129,6 +129,7 @@ struct BatchDev {
     // correlated trajectories (sym_update_blocks each)
     const int64_t* sprefix;  // device array [ncorr + 1]
     int64_t sunits;
+    int sym_w8;              // every correlated N is a multiple of 16 (k_sym_update_w8)
 };
 
 struct Bucket {

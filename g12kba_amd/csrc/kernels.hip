@@ -2009,7 +2009,7 @@ __global__ __launch_bounds__(64) void k_d_from_state(BTileArgs a) {
 // element with K = -i C, the partial traces D of the next stage input, and the
 // Hermitian images in the diagonal blocks.
 template <bool C>
-__global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef r) {
+__device__ __forceinline__ void sym_update_body(const BatchDev& b, const StageRef& r) {
     typedef typename std::conditional<C, fv2, dv2>::type LT;
     __shared__ LT la[SU_LA];
     __shared__ LT lb[SU_LB];
@@ -2069,14 +2069,15 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
         struct Item {
             uint32_t o;
             bool ok;
-            LT xv;           // this element of X
-            double2 sv, av;  // S (or Sc); mixed stage 2: K1; FP64 stage 2: Y1
-            double2 a2;      // mixed stage 2: K2; FP64 stage 2: Y2
+            LT xv;       // this element of X
+            double2 sv;  // S (or Sc)
+            LT av, a2;   // stage 2, as stored: K1, K2 (mixed) or Y1, Y2 (FP64)
         };
         auto gather = [&](uint32_t o, Item& w, bool with_state) {
             w.ok = live;
             w.o = o;
-            w.sv = w.av = w.a2 = make_double2(0.0, 0.0);
+            w.sv = make_double2(0.0, 0.0);
+            w.av = w.a2 = LT{0, 0};
             if (w.ok) {
                 if constexpr (C && nt)
                     w.xv = __builtin_nontemporal_load((const GLOBAL fv2*)((const GLOBAL char*)X + (o << 3)));
@@ -2086,13 +2087,14 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
                     w.xv = __builtin_nontemporal_load((const GLOBAL dv2*)((const GLOBAL char*)X + (o << 4)));
                 if (with_state) {
                     w.sv = s_c64 ? cld(Sc, o, true) : gld(S, o);
-                    if (C && ss.fold) {
-                        w.av = cld(A, o, true);        // K1
-                        w.a2 = cld(A, k2o + o, true);  // K2
-                    }
-                    if (fold64) {
-                        w.av = gld(Y, o);
-                        w.a2 = gld(A, o);
+                    if constexpr (C) {
+                        if (ss.fold) {
+                            w.av = *(const GLOBAL fv2*)((const GLOBAL char*)A + (o << 3));          // K1
+                            w.a2 = *(const GLOBAL fv2*)((const GLOBAL char*)A + ((k2o + o) << 3));  // K2
+                        }
+                    } else if (fold64) {
+                        w.av = *(const GLOBAL dv2*)((const GLOBAL char*)Y + (o << 4));  // Y1
+                        w.a2 = *(const GLOBAL dv2*)((const GLOBAL char*)A + (o << 4));  // Y2
                     }
                 }
             }
@@ -2122,13 +2124,14 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
         auto update = [&](const Item& w, uint32_t im, int bp, int bq, int z, double2 c) {
             const double2 K = cmul_mi(c);
             const double2 wK = cscale(K, ss.wk);
-            double2 av = w.av;  // at stage 2: K1 + 2 K2
+            const double2 v1 = tod(w.av), v2 = tod(w.a2);
+            double2 av = v1;  // at stage 2: K1 + 2 K2
             if (fold64) {
-                const double2 k1 = make_double2((w.av.x - w.sv.x) / ss.c, (w.av.y - w.sv.y) / ss.c);
-                const double2 k2 = make_double2((w.a2.x - w.sv.x) / ss.c, (w.a2.y - w.sv.y) / ss.c);
+                const double2 k1 = make_double2((v1.x - w.sv.x) / ss.c, (v1.y - w.sv.y) / ss.c);
+                const double2 k2 = make_double2((v2.x - w.sv.x) / ss.c, (v2.y - w.sv.y) / ss.c);
                 av = cadd(k1, cscale(k2, 2.0));
             } else if (C && ss.fold) {
-                av = cadd(w.av, cscale(w.a2, 2.0));
+                av = cadd(v1, cscale(v2, 2.0));
             }
             double2 next;
             if (ss.last) {
@@ -2163,6 +2166,21 @@ __global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef 
         body(std::true_type{});
     else
         body(std::false_type{});
+}
+
+// The pass as a kernel.  The FP64 form fits 64 VGPRs, two 1024-thread blocks
+// per CU.  The complex64 form needs 68; held to 64 (8 waves per SIMD, two
+// small spills) it ran the N = 64 mixed update 10% faster, but the 256-draw
+// ensemble (N = 6..60, most blocks on the cached-store form) 2.6% slower, so
+// both are built and a launch takes the bounded one when every trajectory's
+// rows fill whole lines (k_sym_update_w8).
+template <bool C>
+__global__ __launch_bounds__(SU_THREADS) void k_sym_update(BatchDev b, StageRef r) {
+    sym_update_body<C>(b, r);
+}
+__global__ __launch_bounds__(SU_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sym_update_w8(BatchDev b,
+                                                                                                     StageRef r) {
+    sym_update_body<true>(b, r);
 }
 
 // flag |= 1 unless T is pair-symmetric (and, with herm, Hermitian) within tol,
@@ -2440,7 +2458,9 @@ void launch_b_d_from_state(const BatchDev& b, const Bucket& k, int precision, hi
 
 void launch_b_sym_update(const BatchDev& b, const StageRef& r, int precision, hipStream_t s) {
     if (b.sunits <= 0) return;
-    if (precision != G12_PREC_FP64)
+    if (precision != G12_PREC_FP64 && b.sym_w8)
+        hipLaunchKernelGGL(k_sym_update_w8, dim3((unsigned)b.sunits), dim3(SU_THREADS), 0, s, b, r);
+    else if (precision != G12_PREC_FP64)
         hipLaunchKernelGGL(k_sym_update<true>, dim3((unsigned)b.sunits), dim3(SU_THREADS), 0, s, b, r);
     else
         hipLaunchKernelGGL(k_sym_update<false>, dim3((unsigned)b.sunits), dim3(SU_THREADS), 0, s, b, r);
