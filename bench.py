@@ -35,8 +35,8 @@ TRAFFIC_BYTES = {("fp64", "tile_fused"): 2.995e9, ("fp64", "tile_strided"): 1.09
                  ("fp32_3m", "tile_fused"): 2.455e9, ("fp32_3m", "tile_strided"): 0.556e9,
                  ("bf16x3", "tile_fused"): 2.328e9, ("bf16x3", "tile_strided"): 0.574e9,
                  ("fp64", "tile_products"): 0.566e9, ("fp64", "pair_update"): 1.121e9,
-                 ("fp32_3m", "tile_products"): 0.283e9, ("fp32_3m", "pair_update"): 0.867e9,
-                 ("bf16x3", "tile_products"): 0.283e9, ("bf16x3", "pair_update"): 0.868e9}
+                 ("fp32_3m", "tile_products"): 0.283e9, ("fp32_3m", "pair_update"): 0.869e9,
+                 ("bf16x3", "tile_products"): 0.283e9, ("bf16x3", "pair_update"): 0.869e9}
 
 TRAFFIC_SOURCE = {("fp64", False): "profiles/tile_kernels_n64_fp64_hbm_traffic.txt",
                   ("fp32_3m", False): "profiles/tile_kernels_n64_mixed_hbm_traffic.txt",
