@@ -355,11 +355,10 @@ __global__ void k_record(BatchDev b, StageRef r) {
     if (failed(d.fail)) return;
     const StageShared ss = stage_of(r);
     const int n2 = d.N * d.N;
-    double2* dst = d.traj ? d.traj + (int64_t)ss.slot * n2 : nullptr;
-    if (dst)
-        for (int e = threadIdx.x; e < n2; e += blockDim.x) dst[e] = d.S1[e];
-    __syncthreads();
-    if (threadIdx.x == 0) {
+    // one element per thread over the blocks of the trajectory's row of the grid
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d.traj && e < n2) gstore(d.traj + (int64_t)ss.slot * n2, e, gload(d.S1, e));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         const double dens = 2 * d.S1[0].y;
         if (fabs(dens) > 3 || isnan(dens)) *d.fail = ss.step + 1;
     }
@@ -2423,7 +2422,8 @@ void launch_b_g1_rhs(const BatchDev& b, const StageRef& r, hipStream_t s) {
 }
 
 void launch_b_record(const BatchDev& b, const StageRef& r, hipStream_t s) {
-    hipLaunchKernelGGL(k_record, dim3(1, b.ntraj), dim3(256), 0, s, b, r);
+    const unsigned nb = (unsigned)((b.maxN * b.maxN + 255) / 256);
+    hipLaunchKernelGGL(k_record, dim3(nb, b.ntraj), dim3(256), 0, s, b, r);
 }
 
 void launch_b_advance_counter(int* counter, hipStream_t s) {
