@@ -12,7 +12,8 @@ created, keeps the general path, so both run on the same input here:
   own error bounds of each other);
 * an uneven batch (N = 6 ... 60, every tile bucket, N not a multiple of 4 or
   16), adiabatic switching from G2 = 0 plus production steps, in every
-  precision;
+  precision, and a batch whose N are all multiples of 16 (N = 16, 32, 48: the
+  mixed runs' 8-wave update build with non-temporal stores in every block);
 * path selection: non-symmetric, non-Hermitian or G1-not-anti-Hermitian states
   take the general path;
 * layout conversion: a change of path converts every trajectory's state
@@ -134,6 +135,35 @@ def test_paths_agree_uneven_batch(prec):
         G1a, Gxxa, Gxya = st_a[i]
         G1b, Gxxb, Gxyb = st_b[i]
         assert np.abs(Gxxa).max() > 1e-8  # correlations have built up
+        if prec == "fp64":
+            assert relerr(G1a, G1b) < 1e-12 and relerr(tr_a[i], tr_b[i]) < 1e-12
+            assert relerr(Gxxa, Gxxb) < 1e-10 and relerr(Gxya, Gxyb) < 1e-10
+        else:
+            assert np.abs(np.diagonal(G1a - G1b).imag).max() < 1e-6
+
+
+def aligned_params(prec):
+    geoms = [(8, 2), (16, 2), (16, 3)]  # N = 16, 32, 48: every N a multiple of 16
+    return [g.make_params(Ns=Ns, Nb=Nb, t1=1.0, t2=0.35 + 0.05 * k, U=1.1 + 0.2 * k, dt=0.02, time_steps=5,
+                          quench_strength=1.0, Nq=max(1, Ns // 4), q_type="full", EHM=True, V_law="coulomb",
+                          quench_rate=2.0, quench_on_midpoint=0.05, quench_gate_time=0.0, AS_rate=0.5,
+                          AS_midpoint=10.0, AS_tail=3, precision=prec) for k, (Ns, Nb) in enumerate(geoms)]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_3m", "bf16x3"])
+def test_paths_agree_aligned_batch(prec):
+    """A batch whose N are all multiples of 16 (the mixed runs take the
+    update pass's 8-wave build and its non-temporal form for every block)
+    against the general path."""
+    params = aligned_params(prec)
+    sym_a, rc_a, tr_a, st_a = run_batch(params, general=False)
+    sym_b, rc_b, tr_b, st_b = run_batch(params, general=True)
+    print(f"aligned batch {prec}: paths {sym_a}/{sym_b}", flush=True)
+    assert sym_a and not sym_b and rc_a == rc_b
+    for i in range(len(params)):
+        G1a, Gxxa, Gxya = st_a[i]
+        G1b, Gxxb, Gxyb = st_b[i]
+        assert np.abs(Gxxa).max() > 1e-8
         if prec == "fp64":
             assert relerr(G1a, G1b) < 1e-12 and relerr(tr_a[i], tr_b[i]) < 1e-12
             assert relerr(Gxxa, Gxxb) < 1e-10 and relerr(Gxya, Gxyb) < 1e-10
