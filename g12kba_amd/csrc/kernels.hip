@@ -1107,7 +1107,8 @@ __device__ inline void load_view(View& v, const BTileArgs& a, const StageShared&
 // A strided tile (fixed p2, q2; rows p1, columns q1) then reads and writes
 // four lanes per line (32-byte runs at a 32-byte pitch) instead of one lane
 // per line, and a contiguous tile (fixed p1, q1; rows p2, columns q2) 32-byte
-// runs of four q2.
+// runs of four q2.  These layouts serve the general path; the pair-symmetric
+// path keeps every buffer in the Hermitian half layout (half_geom, hblock).
 // Which complex64 buffers use the blocked layout (mixed-precision runs).
 #ifndef G12_BLOCKED_Y
 #define G12_BLOCKED_Y 1
@@ -1310,8 +1311,9 @@ __device__ __forceinline__ double2 lane_value(double2 v, int src) {
 // (h X - X h)^dagger = X^dagger h - h X^dagger with the input tile
 // (t1, t0) = (tile (t0, t1))^dagger, Qp[t1][t0][.] = conj Qp[t0][t1][.] and
 // F[t1][t0][.] = -conj F[t0][t1][.] for G1 anti-Hermitian and D Hermitian.
-// Only the tiles t0 <= t1 are computed; the result goes through LDS and
-// leaves as rows of both tiles (t0 = t1: (R - R^dagger) / 2).
+// Only the tiles t0 <= t1 are computed and stored, in the Hermitian half
+// layout (half_geom): off-diagonal tiles straight from the MFMA registers,
+// diagonal tiles through LDS as (R - R^dagger) / 2.
 template <int NP, int NW, int PREC, bool FUSED, bool FULL, bool SYM = false>
 __global__ __launch_bounds__(64 * NW) void k_tile(BTileArgs a) {
     static_assert(!(FUSED && SYM), "the pair-symmetric products run unfused");
