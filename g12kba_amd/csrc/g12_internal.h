@@ -67,7 +67,9 @@ struct StageRef {
 
 // Device buffers of one trajectory.  G2 tensors are stored in the natural
 // layout T[p1][p2][q1][q2] (pair index p1*N + p2, one-particle index
-// p = site + orb*Ns); index 0 is xxxx, 1 is xyxy.
+// p = site + orb*Ns) on the general path, and in the Hermitian half layout
+// (sym_units.h, hblock) on the pair-symmetric path; index 0 is xxxx, 1 is
+// xyxy.
 struct DevState {
     int Ns, Nb, N;
     int64_t n2, n3, n4;
@@ -77,7 +79,7 @@ struct DevState {
     // state, next stage input, RK accumulator, strided-pass result; in
     // mixed-precision runs Y2 and R2 hold complex64 elements
     double2 *S2[2], *Y2[2], *A2[2], *R2[2];
-    // mixed-precision runs: the complex64 copy of S (blocked like Y2), the
+    // mixed-precision runs: the complex64 copy of S (laid out like Y2), the
     // stage-0 input, written with every new state; null in FP64 runs
     double2* Sc[2];
     // pair-symmetric update (k_sym_update): the unit table of this N
